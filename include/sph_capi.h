@@ -1,0 +1,208 @@
+/*
+ * sph_capi.h -- C ABI of the MI355X (gfx950) SPH + FMM step core, exported by
+ * libsphhip.so.
+ *
+ * This is the drop-in boundary for the reference's `class Compute`
+ * (SPH/Compute/Compute.hpp:40-148).  The reference encodes each pass into a
+ * Metal command buffer; here each pass is enqueued on the context's HIP stream
+ * and nothing synchronises until sph_sync().  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Return codes: 0 = OK, negative = SPH_E_* error, positive = hipError_t.
+ *    sph_last_error() gives the message of the last failure on a context.
+ *  - Particle arrays are in original-ID order (the order of the initial
+ *    conditions), as in the reference.  3-vectors use a 16-byte stride
+ *    (x, y, z, unused), matching simd::float3 (Compute.cpp:95-97).
+ *  - A context owns all device memory and one compute stream plus one copy
+ *    stream.  Calls on one context must come from one thread at a time.
+ *    Host buffers stay owned by the caller.
+ */
+#ifndef SPH_CAPI_H
+#define SPH_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+#define SPH_OK 0
+#define SPH_E_ARG (-1)       /* invalid argument / size                     */
+#define SPH_E_STATE (-2)     /* call not valid in the current state         */
+#define SPH_E_NOTREE (-3)    /* gravity requested before a tree was uploaded */
+#define SPH_E_CAPACITY (-4)  /* a fixed-size resource would overflow        */
+#define SPH_E_NODEVICE (-5)  /* no HIP device visible                       */
+
+/* ------------------------------------------------------------------ params */
+/* Reference quirks that have a corrected form (bit flags for sph_params.fixes).
+ * Zero = the reference as written. */
+#define SPH_FIX_M2M_ZZ        (1u << 0) /* M2M.metal:29 uses expansion[Z] in the ZZ shift */
+#define SPH_FIX_SYNC_RANGE    (1u << 1) /* M2M.metal:150 childEnd from the parent's start  */
+#define SPH_FIX_SOFTENING     (1u << 2) /* kernels.h:93,97 integer-literal divisions       */
+#define SPH_FIX_DENSITY_CACHE (1u << 3) /* density.metal:173-199 density not reset on rescan */
+#define SPH_FIX_ALL           (0xFu)
+
+/* Particle activity rule used by the gravity up pass (P2M.metal:31). */
+#define SPH_ACTIVE_AS_WRITTEN 0 /* active iff globalTime <= nextActiveTime (every particle, every step) */
+#define SPH_ACTIVE_README     1 /* active iff nextActiveTime <= globalTime (README.md:62)            */
+
+typedef struct sph_params {
+    /* Hydro (Parameters.h:11-17) */
+    float viscosity_alpha;      /* VISCOSITY_ALPHA */
+    float density_eta;          /* DENSITY_ETA: NR successive-iterate tolerance */
+    float max_smoothing_length; /* MAX_SMOOTHING_LENGTH */
+    int32_t max_nr_iterations;  /* MAX_DENSITY_NR_ITTERATIONS */
+    int32_t n_neighbours_estim; /* N_NEIGHBOURS_ESTIM: density neighbour-cache size */
+    float resolution_eta;       /* RESOLUTION_ETA */
+    /* EOS axes (Parameters.h:21-24) */
+    float eos_min_rho, eos_max_rho, eos_min_u, eos_max_u;
+    /* Time stepping (Parameters.h:27-33) */
+    float cfl, dt0, gamma;
+    int32_t r_max;              /* R_MAX */
+    int32_t activity_rule;      /* SPH_ACTIVE_* */
+    int32_t max_rungs;          /* >0: clamp each particle's tick step to dt_min << (max_rungs-1); 0 = no cap */
+    /* Gravity (Parameters.h:36-43, poles.h:16) */
+    float gravity_eta;          /* GRAVITY_ETA (MAC tolerance) */
+    float gravity_g;            /* G in code units */
+    float gravity_softening;    /* GRAVITY_SMOOTHING_LENGTH */
+    int32_t gravity_max_recursion;   /* GRAVITY_MAX_RECURSION (1..4) */
+    int32_t max_unchecked_pointers;  /* MAX_UNCHECKED_POINTERS: per-node list length */
+    int32_t max_children_in_leaf;    /* MAX_CHILDREN_IN_LEAF */
+    int32_t max_tree_depth;          /* MAX_TREE_DEPTH */
+    /* Cells (Parameters.h:47-48) */
+    float cell_width;           /* CELL_WIDTH */
+    int32_t cell_power;         /* CELL_POWER: 2^(3*cell_power) cells, 1..8 */
+    /* Corrections and forms */
+    uint32_t fixes;             /* SPH_FIX_* bit set */
+    int32_t reserved[8];
+} sph_params;
+
+/* One EOS table: res x res float2 (pressure, sound speed) in code units.
+ * Row index = density sample, column index = internal-energy sample, sample i
+ * at exponent i/res between the axis limits (ANEOS.cpp:123-136). */
+typedef struct sph_eos_table {
+    int32_t material_id;        /* 400 = forsterite slot, anything else = Fe85Si15 slot (density.metal:45-50) */
+    int32_t res;
+    const float* pc;            /* 2*res*res floats, row-major [rho][u] */
+} sph_eos_table;
+
+/* Initial conditions in the reference's DataStruct layout (hdfHandler.hpp:15-26). */
+typedef struct sph_ic_view {
+    int32_t n;
+    const float* pos4;          /* n x 4 floats */
+    const float* vel4;          /* n x 4 floats */
+    const float* rho;
+    const float* u;
+    const float* m;
+    const int32_t* mat;
+    const float* p;
+    const float* h;
+} sph_ic_view;
+
+typedef struct sph_ctx sph_ctx;
+
+/* Fills the reference's Parameters.h defaults. */
+void sph_params_default(sph_params* out);
+
+/* Compute::Compute + buildBuffers + loadInitialConditions (Compute.cpp:22-84,
+ * 92-132): allocates every device buffer, uploads the initial conditions and
+ * the two EOS tables, sets active = true, nextActiveTime = 0, globalTime = 0. */
+int sph_create(const sph_params* params, const sph_ic_view* ic,
+               const sph_eos_table* tables, int ntables, sph_ctx** out);
+void sph_destroy(sph_ctx* ctx);
+const char* sph_last_error(const sph_ctx* ctx);
+int sph_num_particles(const sph_ctx* ctx);
+
+/* ------------------------------------------------------------ octree handoff */
+/* Pinned host memory for the octree thread's staging buffers. */
+void* sph_host_alloc(size_t bytes);
+void sph_host_free(void* p);
+
+/* Copies the current positions into a device snapshot buffer on the compute
+ * stream (ordered after every pass enqueued so far), then into `pinned_dst4`
+ * (n x 4 floats) on the copy stream.  Replaces the CPU read of the shared
+ * position buffer in Compute::updateOctreeData (Compute.cpp:511). */
+int sph_snapshot_positions(sph_ctx* ctx, float* pinned_dst4);
+/* Blocks the calling thread until the last snapshot has landed in host memory. */
+int sph_wait_snapshot(sph_ctx* ctx);
+
+/* Compute::updateOctreeBuffer (Compute.cpp:456-508): takes a tree in the
+ * reference's flat layout (octree.cpp:14-141) -- `octree` = octreeData,
+ * `level_nodes` = the concatenated treeLevels lists (tree offsets),
+ * `level_counts[l]` = size of treeLevels[l], `node_count` = nodeValues.
+ * The upload runs on the copy stream; passes enqueued after this call use the
+ * new tree.  When `octree`/`level_nodes` are pinned the call does not block. */
+int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints,
+                    const int32_t* level_nodes, const int32_t* level_counts,
+                    int n_levels, int32_t node_count);
+/* 1 once the last uploaded tree's host buffers may be reused, else 0. */
+int sph_tree_upload_done(sph_ctx* ctx);
+
+/* ----------------------------------------------------------------- passes */
+int sph_sort(sph_ctx* ctx);          /* Compute::sort               Compute.cpp:265-326 */
+int sph_gravity(sph_ctx* ctx);       /* Compute::gravitationalPass  Compute.cpp:328-377 */
+int sph_density(sph_ctx* ctx);       /* Compute::densityPass        Compute.cpp:179-208 */
+int sph_acceleration(sph_ctx* ctx);  /* Compute::accelerationPass   Compute.cpp:210-240 */
+int sph_step(sph_ctx* ctx);          /* Compute::stepPass           Compute.cpp:242-263 */
+/* Headless::step's commit + waitUntilCompleted (Headless.cpp:53-54). */
+int sph_sync(sph_ctx* ctx);
+
+/* ------------------------------------------------------------- state access */
+enum sph_field {
+    SPH_F_POS = 0,        /* float4 x n */
+    SPH_F_VEL,            /* float4 x n */
+    SPH_F_ACC,            /* float4 x n */
+    SPH_F_RHO,            /* float x n  */
+    SPH_F_U,
+    SPH_F_MASS,
+    SPH_F_PRESSURE,
+    SPH_F_H,
+    SPH_F_MAT,            /* int32 x n  */
+    SPH_F_GRADTERM,
+    SPH_F_SOUNDSPEED,
+    SPH_F_DU,
+    SPH_F_BALSARA,
+    SPH_F_DHDT,
+    SPH_F_ACTIVE,         /* uint8 x n  */
+    SPH_F_NEXT_ACTIVE,    /* int32 x n  */
+    SPH_F_GRAVNORM,       /* float x n  */
+    SPH_F_CELL_ARRAY,     /* uint32 x 2n: sorted (cell key, particle index) pairs */
+    SPH_F_CELL_START,     /* uint32 x 2^(3*cell_power): first sorted slot, UINT_MAX if empty */
+    SPH_F_CELL_END,       /* uint32 x 2^(3*cell_power): last sorted slot (inclusive), 0 if empty */
+    SPH_F_GLOBAL_TIME,    /* int32 x 1 (ticks of DT_MIN1) */
+    SPH_F_DT,             /* int32 x 1 (global tick step of the last acceleration pass) */
+    SPH_F_NR_ITERS,       /* int32 x n: NR iterations of the last density pass (diagnostic) */
+    SPH_F_MULTIPOLES,     /* float x 24 x node_count, see below */
+    SPH_F_LOCALS,         /* float x 16 x node_count, see below */
+    SPH_F_COUNT
+};
+/* Multipole record (24 floats, poles.h:46-55 reordered):
+ *   0-2 pos, 3 size, 4-6 min, 7 minGrav, 8-10 max, 11 active (0/1),
+ *   12 M, 13-15 X Y Z (always 0: never consumed at P = 2),
+ *   16-21 XX XY XZ YY YZ ZZ, 22 power[2], 23 power[1] (= 0); power[0] = M.
+ * Local record (16 floats, poles.h:58-61): 0-2 pos, 3 unused,
+ *   4-13 expansion M X Y Z XX XY XZ YY YZ ZZ, 14-15 unused. */
+
+int64_t sph_field_bytes(const sph_ctx* ctx, int field);
+/* Synchronous copies (they drain the compute stream first). */
+int sph_download(sph_ctx* ctx, int field, void* dst);
+int sph_upload(sph_ctx* ctx, int field, const void* src);
+
+/* ------------------------------------------------------------------ timing */
+enum sph_pass {
+    SPH_PASS_SORT = 0, SPH_PASS_GRAVITY_UP, SPH_PASS_GRAVITY_DOWN, SPH_PASS_DENSITY,
+    SPH_PASS_ACCELERATION, SPH_PASS_STEP, SPH_PASS_COUNT
+};
+/* Enables HIP-event timing of every pass (0 = off, the default). */
+int sph_set_timing(sph_ctx* ctx, int enabled);
+/* Accumulated milliseconds per pass since the last reset (syncs the stream).
+ * `ms` has SPH_PASS_COUNT entries; `calls` (optional) counts the passes. */
+int sph_pass_times(sph_ctx* ctx, double* ms, int64_t* calls);
+int sph_reset_pass_times(sph_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH_CAPI_H */

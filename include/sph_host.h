@@ -1,0 +1,109 @@
+/*
+ * sph_host.h -- C API of the host layer (libsphhost.so): synthetic inputs,
+ * the CPU octree builder, and the headless driver built on sph_capi.h.
+ *
+ * The C++ classes behind it keep the reference's names (Compute, Headless,
+ * buildOctree); this flat API exists so that Python tests and bench.py can
+ * drive them through ctypes.
+ */
+#ifndef SPH_HOST_H
+#define SPH_HOST_H
+
+#include <stdint.h>
+#include "sph_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- EOS tables */
+/* Tillotson parameter set (SI units) for one material slot. */
+typedef struct sph_tillotson {
+    double rho0, A, B, E0, Eiv, Ecv, a, b, alpha, beta;
+} sph_tillotson;
+
+/* Published parameter sets recorded with the table: granite (used for the
+ * forsterite slot, material 400) and iron (Fe85Si15 slot, material 402). */
+void sph_tillotson_granite(sph_tillotson* out);
+void sph_tillotson_iron(sph_tillotson* out);
+
+/* Fills `pc_out` (2*res*res floats, row-major [rho][u]) with (P, c) in code
+ * units on the reference's log-spaced axes (ANEOS.cpp:123-136): sample i at
+ * min * (max/min)^(i/res).  `seed` is recorded for reproducibility; the table
+ * is deterministic and adds no noise. */
+int sph_make_eos_table(const sph_params* prm, const sph_tillotson* t, int res, uint64_t seed, float* pc_out);
+
+/* ------------------------------------------------------ initial conditions */
+typedef struct sph_impact_config {
+    int32_t n_total;
+    float target_fraction;        /* share of particles in the target (0.89) */
+    float target_radius;          /* code units (Mm) */
+    float target_mass;            /* code units (1e24 kg) */
+    float impactor_radius;
+    float impactor_mass;
+    float core_mass_fraction;     /* 0.3 */
+    float core_radius_fraction;   /* 0.55 */
+    float u0;                     /* initial specific internal energy, code units (1e12 J/kg) */
+    float separation_factor;      /* centre distance / (R_t + R_i) */
+    float impact_parameter;       /* b / (R_t + R_i) */
+    float closing_speed;          /* code units (Mm/s) */
+    uint64_t seed;
+} sph_impact_config;
+
+/* Oblique 45-degree impact of the configs (R_t 6.4, M_t 5.3, R_i 3.6, M_i 0.8,
+ * u0 2e-6, separation 1.5, b = sin 45deg, v = 0.009). */
+void sph_impact_config_default(sph_impact_config* out, int32_t n_total, uint64_t seed);
+/* Head-on variant (b = 0.3, v = 0.012, u0 = 5e-6). */
+void sph_impact_config_headon(sph_impact_config* out, int32_t n_total, uint64_t seed);
+
+/* Writes the two-body initial conditions in DataStruct layout
+ * (hdfHandler.hpp:15-26): arrays of length n_total (pos4/vel4: 4*n_total).
+ * Pressure comes from the EOS tables (pc_400 for material 400, pc_402 for 402). */
+int sph_make_impact(const sph_params* prm, const sph_impact_config* cfg, const float* pc_400,
+                    const float* pc_402, int res, float* pos4, float* vel4, float* rho, float* u,
+                    float* m, int32_t* mat, float* p, float* h);
+
+/* ---------------------------------------------------------------- octree */
+/* buildOctree (octree.cpp:111-141) output: octreeData, treeLevels flattened,
+ * nodeValues.  Library-owned; free with sph_octree_free. */
+typedef struct sph_octree {
+    int32_t* octree;
+    int64_t n_ints;
+    int32_t* level_nodes;
+    int32_t* level_counts;
+    int32_t n_levels;
+    int32_t node_count;
+} sph_octree;
+/* `nthreads` <= 1 builds serially; otherwise subtrees are built in parallel.
+ * The result is identical for every thread count. */
+int sph_octree_build(const float* pos4, int32_t n, int32_t max_leaf, int32_t max_depth,
+                     int32_t nthreads, sph_octree* out);
+void sph_octree_free(sph_octree* t);
+
+/* -------------------------------------------------------------- headless */
+#define SPH_TREE_ASYNC 0   /* rebuild on a CPU thread, swap when ready (Compute.cpp:456-508) */
+#define SPH_TREE_EVERY 1   /* rebuild from the current positions before every step */
+#define SPH_TREE_FIXED 2   /* keep the first tree */
+
+typedef struct sph_headless sph_headless;
+
+/* Headless::Headless (Headless.cpp:19-23) with synthetic inputs instead of
+ * the HDF5 file: builds the EOS tables and the impact, creates the context
+ * and the initial tree. */
+int sph_headless_create(const sph_params* prm, const sph_impact_config* cfg, int eos_res,
+                        int tree_mode, int tree_threads, sph_headless** out);
+void sph_headless_destroy(sph_headless* h);
+/* Headless::step (Headless.cpp:37-55): updateOctreeBuffer, then sort,
+ * gravity, density, acceleration, step, then wait. */
+int sph_headless_step(sph_headless* h);
+/* Same sequence without the final wait (the caller syncs). */
+int sph_headless_enqueue_step(sph_headless* h);
+sph_ctx* sph_headless_ctx(sph_headless* h);
+/* Tree statistics: number of trees swapped in so far and the last build ms. */
+int sph_headless_tree_stats(sph_headless* h, int64_t* trees_swapped, double* last_build_ms);
+const char* sph_headless_last_error(const sph_headless* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+/*
+ * oracle.h -- CPU restatement of the reference's per-step SPH + FMM core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the CPU
+ * baseline: only the tests, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.  The product (libsphhip.so and the host library)
+ * never links or calls it.
+ *
+ * Every function restates one reference kernel or helper in plain C++ with
+ * FP32 arithmetic (all literals float, as in Metal Shading Language, which has
+ * no double), OpenMP over the reference's thread index where the reference
+ * runs threads independently.  Each function cites the reference file:line it
+ * follows.  Built with -ffp-contract=off so that no multiply-add is fused.
+ *
+ * Pins (which test checks which function) are listed in oracle/README.md.
+ */
+#ifndef SPH_ORACLE_H
+#define SPH_ORACLE_H
+
+#include <stdint.h>
+#include "../include/sph_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device buffers of Compute (Compute.cpp:92-132) as host arrays.  3-vectors use
+ * a 16-byte stride.  Arrays are owned by the caller. */
+typedef struct orc_state {
+    int32_t n;
+    float* pos;      /* 4n */
+    float* vel;      /* 4n */
+    float* acc;      /* 4n */
+    float* rho;
+    float* u;
+    float* m;
+    float* pressure;
+    float* h;
+    int32_t* mat;
+    float* gradterm;
+    float* soundspeed;
+    float* du;
+    float* balsara;
+    float* dhdt;
+    uint8_t* active;
+    int32_t* next_active;
+    float* gravnorm;
+    uint32_t* cell_array;   /* 2n, sorted (key, index) */
+    uint32_t* cell_start;   /* 2^(3*cell_power) */
+    uint32_t* cell_end;     /* 2^(3*cell_power) */
+    int32_t* global_time;   /* 1 */
+    int32_t* dt;            /* 1 */
+    int32_t* nr_iters;      /* n, optional (may be NULL) */
+    const float* eos_forsterite; /* 2*res*res */
+    const float* eos_fe;         /* 2*res*res */
+    int32_t eos_res;
+} orc_state;
+
+/* ---- utils/morton.h, sort.metal ---- */
+uint32_t orc_morton3d(uint32_t x, uint32_t y, uint32_t z);
+void orc_hash(const sph_params* prm, const orc_state* s);
+/* hist/scan/sum/sort x SORTING_ITTERATIONS, ping-pong as Compute.cpp:276-312. */
+void orc_radix_sort(const sph_params* prm, const orc_state* s);
+void orc_find_cell_positions(const sph_params* prm, const orc_state* s);
+void orc_sort(const sph_params* prm, const orc_state* s);
+
+/* ---- utils/kernels.h (scalars exposed for the kernel tests) ---- */
+float orc_W(float r, float h1);
+void orc_gradW(const float* x_ij3, float r, float r1, float h1, float* out3);
+float orc_dW_dh(float r, float h1);
+float orc_dphi_dr(const float* x_ij3, float h, int fixed);
+void orc_cell_range(const sph_params* prm, const float* pos3, float h, int32_t* out6);
+void orc_eos_sample(const sph_params* prm, const float* table, int32_t res,
+                    float u, float rho, float* out2);
+
+/* ---- density.metal, accelerations.metal, step.metal ---- */
+void orc_density(const sph_params* prm, const orc_state* s);
+void orc_acceleration(const sph_params* prm, const orc_state* s);
+void orc_step(const sph_params* prm, const orc_state* s);
+
+/* ---- octree/octree.cpp ---- */
+typedef struct orc_tree {
+    int32_t* octree;        /* octreeData */
+    int64_t n_ints;
+    int32_t* level_nodes;   /* concatenated treeLevels */
+    int32_t* level_counts;  /* per level */
+    int32_t n_levels;
+    int32_t node_count;     /* nodeValues */
+} orc_tree;
+/* Builds into library-owned storage; free with orc_free_tree. */
+int orc_build_octree(const float* pos4, int32_t n, int32_t max_leaf, int32_t max_depth, orc_tree* out);
+void orc_free_tree(orc_tree* t);
+
+/* ---- gravity/** ---- */
+/* Records use the sph_capi.h SPH_F_MULTIPOLES / SPH_F_LOCALS layouts. */
+typedef struct orc_gravity_buffers {
+    float* multipoles;       /* 24 x node_count */
+    float* locals;           /* 16 x node_count */
+    uint32_t* parent_index;  /* node_count */
+    int32_t* list_a;         /* max_level_nodes * max_unchecked_pointers */
+    int32_t* list_b;
+    int64_t list_len;        /* entries per list buffer */
+} orc_gravity_buffers;
+void orc_gravity_up(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g);
+void orc_gravity_down(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g);
+
+/* FP64 direct sum over all n sources at the sampled targets: acceleration
+ * (G included) and potential.  `softening` > 0 uses the cubic-spline softened
+ * force (the corrected dphi_dr form) with that length; 0 = pure Newton. */
+void orc_direct_sum_fp64(const float* pos4, const float* m, int32_t n,
+                         const int32_t* targets, int32_t n_targets, double G,
+                         double softening, double* acc3_out, double* phi_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,416 @@
+"""MI355X (gfx950) SPH + FMM step core -- ctypes bindings.
+
+The compute path is native: ``lib/libsphhip.so`` (hand-written HIP kernels
+behind the C ABI of ``include/sph_capi.h``) and ``lib/libsphhost.so`` (the
+C++ host layer of ``include/sph_host.h``: synthetic inputs, the CPU octree
+builder and the ``Compute``/``Headless`` driver).  This module only moves
+numpy arrays across that boundary.  Loading fails loudly when the libraries
+have not been built; there is no Python fallback for any pass.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "libsphhip.so")
+HOST_LIB = os.path.join(LIB_DIR, "libsphhost.so")
+
+# ---------------------------------------------------------------- constants
+SPH_OK = 0
+SPH_FIX_M2M_ZZ = 1 << 0
+SPH_FIX_SYNC_RANGE = 1 << 1
+SPH_FIX_SOFTENING = 1 << 2
+SPH_FIX_DENSITY_CACHE = 1 << 3
+SPH_FIX_ALL = 0xF
+SPH_ACTIVE_AS_WRITTEN = 0
+SPH_ACTIVE_README = 1
+TREE_ASYNC, TREE_EVERY, TREE_FIXED = 0, 1, 2
+
+FIELDS = [
+    "pos", "vel", "acc", "rho", "u", "mass", "pressure", "h", "mat", "gradterm",
+    "soundspeed", "du", "balsara", "dhdt", "active", "next_active", "gravnorm",
+    "cell_array", "cell_start", "cell_end", "global_time", "dt", "nr_iters",
+    "multipoles", "locals",
+]
+FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
+_F4 = {"pos", "vel", "acc"}
+_I32 = {"mat", "next_active", "global_time", "dt", "nr_iters"}
+_U32 = {"cell_array", "cell_start", "cell_end"}
+PASSES = ["sort", "gravity_up", "gravity_down", "density", "acceleration", "step"]
+
+
+class SphParams(ctypes.Structure):
+    _fields_ = [
+        ("viscosity_alpha", ctypes.c_float), ("density_eta", ctypes.c_float),
+        ("max_smoothing_length", ctypes.c_float), ("max_nr_iterations", ctypes.c_int32),
+        ("n_neighbours_estim", ctypes.c_int32), ("resolution_eta", ctypes.c_float),
+        ("eos_min_rho", ctypes.c_float), ("eos_max_rho", ctypes.c_float),
+        ("eos_min_u", ctypes.c_float), ("eos_max_u", ctypes.c_float),
+        ("cfl", ctypes.c_float), ("dt0", ctypes.c_float), ("gamma", ctypes.c_float),
+        ("r_max", ctypes.c_int32), ("activity_rule", ctypes.c_int32), ("max_rungs", ctypes.c_int32),
+        ("gravity_eta", ctypes.c_float), ("gravity_g", ctypes.c_float), ("gravity_softening", ctypes.c_float),
+        ("gravity_max_recursion", ctypes.c_int32), ("max_unchecked_pointers", ctypes.c_int32),
+        ("max_children_in_leaf", ctypes.c_int32), ("max_tree_depth", ctypes.c_int32),
+        ("cell_width", ctypes.c_float), ("cell_power", ctypes.c_int32),
+        ("fixes", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class SphEosTable(ctypes.Structure):
+    _fields_ = [("material_id", ctypes.c_int32), ("res", ctypes.c_int32), ("pc", ctypes.POINTER(ctypes.c_float))]
+
+
+class SphIcView(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32), ("pos4", ctypes.POINTER(ctypes.c_float)), ("vel4", ctypes.POINTER(ctypes.c_float)),
+        ("rho", ctypes.POINTER(ctypes.c_float)), ("u", ctypes.POINTER(ctypes.c_float)),
+        ("m", ctypes.POINTER(ctypes.c_float)), ("mat", ctypes.POINTER(ctypes.c_int32)),
+        ("p", ctypes.POINTER(ctypes.c_float)), ("h", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+class SphTillotson(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("rho0", "A", "B", "E0", "Eiv", "Ecv", "a", "b", "alpha", "beta")]
+
+
+class SphImpactConfig(ctypes.Structure):
+    _fields_ = [
+        ("n_total", ctypes.c_int32), ("target_fraction", ctypes.c_float), ("target_radius", ctypes.c_float),
+        ("target_mass", ctypes.c_float), ("impactor_radius", ctypes.c_float), ("impactor_mass", ctypes.c_float),
+        ("core_mass_fraction", ctypes.c_float), ("core_radius_fraction", ctypes.c_float), ("u0", ctypes.c_float),
+        ("separation_factor", ctypes.c_float), ("impact_parameter", ctypes.c_float),
+        ("closing_speed", ctypes.c_float), ("seed", ctypes.c_uint64),
+    ]
+
+
+class SphOctree(ctypes.Structure):
+    _fields_ = [
+        ("octree", ctypes.POINTER(ctypes.c_int32)), ("n_ints", ctypes.c_int64),
+        ("level_nodes", ctypes.POINTER(ctypes.c_int32)), ("level_counts", ctypes.POINTER(ctypes.c_int32)),
+        ("n_levels", ctypes.c_int32), ("node_count", ctypes.c_int32),
+    ]
+
+
+# C-ABI symbols of include/sph_capi.h and include/sph_host.h
+CAPI_SYMBOLS = [
+    "sph_params_default", "sph_create", "sph_destroy", "sph_last_error", "sph_num_particles",
+    "sph_host_alloc", "sph_host_free", "sph_snapshot_positions", "sph_wait_snapshot", "sph_upload_tree",
+    "sph_tree_upload_done", "sph_sort", "sph_gravity", "sph_density", "sph_acceleration", "sph_step",
+    "sph_sync", "sph_field_bytes", "sph_download", "sph_upload", "sph_set_timing", "sph_pass_times",
+    "sph_reset_pass_times",
+]
+HOST_SYMBOLS = [
+    "sph_tillotson_granite", "sph_tillotson_iron", "sph_make_eos_table", "sph_impact_config_default",
+    "sph_impact_config_headon", "sph_make_impact", "sph_octree_build", "sph_octree_free",
+    "sph_headless_create", "sph_headless_destroy", "sph_headless_step", "sph_headless_enqueue_step",
+    "sph_headless_ctx", "sph_headless_tree_stats", "sph_headless_last_error",
+]
+
+_hip: Optional[ctypes.CDLL] = None
+_host: Optional[ctypes.CDLL] = None
+
+
+class SphError(RuntimeError):
+    pass
+
+
+def _fp(a: np.ndarray, t=ctypes.c_float):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def libs():
+    """Loads libsphhip.so and libsphhost.so (built by __graft_entry__.build())."""
+    global _hip, _host
+    if _hip is None:
+        for p in (HIP_LIB, HOST_LIB):
+            if not os.path.exists(p):
+                raise SphError(f"{p} is missing: build the HIP extension first (__graft_entry__.build())")
+        _hip = ctypes.CDLL(HIP_LIB, mode=ctypes.RTLD_GLOBAL)
+        _host = ctypes.CDLL(HOST_LIB, mode=ctypes.RTLD_GLOBAL)
+        h, o = _hip, _host
+        h.sph_params_default.argtypes = [ctypes.POINTER(SphParams)]
+        h.sph_params_default.restype = None
+        h.sph_create.argtypes = [ctypes.POINTER(SphParams), ctypes.POINTER(SphIcView), ctypes.POINTER(SphEosTable),
+                                 ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        h.sph_destroy.argtypes = [ctypes.c_void_p]
+        h.sph_destroy.restype = None
+        h.sph_last_error.argtypes = [ctypes.c_void_p]
+        h.sph_last_error.restype = ctypes.c_char_p
+        h.sph_num_particles.argtypes = [ctypes.c_void_p]
+        for name in ("sph_sort", "sph_gravity", "sph_density", "sph_acceleration", "sph_step", "sph_sync",
+                     "sph_wait_snapshot", "sph_tree_upload_done", "sph_reset_pass_times"):
+            getattr(h, name).argtypes = [ctypes.c_void_p]
+        h.sph_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        h.sph_pass_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+        h.sph_field_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        h.sph_field_bytes.restype = ctypes.c_int64
+        h.sph_download.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        h.sph_upload.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        h.sph_upload_tree.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int,
+                                      ctypes.c_int32]
+        h.sph_snapshot_positions.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        h.sph_host_alloc.argtypes = [ctypes.c_size_t]
+        h.sph_host_alloc.restype = ctypes.c_void_p
+        h.sph_host_free.argtypes = [ctypes.c_void_p]
+        h.sph_host_free.restype = None
+        o.sph_tillotson_granite.argtypes = [ctypes.POINTER(SphTillotson)]
+        o.sph_tillotson_iron.argtypes = [ctypes.POINTER(SphTillotson)]
+        o.sph_make_eos_table.argtypes = [ctypes.POINTER(SphParams), ctypes.POINTER(SphTillotson), ctypes.c_int,
+                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]
+        o.sph_impact_config_default.argtypes = [ctypes.POINTER(SphImpactConfig), ctypes.c_int32, ctypes.c_uint64]
+        o.sph_impact_config_headon.argtypes = [ctypes.POINTER(SphImpactConfig), ctypes.c_int32, ctypes.c_uint64]
+        o.sph_make_impact.argtypes = [ctypes.POINTER(SphParams), ctypes.POINTER(SphImpactConfig),
+                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int] + \
+            [ctypes.POINTER(ctypes.c_float)] * 5 + [ctypes.POINTER(ctypes.c_int32)] + \
+            [ctypes.POINTER(ctypes.c_float)] * 2
+        o.sph_octree_build.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.POINTER(SphOctree)]
+        o.sph_octree_free.argtypes = [ctypes.POINTER(SphOctree)]
+        o.sph_octree_free.restype = None
+        o.sph_headless_create.argtypes = [ctypes.POINTER(SphParams), ctypes.POINTER(SphImpactConfig), ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        o.sph_headless_destroy.argtypes = [ctypes.c_void_p]
+        o.sph_headless_destroy.restype = None
+        o.sph_headless_step.argtypes = [ctypes.c_void_p]
+        o.sph_headless_enqueue_step.argtypes = [ctypes.c_void_p]
+        o.sph_headless_ctx.argtypes = [ctypes.c_void_p]
+        o.sph_headless_ctx.restype = ctypes.c_void_p
+        o.sph_headless_tree_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                              ctypes.POINTER(ctypes.c_double)]
+        o.sph_headless_last_error.argtypes = [ctypes.c_void_p]
+        o.sph_headless_last_error.restype = ctypes.c_char_p
+    return _hip, _host
+
+
+# ------------------------------------------------------------------ inputs
+def default_params(**overrides) -> SphParams:
+    h, _ = libs()
+    p = SphParams()
+    h.sph_params_default(ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def params_dict(p: SphParams) -> Dict[str, float]:
+    return {name: getattr(p, name) for name, _ in SphParams._fields_ if name != "reserved"}
+
+
+def make_eos_tables(params: SphParams, res: int = 256) -> Dict[int, np.ndarray]:
+    """Synthetic Tillotson (P, c) tables: {400: granite, 402: iron}, each (res, res, 2) float32."""
+    _, o = libs()
+    out = {}
+    for mat, fn in ((400, o.sph_tillotson_granite), (402, o.sph_tillotson_iron)):
+        t = SphTillotson()
+        fn(ctypes.byref(t))
+        tab = np.zeros((res, res, 2), np.float32)
+        rc = o.sph_make_eos_table(ctypes.byref(params), ctypes.byref(t), res, mat, _fp(tab))
+        if rc != SPH_OK:
+            raise SphError(f"sph_make_eos_table failed: {rc}")
+        out[mat] = tab
+    return out
+
+
+def impact_config(n_total: int, seed: int, headon: bool = False, **overrides) -> SphImpactConfig:
+    _, o = libs()
+    c = SphImpactConfig()
+    (o.sph_impact_config_headon if headon else o.sph_impact_config_default)(ctypes.byref(c), n_total, seed)
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+def make_impact(params: SphParams, cfg: SphImpactConfig, tables: Dict[int, np.ndarray]) -> Dict[str, np.ndarray]:
+    """Two-body initial conditions in DataStruct layout (hdfHandler.hpp:15-26)."""
+    _, o = libs()
+    n = cfg.n_total
+    ic = {
+        "pos": np.zeros((n, 4), np.float32), "vel": np.zeros((n, 4), np.float32),
+        "rho": np.zeros(n, np.float32), "u": np.zeros(n, np.float32), "mass": np.zeros(n, np.float32),
+        "mat": np.zeros(n, np.int32), "pressure": np.zeros(n, np.float32), "h": np.zeros(n, np.float32),
+    }
+    t400 = np.ascontiguousarray(tables[400])
+    t402 = np.ascontiguousarray(tables[402])
+    rc = o.sph_make_impact(ctypes.byref(params), ctypes.byref(cfg), _fp(t400), _fp(t402), t400.shape[0],
+                           _fp(ic["pos"]), _fp(ic["vel"]), _fp(ic["rho"]), _fp(ic["u"]), _fp(ic["mass"]),
+                           _fp(ic["mat"], ctypes.c_int32), _fp(ic["pressure"]), _fp(ic["h"]))
+    if rc != SPH_OK:
+        raise SphError(f"sph_make_impact failed: {rc}")
+    return ic
+
+
+def octree_build(pos4: np.ndarray, max_leaf: int = 4, max_depth: int = 7, nthreads: int = 1) -> Dict[str, object]:
+    """CPU octree (octree.cpp layout): octree ints, per-level node offsets, node count."""
+    _, o = libs()
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    t = SphOctree()
+    rc = o.sph_octree_build(_fp(pos4), pos4.shape[0], max_leaf, max_depth, nthreads, ctypes.byref(t))
+    if rc != SPH_OK:
+        raise SphError(f"sph_octree_build failed: {rc}")
+    try:
+        octree = np.ctypeslib.as_array(t.octree, shape=(t.n_ints,)).copy()
+        counts = np.ctypeslib.as_array(t.level_counts, shape=(t.n_levels,)).copy()
+        nodes = np.ctypeslib.as_array(t.level_nodes, shape=(int(counts.sum()),)).copy()
+        return {"octree": octree, "level_nodes": nodes, "level_counts": counts, "node_count": int(t.node_count)}
+    finally:
+        o.sph_octree_free(ctypes.byref(t))
+
+
+# ----------------------------------------------------------------- context
+class Context:
+    """One sph_ctx: device state of a simulation, driven pass by pass."""
+
+    def __init__(self, params: SphParams, ic: Dict[str, np.ndarray], tables: Dict[int, np.ndarray]):
+        h, _ = libs()
+        self._h = h
+        self.params = params
+        self.n = int(ic["mass"].shape[0])
+        self._keep = {k: np.ascontiguousarray(v) for k, v in ic.items()}
+        view = SphIcView(self.n, _fp(self._keep["pos"]), _fp(self._keep["vel"]), _fp(self._keep["rho"]),
+                         _fp(self._keep["u"]), _fp(self._keep["mass"]), _fp(self._keep["mat"], ctypes.c_int32),
+                         _fp(self._keep["pressure"]), _fp(self._keep["h"]))
+        self._tabs = [np.ascontiguousarray(tables[400]), np.ascontiguousarray(tables[402])]
+        arr = (SphEosTable * 2)(SphEosTable(400, self._tabs[0].shape[0], _fp(self._tabs[0])),
+                                SphEosTable(402, self._tabs[1].shape[0], _fp(self._tabs[1])))
+        ptr = ctypes.c_void_p()
+        rc = h.sph_create(ctypes.byref(params), ctypes.byref(view), arr, 2, ctypes.byref(ptr))
+        if rc != SPH_OK:
+            raise SphError(f"sph_create failed: {rc}")
+        self.ptr = ptr
+
+    @classmethod
+    def from_handle(cls, ptr: int, params: SphParams):
+        obj = cls.__new__(cls)
+        obj._h = libs()[0]
+        obj.params = params
+        obj.ptr = ctypes.c_void_p(ptr)
+        obj.n = obj._h.sph_num_particles(obj.ptr)
+        obj._borrowed = True
+        return obj
+
+    def close(self):
+        if getattr(self, "ptr", None) and not getattr(self, "_borrowed", False):
+            self._h.sph_destroy(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != SPH_OK:
+            raise SphError(f"{what} failed ({rc}): {self._h.sph_last_error(self.ptr).decode()}")
+
+    def sort(self): self._check(self._h.sph_sort(self.ptr), "sph_sort")
+    def gravity(self): self._check(self._h.sph_gravity(self.ptr), "sph_gravity")
+    def density(self): self._check(self._h.sph_density(self.ptr), "sph_density")
+    def acceleration(self): self._check(self._h.sph_acceleration(self.ptr), "sph_acceleration")
+    def step(self): self._check(self._h.sph_step(self.ptr), "sph_step")
+    def sync(self): self._check(self._h.sph_sync(self.ptr), "sph_sync")
+
+    def upload_tree(self, tree: Dict[str, object]):
+        oc = np.ascontiguousarray(tree["octree"], np.int32)
+        ln = np.ascontiguousarray(tree["level_nodes"], np.int32)
+        lc = np.ascontiguousarray(tree["level_counts"], np.int32)
+        rc = self._h.sph_upload_tree(self.ptr, _fp(oc, ctypes.c_int32), oc.size, _fp(ln, ctypes.c_int32),
+                                     _fp(lc, ctypes.c_int32), lc.size, int(tree["node_count"]))
+        self._check(rc, "sph_upload_tree")
+        self.sync()  # pageable source buffers: wait until the copy has landed
+
+    def _shape_dtype(self, field: str, nbytes: int):
+        if field in _F4:
+            return (self.n, 4), np.float32
+        if field == "multipoles":
+            return (nbytes // 96, 24), np.float32
+        if field == "locals":
+            return (nbytes // 64, 16), np.float32
+        if field == "cell_array":
+            return (self.n, 2), np.uint32
+        if field in _U32:
+            return (nbytes // 4,), np.uint32
+        if field in _I32:
+            return (nbytes // 4,), np.int32
+        if field == "active":
+            return (nbytes,), np.uint8
+        return (nbytes // 4,), np.float32
+
+    def download(self, field: str) -> np.ndarray:
+        fid = FIELD_ID[field]
+        nbytes = self._h.sph_field_bytes(self.ptr, fid)
+        if nbytes < 0:
+            raise SphError(f"unknown field {field}")
+        shape, dt = self._shape_dtype(field, nbytes)
+        out = np.zeros(shape, dt)
+        self._check(self._h.sph_download(self.ptr, fid, out.ctypes.data_as(ctypes.c_void_p)), f"download {field}")
+        return out
+
+    def upload(self, field: str, arr: np.ndarray):
+        fid = FIELD_ID[field]
+        nbytes = self._h.sph_field_bytes(self.ptr, fid)
+        shape, dt = self._shape_dtype(field, nbytes)
+        a = np.ascontiguousarray(arr, dt).reshape(shape)
+        self._check(self._h.sph_upload(self.ptr, fid, a.ctypes.data_as(ctypes.c_void_p)), f"upload {field}")
+
+    def set_timing(self, on: bool = True):
+        self._check(self._h.sph_set_timing(self.ptr, 1 if on else 0), "sph_set_timing")
+
+    def pass_times(self) -> Dict[str, Dict[str, float]]:
+        ms = (ctypes.c_double * len(PASSES))()
+        calls = (ctypes.c_int64 * len(PASSES))()
+        self._check(self._h.sph_pass_times(self.ptr, ms, calls), "sph_pass_times")
+        return {p: {"ms": ms[i], "calls": calls[i]} for i, p in enumerate(PASSES)}
+
+    def reset_pass_times(self):
+        self._check(self._h.sph_reset_pass_times(self.ptr), "sph_reset_pass_times")
+
+
+class Headless:
+    """Headless::step driver (Headless.cpp:37-55) on synthetic inputs."""
+
+    def __init__(self, params: SphParams, cfg: SphImpactConfig, eos_res: int = 256, tree_mode: int = TREE_ASYNC,
+                 tree_threads: int = 8):
+        _, o = libs()
+        self._o = o
+        self.params = params
+        ptr = ctypes.c_void_p()
+        rc = o.sph_headless_create(ctypes.byref(params), ctypes.byref(cfg), eos_res, tree_mode, tree_threads,
+                                   ctypes.byref(ptr))
+        if rc != SPH_OK:
+            raise SphError(f"sph_headless_create failed: {rc}")
+        self.ptr = ptr
+        self.ctx = Context.from_handle(o.sph_headless_ctx(ptr), params)
+
+    def step(self):
+        rc = self._o.sph_headless_step(self.ptr)
+        if rc != SPH_OK:
+            raise SphError(f"Headless::step failed ({rc}): {self._o.sph_headless_last_error(self.ptr).decode()}")
+
+    def enqueue_step(self):
+        rc = self._o.sph_headless_enqueue_step(self.ptr)
+        if rc != SPH_OK:
+            raise SphError(f"Headless::step failed ({rc}): {self._o.sph_headless_last_error(self.ptr).decode()}")
+
+    def tree_stats(self):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        self._o.sph_headless_tree_stats(self.ptr, ctypes.byref(n), ctypes.byref(ms))
+        return int(n.value), float(ms.value)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self._o.sph_headless_destroy(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

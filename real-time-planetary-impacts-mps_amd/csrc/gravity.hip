@@ -1,0 +1,672 @@
+/*
+ * gravity.hip -- FMM self-gravity on the CPU-built octree, for gfx950.
+ *
+ * Reference: Compute::gravitationalPass (Compute.cpp:328-377), upPass /
+ * downPass (gravity.metal:36-143), helpers.h:15-147 and the poles/ kernels.
+ *
+ * The uploaded tree keeps the reference's flat layout (octree.cpp); a
+ * conversion kernel turns it into per-node tables indexed by node id (the
+ * pre-order node number the reference calls dataPointer): kind/particle
+ * range, 8 child ids, and the slot of the parent in its level list
+ * (parentIndexes, which the reference rewrites in every up pass, M2M.metal:105).
+ * Unchecked-node lists hold node ids instead of tree offsets; a list slot is
+ * `slot in level x max_unchecked_pointers`, -1 terminated, ping-ponged per
+ * level as in the reference.
+ *
+ * Up pass: one launch per level, deepest first, one thread per node (leaf =
+ * P2M over its particles, branch = M2M over its children).
+ * Down pass: one launch per level, root first, one thread per node: scan the
+ * parent's list with the multipole acceptance criterion, M2L the accepted
+ * nodes, push the rejected branch children (or force-accept them once the
+ * list is full), then L2L to the active children or, at a leaf, L2P plus the
+ * intra-leaf P2P.
+ */
+#include <cfloat>
+
+#include "sph_internal.h"
+
+namespace {
+
+enum { eM = 0, eX, eY, eZ, eXX, eXY, eXZ, eYY, eYZ, eZZ, eNT };
+
+struct Tree {
+    const int32_t* octree;
+    const int4* info;        /* (nParticles, treePointer, level, 0) */
+    const int32_t* child;
+    const int32_t* parent_slot;
+    float4* far;             /* 3 per node */
+    float4* aux;             /* 3 per node */
+    float4* loc;             /* 4 per node */
+};
+
+/* ------------------------------------------------------------ conversion */
+__global__ void k_fill_i32(int32_t* __restrict__ a, int64_t count, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) a[i] = v;
+}
+
+struct ConvertArgs {
+    const int32_t* octree;
+    int64_t n_ints;
+    const int32_t* level_tp;
+    int32_t* level_id;
+    int count;
+    int level;
+    int n_levels;
+    int node_count;
+    int n_particles;
+    int4* info;
+    int32_t* child;
+    int32_t* parent_slot;
+    int32_t* parent_level;
+    int32_t* seen;
+    uint32_t* flags; /* [0] != 0: the uploaded tree is malformed, gravity is skipped */
+};
+
+/* One thread per level slot: node id, kind, children and the parent slot
+ * (parentIndexes), with every index checked against its buffer so that a
+ * malformed tree disables gravity instead of reading out of bounds. */
+__global__ void k_tree_convert(ConvertArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.count) return;
+    const int tp = a.level_tp[s];
+    bool bad = tp < 0 || (int64_t)tp + 2 > a.n_ints;
+    const int np = bad ? 0 : a.octree[tp];
+    const int id = bad ? -1 : a.octree[tp + 1];
+    bad = bad || id < 0 || id >= a.node_count || np < 0;
+    bad = bad || (np > 0 && (int64_t)tp + 2 + np > a.n_ints) || (np == 0 && (int64_t)tp + 10 > a.n_ints);
+    bad = bad || (np == 0 && a.level == a.n_levels - 1) || (a.level == 0 && (tp != 0 || id != 0));
+    if (bad) {
+        a.flags[0] = 1u;
+        a.level_id[s] = 0;
+        return;
+    }
+    a.level_id[s] = id;
+    a.info[id] = make_int4(np, tp, a.level, 0);
+    atomicAdd(&a.seen[id], 1);
+    if (np > 0) {
+        for (int k = 0; k < np; ++k) {
+            const int p = a.octree[tp + 2 + k];
+            if (p < 0 || p >= a.n_particles) a.flags[0] = 1u;
+        }
+        return;
+    }
+    for (int c = 0; c < 8; ++c) {
+        const int cp = a.octree[tp + 2 + c];
+        int cid = -1;
+        if (cp != -1) {
+            if (cp < 0 || (int64_t)cp + 2 > a.n_ints) {
+                a.flags[0] = 1u;
+            } else {
+                cid = a.octree[cp + 1];
+                if (cid < 0 || cid >= a.node_count) {
+                    a.flags[0] = 1u;
+                    cid = -1;
+                }
+            }
+        }
+        a.child[8 * id + c] = cid;
+        if (cid >= 0) {
+            a.parent_slot[cid] = s;
+            a.parent_level[cid] = a.level;
+        }
+    }
+}
+
+/* Every node id appears once in the level lists, and every non-root node is
+ * the child of a node one level up. */
+__global__ void k_tree_check(int node_count, const int4* __restrict__ info, const int32_t* __restrict__ parent_slot,
+                             const int32_t* __restrict__ parent_level, const int32_t* __restrict__ seen,
+                             uint32_t* __restrict__ flags) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= node_count) return;
+    if (seen[id] != 1) {
+        flags[0] = 1u;
+        return;
+    }
+    const int level = info[id].z;
+    if (level == 0 ? id != 0 : (parent_slot[id] < 0 || parent_level[id] != level - 1)) flags[0] = 1u;
+}
+
+/* ---------------------------------------------------------------- up pass */
+struct UpArgs {
+    Tree t;
+    const uint32_t* flags;
+    const int32_t* level_id;
+    int count;
+    const float4* pos;
+    const float* m;
+    const int32_t* next_active;
+    const float* gravnorm;
+    uint8_t* active;
+    const int32_t* time;
+    int readme_rule;
+    int fix_zz;
+    int fix_range;
+};
+
+__device__ __forceinline__ float power2(float XX, float XY, float XZ, float YY, float YZ, float ZZ) {
+    /* helpers.metal:63-75 */
+    float p2 = ZZ * ZZ;
+    p2 += 0.5f * YZ * YZ;
+    p2 += YY * YY;
+    p2 += 0.5f * XZ * XZ;
+    p2 += 0.5f * XY * XY;
+    p2 += 0.5f * XX * XX;
+    return sqrtf(p2);
+}
+
+/* P2M.metal:12-107 */
+__device__ void p2m(const UpArgs& a, int id, int np, int tp) {
+    const int32_t* pl = a.t.octree + tp + 2;
+    const int gt = a.time[0];
+    float M = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+    float minGrav = FLT_MAX;
+    bool act = false;
+    float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+    for (int k = 0; k < np; ++k) {
+        const int p = pl[k];
+        const float pm = a.m[p];
+        const float4 x = a.pos[p];
+        const int na = a.next_active[p];
+        const bool isActive = a.readme_rule ? (na <= gt) : (gt <= na);
+        act = act || isActive;
+        mxx = fmaxf(mxx, x.x); mxy = fmaxf(mxy, x.y); mxz = fmaxf(mxz, x.z);
+        mnx = fminf(mnx, x.x); mny = fminf(mny, x.y); mnz = fminf(mnz, x.z);
+        M += pm;
+        px += x.x * pm;
+        py += x.y * pm;
+        pz += x.z * pm;
+        if (isActive) minGrav = fminf(minGrav, a.gravnorm[p]);
+    }
+    for (int k = 0; k < np; ++k) a.active[pl[k]] = act ? 1 : 0;
+    px /= M;
+    py /= M;
+    pz /= M;
+    float X = 0.f, Y = 0.f, Z = 0.f, XX = 0.f, XY = 0.f, XZ = 0.f, YY = 0.f, YZ = 0.f, ZZ = 0.f;
+    for (int k = 0; k < np; ++k) {
+        const int p = pl[k];
+        const float pm = a.m[p];
+        const float4 x = a.pos[p];
+        const float rx = x.x - px, ry = x.y - py, rz = x.z - pz;
+        X -= rx * pm;
+        Y -= ry * pm;
+        Z -= rz * pm;
+        XX += pm * rx * rx;
+        XY += pm * rx * ry;
+        XZ += pm * rx * rz;
+        YY += pm * ry * ry;
+        YZ += pm * ry * rz;
+        ZZ += pm * rz * rz;
+    }
+    XX *= 0.5f;
+    YY *= 0.5f;
+    ZZ *= 0.5f;
+    const float size = fmaxf(fmaxf(mxx - mnx, mxy - mny), mxz - mnz);
+    a.t.far[3 * id + 0] = make_float4(px, py, pz, size);
+    a.t.far[3 * id + 1] = make_float4(M, XX, XY, XZ);
+    a.t.far[3 * id + 2] = make_float4(YY, YZ, ZZ, power2(XX, XY, XZ, YY, YZ, ZZ));
+    a.t.aux[3 * id + 0] = make_float4(mnx, mny, mnz, minGrav);
+    a.t.aux[3 * id + 1] = make_float4(mxx, mxy, mxz, act ? 1.f : 0.f);
+    a.t.aux[3 * id + 2] = make_float4(X, Y, Z, 0.f);
+}
+
+/* M2M.metal:75-160 with transformMultipole (M2M.metal:12-73) */
+__device__ void m2m(const UpArgs& a, int id, int tp) {
+    const int32_t* ch = a.t.child + 8 * id;
+    float minGrav = FLT_MAX, mass = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+    bool act = false, allLeaves = true;
+    float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+    for (int c = 0; c < 8; ++c) {
+        const int cid = ch[c];
+        if (cid < 0) continue;
+        const float4 f0 = a.t.far[3 * cid], f1 = a.t.far[3 * cid + 1];
+        const float4 x0 = a.t.aux[3 * cid], x1 = a.t.aux[3 * cid + 1];
+        minGrav = fminf(x0.w, minGrav);
+        mass += f1.x;
+        px += f0.x * f1.x;
+        py += f0.y * f1.x;
+        pz += f0.z * f1.x;
+        act = act || (x1.w != 0.f);
+        if (allLeaves && a.t.info[cid].x == 0) allLeaves = false;
+        mxx = fmaxf(mxx, x1.x); mxy = fmaxf(mxy, x1.y); mxz = fmaxf(mxz, x1.z);
+        mnx = fminf(mnx, x0.x); mny = fminf(mny, x0.y); mnz = fminf(mnz, x0.z);
+    }
+    const float size = fmaxf(fmaxf(mxx - mnx, mxy - mny), mxz - mnz);
+    px /= mass;
+    py /= mass;
+    pz /= mass;
+    float M = 0.f, XX = 0.f, XY = 0.f, XZ = 0.f, YY = 0.f, YZ = 0.f, ZZ = 0.f;
+    for (int c = 0; c < 8; ++c) {
+        const int cid = ch[c];
+        if (cid < 0) continue;
+        const float4 f0 = a.t.far[3 * cid], f1 = a.t.far[3 * cid + 1], f2 = a.t.far[3 * cid + 2];
+        const float rx = px - f0.x, ry = py - f0.y, rz = pz - f0.z;
+        const float cM = f1.x;
+        /* as written the ZZ shift uses the child's dipole Z (M2M.metal:29) */
+        const float zzw = a.fix_zz ? cM : a.t.aux[3 * cid + 2].z;
+        M += cM;
+        XX += f1.y + 0.5f * rx * rx * cM;
+        XY += f1.z + rx * ry * cM;
+        XZ += f1.w + rx * rz * cM;
+        YY += f2.x + 0.5f * ry * ry * cM;
+        YZ += f2.y + ry * rz * cM;
+        ZZ += f2.z + 0.5f * rz * rz * zzw;
+        if (allLeaves && act && a.t.aux[3 * cid + 1].w == 0.f) {
+            /* leaf-sync activation (M2M.metal:145-154); as written the end of
+             * the particle range starts from the parent's record */
+            a.t.aux[3 * cid + 1].w = 1.f;
+            const int4 ci = a.t.info[cid];
+            const int cstart = ci.y + 2;
+            const int cend = min((a.fix_range ? cstart : tp + 2) + ci.x, cstart + ci.x);
+            for (int j = cstart; j < cend; ++j) a.active[a.t.octree[j]] = 1;
+        }
+    }
+    a.t.far[3 * id + 0] = make_float4(px, py, pz, size);
+    a.t.far[3 * id + 1] = make_float4(M, XX, XY, XZ);
+    a.t.far[3 * id + 2] = make_float4(YY, YZ, ZZ, power2(XX, XY, XZ, YY, YZ, ZZ));
+    a.t.aux[3 * id + 0] = make_float4(mnx, mny, mnz, minGrav);
+    a.t.aux[3 * id + 1] = make_float4(mxx, mxy, mxz, act ? 1.f : 0.f);
+    a.t.aux[3 * id + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_up(UpArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.count || a.flags[0] != 0u) return;
+    const int id = a.level_id[s];
+    const int4 inf = a.t.info[id];
+    if (inf.x > 0) p2m(a, id, inf.x, inf.y);
+    else m2m(a, id, inf.y);
+}
+
+/* -------------------------------------------------------------- down pass */
+struct DownArgs {
+    Tree t;
+    const uint32_t* flags;
+    const int32_t* level_id;
+    int count;
+    const int32_t* in;
+    int32_t* out;
+    int maxu;
+    int max_rec;
+    float eta;
+    float eps;        /* GRAVITY_SMOOTHING_LENGTH */
+    float G;
+    int fix_soft;
+    const float4* pos;
+    const float* m;
+    const uint8_t* active;
+    float4* acc;
+    float* gravnorm;
+};
+
+struct LocalExp {
+    float px, py, pz;
+    float e[eNT];
+};
+
+/* accept.metal:14-55 at P = 2 (the n = 1 term has power[1] = 0). */
+__device__ __forceinline__ bool mac_accept(float ax, float ay, float az, float rho_A, float minGrav, float4 b0,
+                                           float4 b1, float4 b2, float eta) {
+    const float rho_B = b0.w;
+    const float dx = ax - b0.x, dy = ay - b0.y, dz = az - b0.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float rho_sum = rho_A + rho_B;
+    if (rho_sum * rho_sum > r2) return false;
+    const float rho_max = fmaxf(rho_A, rho_B);
+    float E = b1.x * (rho_A * rho_A);
+    E += b2.w;
+    E *= 8.f;
+    if (rho_A + rho_B > 0.f) {
+        E *= rho_max;
+        E /= (rho_A + rho_B);
+    }
+    return E < eta * minGrav * r2 * r2;
+}
+
+/* M2L.metal:12-108 + derivatives.metal:59-156 at P = 2, accumulated. */
+__device__ __forceinline__ void m2l_add(LocalExp& L, float4 b0, float4 b1, float4 b2, float eps) {
+    const float vx = L.px - b0.x, vy = L.py - b0.y, vz = L.pz - b0.z;
+    const float r2 = vx * vx + vy * vy + vz * vz;
+    const float r = sqrtf(r2);
+    const float r_inv = r > 1e-6f ? 1.f / r : 0.f;
+    float Dt_1, Dt_2, Dt_3;
+    if (r2 < eps * eps) {
+        const float eps_inv = 1.f / eps;
+        const float u = r * eps_inv;
+        float s1 = -3.f * u + 15.f;
+        s1 = s1 * u - 28.f; s1 = s1 * u + 21.f; s1 = s1 * u; s1 = s1 * u - 7.f; s1 = s1 * u; s1 = s1 * u + 3.f;
+        float s2 = -21.f * u + 90.f;
+        s2 = s2 * u - 140.f; s2 = s2 * u + 84.f; s2 = s2 * u; s2 = s2 * u - 14.f; s2 = s2 * u;
+        float s3 = -105.f * u + 360.f;
+        s3 = s3 * u - 420.f; s3 = s3 * u + 168.f; s3 = s3 * u; s3 = s3 * u;
+        const float eps_inv2 = eps_inv * eps_inv;
+        Dt_1 = eps_inv * s1;
+        Dt_2 = eps_inv2 * s2;
+        Dt_3 = eps_inv2 * eps_inv * s3;
+    } else {
+        Dt_1 = r_inv;
+        Dt_2 = -1.f * Dt_1 * r_inv;
+        Dt_3 = -3.f * Dt_2 * r_inv;
+    }
+    const float rx = vx * r_inv, ry = vy * r_inv, rz = vz * r_inv;
+    const float DX = rx * Dt_2, DY = ry * Dt_2, DZ = rz * Dt_2;
+    const float D2 = Dt_2 * r_inv;
+    const float DXX = rx * rx * Dt_3 + D2, DXY = rx * ry * Dt_3, DXZ = rx * rz * Dt_3;
+    const float DYY = ry * ry * Dt_3 + D2, DYZ = ry * rz * Dt_3, DZZ = rz * rz * Dt_3 + D2;
+    const float M0 = b1.x;
+    float lm = M0 * Dt_1;
+    lm += b1.y * DXX + b2.x * DYY + b2.z * DZZ;
+    lm += b1.z * DXY + b1.w * DXZ + b2.y * DYZ;
+    L.e[eM] += lm;
+    L.e[eX] += M0 * DX;
+    L.e[eY] += M0 * DY;
+    L.e[eZ] += M0 * DZ;
+    L.e[eXX] += M0 * DXX;
+    L.e[eXY] += M0 * DXY;
+    L.e[eXZ] += M0 * DXZ;
+    L.e[eYY] += M0 * DYY;
+    L.e[eYZ] += M0 * DYZ;
+    L.e[eZZ] += M0 * DZZ;
+}
+
+struct NodeA {
+    float x, y, z, size, minGrav;
+};
+
+/* checkAndAddLocalExpansion (helpers.h:34-45) */
+__device__ __forceinline__ bool check_add(const DownArgs& a, const NodeA& A, LocalExp& L, int node, bool force) {
+    const float4 b0 = a.t.far[3 * node], b1 = a.t.far[3 * node + 1], b2 = a.t.far[3 * node + 2];
+    if (force || mac_accept(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.eta)) {
+        m2l_add(L, b0, b1, b2, a.eps);
+        return true;
+    }
+    return false;
+}
+
+/* recursiveScan (helpers.h:83-123) for a leaf */
+__device__ void recursive_scan(const DownArgs& a, const NodeA& A, LocalExp& L, int toScan, int self) {
+    if (check_add(a, A, L, toScan, a.t.info[toScan].x != 0)) return;
+    int nodes[4], idx[4];
+    int depth = 0;
+    nodes[0] = toScan;
+    idx[0] = 0;
+    int count = 0;
+    while (idx[0] < 8 && count < 2000) {
+        count++;
+        const int np = a.t.child[8 * nodes[depth] + idx[depth]];
+        if (np == self || np < 0) {
+            idx[depth]++;
+            while (idx[depth] == 8 && depth != 0) {
+                depth--;
+                idx[depth]++;
+            }
+            continue;
+        }
+        if (!check_add(a, A, L, np, a.t.info[np].x != 0 || depth == a.max_rec - 1)) {
+            depth++;
+            nodes[depth] = np;
+            idx[depth] = 0;
+            continue;
+        }
+        idx[depth]++;
+        while (idx[depth] == 8 && depth != 0) {
+            depth--;
+            idx[depth]++;
+        }
+    }
+}
+
+/* dphi_dr (kernels.h:87-102); as written the fractions are integer divisions */
+__device__ __forceinline__ float dphi_dr(float r, float h, int fixed) {
+    const float h1 = 1.f / fminf(h, 0.16f);
+    const float q = r * h1;
+    const float c43 = fixed ? 4.f / 3.f : 1.f, c65 = fixed ? 6.f / 5.f : 1.f;
+    const float c83 = fixed ? 8.f / 3.f : 2.f, c16 = fixed ? 1.f / 6.f : 0.f, c115 = fixed ? 1.f / 15.f : 0.f;
+    if (q <= 1.f) return (h1 * h1) * (c43 * q - c65 * q * q * q + 0.5f * q * q * q * q);
+    if (q <= 2.f) {
+        const float q1 = 1.f / q;
+        return (h1 * h1) * (c83 * q - 3.f * q * q + c65 * q * q * q - c16 * q * q * q * q - c115 * q1 * q1);
+    }
+    const float r1 = 1.f / r;
+    return r1 * r1;
+}
+
+/* localToLeaf (helpers.h:47-81): L2P plus the intra-leaf P2P */
+__device__ void leaf_field(const DownArgs& a, const LocalExp& L, int4 inf) {
+    const int np = inf.x;
+    const int32_t* pl = a.t.octree + inf.y + 2;
+    for (int k = 0; k < np; ++k) {
+        const int p = pl[k];
+        if (!a.active[p]) continue;
+        const float4 xi = a.pos[p];
+        const float rx = xi.x - L.px, ry = xi.y - L.py, rz = xi.z - L.pz;
+        /* L2P (L2P.metal:12-32) */
+        float ax = L.e[eX], ay = L.e[eY], az = L.e[eZ];
+        ax += rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ];
+        ay += rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ];
+        az += rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ];
+        for (int j = 0; j < np; ++j) {
+            if (j == k) continue;
+            const int pj = pl[j];
+            const float4 xj = a.pos[pj];
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            if (r == 0.f) continue;
+            const float w = a.m[pj] * dphi_dr(r, a.eps, a.fix_soft);
+            ax -= w * (dx / r);
+            ay -= w * (dy / r);
+            az -= w * (dz / r);
+        }
+        a.acc[p] = make_float4(a.G * ax, a.G * ay, a.G * az, 0.f);
+        a.gravnorm[p] = sqrtf(ax * ax + ay * ay + az * az);
+    }
+}
+
+/* downPass (gravity.metal:64-143) for one non-root node */
+__global__ __launch_bounds__(256) void k_down(DownArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.count || a.flags[0] != 0u) return;
+    const int id = a.level_id[s];
+    if (a.t.aux[3 * id + 1].w == 0.f) return;
+    const int4 inf = a.t.info[id];
+    const int np = inf.x;
+    uint32_t outp = (uint32_t)s * (uint32_t)a.maxu;
+    const uint32_t maxOut = outp + (uint32_t)a.maxu;
+    if (np == 0) a.out[outp++] = id;
+    const float4 f0 = a.t.far[3 * id];
+    NodeA A{f0.x, f0.y, f0.z, f0.w, a.t.aux[3 * id].w};
+    LocalExp L;
+    {
+        const float4 l0 = a.t.loc[4 * id], l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2],
+                     l3 = a.t.loc[4 * id + 3];
+        L.px = l0.x; L.py = l0.y; L.pz = l0.z;
+        L.e[eM] = l0.w; L.e[eX] = l1.x; L.e[eY] = l1.y; L.e[eZ] = l1.z; L.e[eXX] = l1.w;
+        L.e[eXY] = l2.x; L.e[eXZ] = l2.y; L.e[eYY] = l2.z; L.e[eYZ] = l2.w; L.e[eZZ] = l3.x;
+    }
+    const int32_t* in = a.in + (int64_t)a.t.parent_slot[id] * a.maxu;
+    for (int i = 0; i < a.maxu; ++i) {
+        const int toScan = in[i];
+        if (toScan == -1) break;
+        if (np != 0) {
+            recursive_scan(a, A, L, toScan, id);
+        } else {
+            /* nonRecursiveScan (helpers.h:125-147) */
+            if (check_add(a, A, L, toScan, false)) continue;
+            for (int c = 0; c < 8; ++c) {
+                const int cid = a.t.child[8 * toScan + c];
+                if (cid == id || cid < 0) continue;
+                if (!check_add(a, A, L, cid, a.t.info[cid].x != 0 || outp == maxOut)) a.out[outp++] = cid;
+            }
+        }
+    }
+    if (outp < maxOut && np == 0) a.out[outp] = -1;
+    if (np == 0) {
+        /* L2L (L2L.metal:90-104) to the active children */
+        for (int c = 0; c < 8; ++c) {
+            const int cid = a.t.child[8 * id + c];
+            if (cid < 0) continue;
+            if (a.t.aux[3 * cid + 1].w == 0.f) continue;
+            const float4 c0 = a.t.far[3 * cid];
+            const float rx = c0.x - L.px, ry = c0.y - L.py, rz = c0.z - L.pz;
+            /* transformLocal (L2L.metal:12-88) */
+            float nM = L.e[eM];
+            nM += rx * L.e[eX] + ry * L.e[eY] + rz * L.e[eZ];
+            nM += 0.5f * (rx * rx * L.e[eXX] + ry * ry * L.e[eYY] + rz * rz * L.e[eZZ]);
+            nM += rx * ry * L.e[eXY] + rx * rz * L.e[eXZ] + ry * rz * L.e[eYZ];
+            const float nX = L.e[eX] + (rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ]);
+            const float nY = L.e[eY] + (rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ]);
+            const float nZ = L.e[eZ] + (rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ]);
+            a.t.loc[4 * cid + 0] = make_float4(L.px + rx, L.py + ry, L.pz + rz, nM);
+            a.t.loc[4 * cid + 1] = make_float4(nX, nY, nZ, L.e[eXX]);
+            a.t.loc[4 * cid + 2] = make_float4(L.e[eXY], L.e[eXZ], L.e[eYY], L.e[eYZ]);
+            a.t.loc[4 * cid + 3] = make_float4(L.e[eZZ], 0.f, 0.f, 0.f);
+        }
+    } else {
+        leaf_field(a, L, inf);
+    }
+}
+
+/* Level 0.  zeroFirstBranches (helpers.h:15-32); the root has node id 0.
+ * A root that is itself a leaf (n <= max_children_in_leaf) makes the
+ * reference read particle indices as child pointers; here its particles get
+ * the intra-leaf P2P with a zero local expansion instead. */
+__global__ void k_down_root(DownArgs a) {
+    if (a.flags[0] != 0u) return;
+    const Tree& t = a.t;
+    const int4 inf = t.info[0];
+    if (inf.x != 0) {
+        const float4 f0 = t.far[0];
+        LocalExp L;
+        L.px = f0.x; L.py = f0.y; L.pz = f0.z;
+        for (int k = 0; k < eNT; ++k) L.e[k] = 0.f;
+        leaf_field(a, L, inf);
+        a.out[0] = -1;
+        return;
+    }
+    for (int c = 0; c < 8; ++c) {
+        const int cid = t.child[c];
+        if (cid < 0) continue;
+        const float4 f0 = t.far[3 * cid];
+        t.loc[4 * cid + 0] = make_float4(f0.x, f0.y, f0.z, 0.f);
+        t.loc[4 * cid + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        t.loc[4 * cid + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        t.loc[4 * cid + 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    a.out[0] = 0;
+    a.out[1] = -1;
+}
+
+Tree make_tree(sph_ctx* ctx) {
+    Tree t;
+    t.octree = ctx->tree.octree;
+    t.info = ctx->tree.node_info;
+    t.child = ctx->tree.node_child;
+    t.parent_slot = ctx->tree.parent_slot;
+    t.far = ctx->tree.mp_far;
+    t.aux = ctx->tree.mp_aux;
+    t.loc = ctx->tree.loc;
+    return t;
+}
+
+}  // namespace
+
+int launch_tree_convert(sph_ctx* ctx) {
+    sph_tree_dev& T = ctx->tree;
+    const int threads = 256;
+    const int K = T.node_count;
+    const int kb = (K + threads - 1) / threads;
+    k_fill_i32<<<kb, threads, 0, ctx->stream>>>(T.parent_slot, K, -1);
+    k_fill_i32<<<kb, threads, 0, ctx->stream>>>(T.parent_level, K, -1);
+    k_fill_i32<<<kb, threads, 0, ctx->stream>>>(T.seen, K, 0);
+    k_fill_i32<<<1, 1, 0, ctx->stream>>>((int32_t*)ctx->grav_flags, 1, 0);
+    SPH_HIP_TRY(hipGetLastError());
+    ConvertArgs a;
+    a.octree = T.octree;
+    a.n_ints = T.n_ints;
+    a.n_levels = (int)T.level_counts.size();
+    a.node_count = K;
+    a.n_particles = ctx->n;
+    a.info = T.node_info;
+    a.child = T.node_child;
+    a.parent_slot = T.parent_slot;
+    a.parent_level = T.parent_level;
+    a.seen = T.seen;
+    a.flags = ctx->grav_flags;
+    for (size_t l = 0; l < T.level_counts.size(); ++l) {
+        a.count = T.level_counts[l];
+        if (a.count == 0) continue;
+        a.level = (int)l;
+        a.level_tp = T.level_tp + T.level_offsets[l];
+        a.level_id = T.level_id + T.level_offsets[l];
+        k_tree_convert<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+        SPH_HIP_TRY(hipGetLastError());
+    }
+    k_tree_check<<<kb, threads, 0, ctx->stream>>>(K, T.node_info, T.parent_slot, T.parent_level, T.seen,
+                                                  ctx->grav_flags);
+    SPH_HIP_TRY(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_gravity_up(sph_ctx* ctx) {
+    sph_tree_dev& T = ctx->tree;
+    UpArgs a;
+    a.t = make_tree(ctx);
+    a.flags = ctx->grav_flags;
+    a.pos = ctx->pos;
+    a.m = ctx->m;
+    a.next_active = ctx->next_active;
+    a.gravnorm = ctx->gravnorm;
+    a.active = ctx->active;
+    a.time = ctx->time;
+    a.readme_rule = ctx->prm.activity_rule == SPH_ACTIVE_README;
+    a.fix_zz = (ctx->prm.fixes & SPH_FIX_M2M_ZZ) ? 1 : 0;
+    a.fix_range = (ctx->prm.fixes & SPH_FIX_SYNC_RANGE) ? 1 : 0;
+    const int threads = 256;
+    for (int l = (int)T.level_counts.size() - 1; l >= 0; --l) {
+        a.count = T.level_counts[l];
+        if (a.count == 0) continue;
+        a.level_id = T.level_id + T.level_offsets[l];
+        k_up<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+        SPH_HIP_TRY(hipGetLastError());
+    }
+    return SPH_OK;
+}
+
+int launch_gravity_down(sph_ctx* ctx) {
+    sph_tree_dev& T = ctx->tree;
+    const sph_params& p = ctx->prm;
+    DownArgs a;
+    a.t = make_tree(ctx);
+    a.flags = ctx->grav_flags;
+    a.maxu = p.max_unchecked_pointers;
+    a.max_rec = p.gravity_max_recursion;
+    a.eta = p.gravity_eta;
+    a.eps = p.gravity_softening;
+    a.G = p.gravity_g;
+    a.fix_soft = (p.fixes & SPH_FIX_SOFTENING) ? 1 : 0;
+    a.pos = ctx->pos;
+    a.m = ctx->m;
+    a.active = ctx->active;
+    a.acc = ctx->acc;
+    a.gravnorm = ctx->gravnorm;
+    const int threads = 256;
+    for (size_t l = 0; l < T.level_counts.size(); ++l) {
+        int32_t* in = (l % 2 == 0) ? ctx->list_a : ctx->list_b;
+        int32_t* out = (l % 2 == 0) ? ctx->list_b : ctx->list_a;
+        if (l == 0) {
+            a.out = out;
+            k_down_root<<<1, 1, 0, ctx->stream>>>(a);
+            SPH_HIP_TRY(hipGetLastError());
+            continue;
+        }
+        a.count = T.level_counts[l];
+        if (a.count == 0) continue;
+        a.level_id = T.level_id + T.level_offsets[l];
+        a.in = in;
+        a.out = out;
+        k_down<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+        SPH_HIP_TRY(hipGetLastError());
+    }
+    return SPH_OK;
+}

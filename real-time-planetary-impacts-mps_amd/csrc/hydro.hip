@@ -1,0 +1,561 @@
+/*
+ * hydro.hip -- SPH density (Newton-Raphson smoothing length, EOS table,
+ * Balsara switch), acceleration (pressure + viscosity, CFL timestep) and the
+ * drift/kick step, for gfx950.
+ *
+ * References: density.metal:12-239, accelerations.metal:12-141,
+ * step.metal:12-46, kernels.h:35-102, cellsToScan.h:21-59.
+ *
+ * One thread per Morton-sorted slot.  Neighbour candidates are read from the
+ * Morton-ordered copies written after the sort (s_posm, s_vel) and at the
+ * start of the acceleration pass (s_hyd, s_bal), so a cell's particles are
+ * contiguous in HBM and neighbouring threads read overlapping cells.  Summation order per
+ * particle is the reference's: cells x -> y -> z over the particle's own cell
+ * box, particles of a cell in sorted order.
+ *
+ * The reference keeps a per-thread cache of up to N_NEIGHBOURS_ESTIM
+ * (index, r) pairs from the last cell scan (density.metal:66-70,113-118).  It
+ * is used (a) inside the NR loop only when h < max_h, where max_h is an int
+ * (so only after a particle sat at h = 1), and (b) for the second pass when
+ * it did not fill up.  Its content is exactly "the members (r^2 <= 4 h_s^2) of
+ * the cell box of the last scanned h_s, in scan order", so instead of a
+ * private array (scratch memory on a GPU) the kernel rescans that box with
+ * that membership test and the cache's own range test (r > 2h), which yields
+ * the same terms in the same order.
+ */
+#include "sph_internal.h"
+
+namespace {
+
+constexpr float kPi1 = 1.f / 3.14159265358979323846f;
+
+struct Cells {
+    const uint2* cell_se;
+    float cell_size;
+    int mask;
+};
+
+/* cellsToScan.h:33-59.  Guard: a non-finite bound gives an empty box and an
+ * extent is capped at 2^cell_power cells per axis (the periodic grid), so a
+ * corrupted particle cannot make the scan unbounded.  Neither guard changes a
+ * box of a finite particle with h <= MAX_SMOOTHING_LENGTH. */
+__device__ __forceinline__ void cell_box(float3 x, float cs, float h, int cells_per_dim, int mn[3], int mx[3]) {
+    const float radius = 2.f * h;
+    const float lo[3] = {(x.x - radius) / cs, (x.y - radius) / cs, (x.z - radius) / cs};
+    const float hi[3] = {(x.x + radius) / cs, (x.y + radius) / cs, (x.z + radius) / cs};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (!(lo[k] <= hi[k]) || !(fabsf(lo[k]) < 1e9f) || !(fabsf(hi[k]) < 1e9f)) {
+            mn[k] = 0;
+            mx[k] = -1;
+            continue;
+        }
+        mn[k] = (int)floorf(lo[k]);
+        mx[k] = (int)floorf(hi[k]);
+        if (mx[k] - mn[k] >= cells_per_dim) mx[k] = mn[k] + cells_per_dim - 1;
+    }
+}
+
+/* kernels.h:35-50 */
+__device__ __forceinline__ float kW(float r, float h1) {
+    const float q = r * h1;
+    if (q > 2.0f) return 0.f;
+    const float fac = kPi1 * h1 * h1 * h1;
+    if (q > 1.0f) {
+        const float tmp = 2.f - q;
+        return fac * 0.25f * tmp * tmp * tmp;
+    }
+    return fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
+}
+/* kernels.h:52-69: scalar factor g with gradW = g * x_ij / r */
+__device__ __forceinline__ float kGradWFactor(float r, float h1) {
+    const float q = r * h1;
+    if (q > 2.f || q < 1e-12f) return 0.f;
+    const float fac = kPi1 * h1 * h1 * h1 * h1;
+    if (q > 1.f) {
+        const float tmp = 2.f - q;
+        return -0.75f * fac * tmp * tmp;
+    }
+    return 3.f * fac * (0.75f * q - 1.f) * q;
+}
+/* kernels.h:71-85 */
+__device__ __forceinline__ float kdWdh(float r, float h1) {
+    const float q = r * h1;
+    if (q > 2.0f) return 0.f;
+    const float fac = -3.f * kPi1 * h1 * h1 * h1 * h1;
+    if (q > 1.0f) return fac * (2.f - 4.f * q + 2.5f * q * q - 0.5f * q * q * q);
+    return fac * (1.f + 0.5f * (3.f * q - 5.f) * q * q);
+}
+
+/* Bilinear sample with Metal's texture conventions (density.metal:45-50):
+ * normalised coordinates, texel centres at (i + 1/2)/res, clamp to edge.
+ * Column = internal energy axis, row = density axis. */
+__device__ __forceinline__ float2 eos_sample(const float2* __restrict__ tab, int res, float u, float rho, float min_u,
+                                             float max_u, float min_rho, float max_rho) {
+    float uc = logf(u / min_u) / logf(max_u / min_u);
+    float rc = logf(rho / min_rho) / logf(max_rho / min_rho);
+    uc = fminf(fmaxf(uc, 0.f), 1.f);
+    rc = fminf(fmaxf(rc, 0.f), 1.f);
+    const float tx = uc * (float)res - 0.5f, ty = rc * (float)res - 0.5f;
+    const float fx = floorf(tx), fy = floorf(ty);
+    const float a = tx - fx, b = ty - fy;
+    const int ix = (int)fx, iy = (int)fy;
+    const int i0 = min(max(ix, 0), res - 1), i1 = min(max(ix + 1, 0), res - 1);
+    const int j0 = min(max(iy, 0), res - 1), j1 = min(max(iy + 1, 0), res - 1);
+    const float2 t00 = tab[j0 * res + i0], t01 = tab[j0 * res + i1];
+    const float2 t10 = tab[j1 * res + i0], t11 = tab[j1 * res + i1];
+    const float w00 = (1.f - a) * (1.f - b), w01 = a * (1.f - b), w10 = (1.f - a) * b, w11 = a * b;
+    return make_float2(w00 * t00.x + w01 * t01.x + w10 * t10.x + w11 * t11.x,
+                       w00 * t00.y + w01 * t01.y + w10 * t10.y + w11 * t11.y);
+}
+
+struct DensityArgs {
+    int n;
+    const uint2* sorted;
+    Cells cells;
+    const float4* s_posm;
+    const float4* s_vel;
+    const uint8_t* active;
+    const int32_t* mat;
+    const float* u;
+    float *rho, *h, *P, *c, *f, *B;
+    int32_t* nr_iters;
+    int32_t* time;
+    const float2* eos0;
+    const float2* eos1;
+    int eos_res;
+    float min_u, max_u, min_rho, max_rho;
+    float eta, res_eta, max_h_len;
+    int max_it, ne;
+    int fix_cache;
+    int dt_reset;
+};
+
+/* density.metal:12-239 */
+__global__ __launch_bounds__(256) void k_density(DensityArgs a) {
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= a.n) return;
+    if (index == 0) a.time[1] = a.dt_reset;
+    const uint32_t ind = a.sorted[index].y;
+    const float2* table = a.mat[ind] == 400 ? a.eos0 : a.eos1;
+    if (!a.active[ind]) {
+        const float rho = a.rho[ind];
+        const float2 pc = eos_sample(table, a.eos_res, a.u[ind], rho, a.min_u, a.max_u, a.min_rho, a.max_rho);
+        a.P[ind] = pc.x;
+        a.c[ind] = pc.y;
+        a.nr_iters[ind] = 0;
+        return;
+    }
+    const float4 pm = a.s_posm[index];
+    const float3 position = make_float3(pm.x, pm.y, pm.z);
+    const float mass_i = pm.w;
+    const uint2* __restrict__ cse = a.cells.cell_se;
+    const float4* __restrict__ spm = a.s_posm;
+    const float cs = a.cells.cell_size;
+    const int mask = a.cells.mask;
+
+    float h_i = a.h[ind];
+    int max_h = 0;
+    float h2x4 = 4.f * h_i * h_i;
+    float h1 = 1.f / h_i;
+    float density = 0.f;
+    float eps = 1.f;
+    int count = 0;
+    int nN = 0;             /* cached-neighbour count of the last scan (capped) */
+    float h_scan = h_i;     /* h of the last scan */
+    float hs2x4 = h2x4;     /* its 4 h^2 */
+    while (eps > a.eta && count < a.max_it) {
+        density = 0.f;
+        const bool use_cache = (h_i < (float)max_h) && (nN < a.ne);
+        const float hb = use_cache ? h_scan : h_i;
+        const float hb2 = use_cache ? hs2x4 : h2x4;
+        const float hsum = h_i + h_i;
+        int mn[3], mx[3];
+        cell_box(position, cs, hb, mask + 1, mn, mx);
+        int cnt = 0;
+        for (int x = mn[0]; x <= mx[0]; x++) {
+            const uint32_t ex = dev_expand_bits((uint32_t)(x & mask));
+            for (int y = mn[1]; y <= mx[1]; y++) {
+                const uint32_t exy = ex | (dev_expand_bits((uint32_t)(y & mask)) << 1);
+                for (int z = mn[2]; z <= mx[2]; z++) {
+                    const uint32_t key = exy | (dev_expand_bits((uint32_t)(z & mask)) << 2);
+                    const uint2 se = cse[key];
+                    if (se.x == 0xFFFFFFFFu) continue;
+                    for (uint32_t k = se.x; k <= se.y; k++) {
+                        const float4 q = spm[k];
+                        const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 > hb2) continue;
+                        const float r = sqrtf(r2);
+                        if (use_cache) {
+                            if (r > hsum) continue;
+                        } else {
+                            cnt++;
+                        }
+                        density += kW(r, h1) * q.w;
+                    }
+                }
+            }
+        }
+        if (!use_cache) {
+            nN = min(cnt, a.ne);
+            h_scan = h_i;
+            hs2x4 = h2x4;
+        }
+        float newH = fminf(a.res_eta * powf(mass_i / density, 1.f / 3.f), a.max_h_len);
+        newH = 0.5f * (h_i + newH);
+        eps = fabsf(newH - h_i) / h_i;
+        if (eps > a.eta) {
+            max_h = (int)fmaxf((float)max_h, h_i);
+            h_i = newH;
+            h1 = 1.f / h_i;
+            h2x4 = 4.f * h_i * h_i;
+            count++;
+        }
+    }
+    a.nr_iters[ind] = count;
+
+    /* second pass (density.metal:146-213) */
+    float density_h = 0.f, velDiv = 0.f;
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    const float4 vi4 = a.s_vel[index];
+    const bool cached = nN < a.ne;
+    if (!cached && a.fix_cache) density = 0.f;
+    {
+        const float hb = cached ? h_scan : h_i;
+        const float hb2 = cached ? hs2x4 : h2x4;
+        const float hsum = h_i + h_i;
+        int mn[3], mx[3];
+        cell_box(position, cs, hb, mask + 1, mn, mx);
+        for (int x = mn[0]; x <= mx[0]; x++) {
+            const uint32_t ex = dev_expand_bits((uint32_t)(x & mask));
+            for (int y = mn[1]; y <= mx[1]; y++) {
+                const uint32_t exy = ex | (dev_expand_bits((uint32_t)(y & mask)) << 1);
+                for (int z = mn[2]; z <= mx[2]; z++) {
+                    const uint32_t key = exy | (dev_expand_bits((uint32_t)(z & mask)) << 2);
+                    const uint2 se = cse[key];
+                    if (se.x == 0xFFFFFFFFu) continue;
+                    for (uint32_t k = se.x; k <= se.y; k++) {
+                        const float4 q = spm[k];
+                        const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 > hb2) continue;
+                        const float r = sqrtf(r2);
+                        if (cached && r > hsum) continue;
+                        const float mass = q.w;
+                        if (!cached) density += kW(r, h1) * mass;
+                        density_h += mass * kdWdh(r, h1);
+                        if (r > 1e-12f) {
+                            const float r1 = 1.f / r;
+                            const float4 vj = a.s_vel[k];
+                            const float vx = vj.x - vi4.x, vy = vj.y - vi4.y, vz = vj.z - vi4.z;
+                            const float g = kGradWFactor(r, h1);
+                            const float gx = g * (dx * r1), gy = g * (dy * r1), gz = g * (dz * r1);
+                            velDiv += mass * (vx * gx + vy * gy + vz * gz);
+                            cx += mass * (vy * gz - vz * gy);
+                            cy += mass * (vz * gx - vx * gz);
+                            cz += mass * (vx * gy - vy * gx);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    velDiv = fabsf(velDiv) / density;
+    const float absCurl = sqrtf(cx * cx + cy * cy + cz * cz) / density;
+    const float fgrad = 1.f / (1.f + (h_i / (3.f * density)) * density_h);
+    const float2 pc = eos_sample(table, a.eos_res, a.u[ind], density, a.min_u, a.max_u, a.min_rho, a.max_rho);
+    const float bal = velDiv / (velDiv + absCurl + 1e-4f * (pc.y / h_i));
+    a.h[ind] = h_i;
+    a.rho[ind] = density;
+    a.f[ind] = fgrad;
+    a.P[ind] = pc.x;
+    a.c[ind] = pc.y;
+    a.B[ind] = bal;
+}
+
+struct AccelArgs {
+    int n;
+    const uint2* sorted;
+    Cells cells;
+    const float4* s_posm;
+    const float4* s_vel;
+    const float4* s_hyd;
+    const float* s_bal;
+    const uint8_t* active;
+    float4* acc;
+    float* du;
+    float* dhdt;
+    int32_t* next_active;
+    int32_t* time;
+    float alpha, cfl, gamma, dt0, dtmin, max_h_len;
+    int r_max;
+};
+
+/* Morton-ordered copy of the per-particle terms the acceleration loop reads
+ * for every candidate: (h, f P / rho^2, rho, c) and the Balsara factor.  The
+ * product f P / rho^2 is formed exactly as accelerations.metal:45,91 forms it. */
+__global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __restrict__ sorted,
+                                                      const float* __restrict__ h, const float* __restrict__ f,
+                                                      const float* __restrict__ P, const float* __restrict__ rho,
+                                                      const float* __restrict__ c, const float* __restrict__ B,
+                                                      float4* __restrict__ s_hyd, float* __restrict__ s_bal) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = sorted[k].y;
+    const float r = rho[j];
+    s_hyd[k] = make_float4(h[j], f[j] * P[j] / (r * r), r, c[j]);
+    s_bal[k] = B[j];
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+/* accelerations.metal:12-141 */
+__global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    int integerDt = 0x7FFFFFFF;
+    if (index < a.n) {
+        const uint32_t ind = a.sorted[index].y;
+        if (a.active[ind]) {
+            const float4 hyd = a.s_hyd[index];
+            const float h_i = hyd.x, fact_i = hyd.y, rho_i = hyd.z, c_i = hyd.w;
+            const float B_i = a.s_bal[index];
+            const float4 xi4 = a.s_posm[index];
+            const float4 vi4 = a.s_vel[index];
+            const float hi1 = 1.f / h_i;
+            const float hi2x4 = h_i * h_i * 4.f;
+            float dvx = 0.f, dvy = 0.f, dvz = 0.f, du_dt = 0.f, dhdt = 0.f;
+            float v_sigi = 1e-12f;
+            const uint2* __restrict__ cse = a.cells.cell_se;
+            const int mask = a.cells.mask;
+            int mn[3], mx[3];
+            cell_box(make_float3(xi4.x, xi4.y, xi4.z), a.cells.cell_size, fminf(h_i, a.max_h_len), mask + 1, mn, mx);
+            for (int x = mn[0]; x <= mx[0]; x++) {
+                const uint32_t ex = dev_expand_bits((uint32_t)(x & mask));
+                for (int y = mn[1]; y <= mx[1]; y++) {
+                    const uint32_t exy = ex | (dev_expand_bits((uint32_t)(y & mask)) << 1);
+                    for (int z = mn[2]; z <= mx[2]; z++) {
+                        const uint32_t key = exy | (dev_expand_bits((uint32_t)(z & mask)) << 2);
+                        const uint2 se = cse[key];
+                        if (se.x == 0xFFFFFFFFu) continue;
+                        for (uint32_t k = se.x; k <= se.y; k++) {
+                            const float4 xj4 = a.s_posm[k];
+                            const float dx = xi4.x - xj4.x, dy = xi4.y - xj4.y, dz = xi4.z - xj4.z;
+                            const float r2 = dx * dx + dy * dy + dz * dz;
+                            const float4 hj = a.s_hyd[k];
+                            const float h_j = hj.x;
+                            const float hj2x4 = h_j * h_j * 4.f;
+                            if ((r2 > hi2x4 && r2 > hj2x4) || r2 < 1e-12f) continue;
+                            const float r = sqrtf(r2);
+                            const float r1 = 1.f / r;
+                            const float m_j = xj4.w;
+                            const float fact_j = hj.y;
+                            const float rho_j = hj.z;
+                            const float c_j = hj.w;
+                            const float4 vj4 = a.s_vel[k];
+                            const float hj1 = 1.f / h_j;
+                            const float vx = vi4.x - vj4.x, vy = vi4.y - vj4.y, vz = vi4.z - vj4.z;
+                            const float rho_ij = 0.5f * (rho_i + rho_j);
+                            const float v_dot_r = vx * dx + vy * dy + vz * dz;
+                            const float mu_ij = v_dot_r < 0.f ? v_dot_r * r1 : 0.f;
+                            const float v_sigij = c_i + c_j - 3.f * mu_ij;
+                            v_sigi = fmaxf(v_sigij, v_sigi);
+                            const float B_ij = 0.5f * (B_i + a.s_bal[k]);
+                            const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij / rho_ij;
+                            const float rx = dx * r1, ry = dy * r1, rz = dz * r1;
+                            const float gi = kGradWFactor(r, hi1), gj = kGradWFactor(r, hj1);
+                            const float gix = gi * rx, giy = gi * ry, giz = gi * rz;
+                            const float gjx = gj * rx, gjy = gj * ry, gjz = gj * rz;
+                            const float gmx = 0.5f * (gix + gjx), gmy = 0.5f * (giy + gjy), gmz = 0.5f * (giz + gjz);
+                            dvx -= m_j * (fact_i * gix + fact_j * gjx + nu_ij * gmx);
+                            dvy -= m_j * (fact_i * giy + fact_j * gjy + nu_ij * gmy);
+                            dvz -= m_j * (fact_i * giz + fact_j * gjz + nu_ij * gmz);
+                            const float vgi = vx * gix + vy * giy + vz * giz;
+                            const float vgm = vx * gmx + vy * gmy + vz * gmz;
+                            du_dt += m_j * (fact_i * vgi + 0.5f * nu_ij * vgm);
+                            dhdt += (m_j / rho_j) * vgi;
+                        }
+                    }
+                }
+            }
+            float4 ac = a.acc[ind];
+            ac.x += dvx;
+            ac.y += dvy;
+            ac.z += dvz;
+            a.acc[ind] = ac;
+            a.du[ind] = du_dt;
+            a.dhdt[ind] = dhdt;
+            const float goaldt = 2.0f * a.cfl * a.gamma * h_i / v_sigi;
+            int r = 0;
+            float dtCandidate = a.dt0;
+            while (dtCandidate > goaldt && r < a.r_max) {
+                dtCandidate *= 0.5f;
+                r++;
+            }
+            integerDt = (int)ceilf(a.dtmin * dtCandidate);
+            a.next_active[ind] = a.time[0] + integerDt;
+        }
+    }
+    const int wmin = wave_min(integerDt);
+    if ((threadIdx.x & 63) == 0 && wmin != 0x7FFFFFFF) atomicMin(&a.time[1], wmin);
+}
+
+struct StepArgs {
+    int n;
+    float4 *pos, *vel;
+    const float4* acc;
+    float *rho, *u, *h;
+    const float *du, *dhdt;
+    const uint8_t* active;
+    int32_t* next_active;
+    int32_t* time;
+    float dtmin1, max_h_len;
+    int max_rungs;
+};
+
+/* step.metal:12-46 */
+__global__ __launch_bounds__(256) void k_step(StepArgs a) {
+    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    const int dtInt = a.time[1];
+    if (index >= a.n) return;
+    const float dt = a.dtmin1 * (float)dtInt;
+    const float4 velocity = a.vel[index];
+    const float4 acceleration = a.acc[index];
+    const float h_i = a.h[index];
+    const float h1 = 1.f / h_i;
+    const float dhdt = a.dhdt[index];
+    a.h[index] = fminf(h_i * expf(h1 * dhdt * dt), a.max_h_len);
+    a.rho[index] *= expf(-3.f * h1 * dhdt * dt);
+    a.vel[index] = make_float4(velocity.x + dt * acceleration.x, velocity.y + dt * acceleration.y,
+                               velocity.z + dt * acceleration.z, velocity.w);
+    a.u[index] += dt * a.du[index];
+    const float4 p = a.pos[index];
+    const float hx = 0.5f * acceleration.x * dt * dt, hy = 0.5f * acceleration.y * dt * dt,
+                hz = 0.5f * acceleration.z * dt * dt;
+    a.pos[index] = make_float4(p.x + (dt * velocity.x + hx), p.y + (dt * velocity.y + hy),
+                               p.z + (dt * velocity.z + hz), p.w);
+    if (a.max_rungs > 0) {
+        const int cap = dtInt << (a.max_rungs - 1);
+        const int gt = a.time[0];
+        if (a.active[index] && a.next_active[index] - gt > cap) a.next_active[index] = gt + cap;
+    } else if (index == 0) {
+        a.time[0] += dtInt;
+    }
+}
+
+__global__ void k_advance_time(int32_t* time) { time[0] += time[1]; }
+
+Cells make_cells(const sph_ctx* ctx) {
+    Cells c;
+    c.cell_se = ctx->cell_se;
+    c.cell_size = ctx->prm.cell_width;
+    c.mask = (1 << ctx->prm.cell_power) - 1;
+    return c;
+}
+
+}  // namespace
+
+int launch_density(sph_ctx* ctx) {
+    const sph_params& p = ctx->prm;
+    DensityArgs a;
+    a.n = ctx->n;
+    a.sorted = ctx->pairs_a;
+    a.cells = make_cells(ctx);
+    a.s_posm = ctx->s_posm;
+    a.s_vel = ctx->s_vel;
+    a.active = ctx->active;
+    a.mat = ctx->mat;
+    a.u = ctx->u;
+    a.rho = ctx->rho;
+    a.h = ctx->h;
+    a.P = ctx->P;
+    a.c = ctx->c;
+    a.f = ctx->f;
+    a.B = ctx->B;
+    a.nr_iters = ctx->nr_iters;
+    a.time = ctx->time;
+    a.eos0 = ctx->eos[0];
+    a.eos1 = ctx->eos[1];
+    a.eos_res = ctx->eos_res;
+    a.min_u = p.eos_min_u;
+    a.max_u = p.eos_max_u;
+    a.min_rho = p.eos_min_rho;
+    a.max_rho = p.eos_max_rho;
+    a.eta = p.density_eta;
+    a.res_eta = p.resolution_eta;
+    a.max_h_len = p.max_smoothing_length;
+    a.max_it = p.max_nr_iterations;
+    a.ne = p.n_neighbours_estim;
+    a.fix_cache = (p.fixes & SPH_FIX_DENSITY_CACHE) ? 1 : 0;
+    const float dtmin1 = p.dt0 / (float)(1 << p.r_max);
+    a.dt_reset = (int)(p.dt0 * (1.f / dtmin1));
+    const int threads = 256;
+    k_density<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+    SPH_HIP_TRY(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_acceleration(sph_ctx* ctx) {
+    const sph_params& p = ctx->prm;
+    AccelArgs a;
+    a.n = ctx->n;
+    a.sorted = ctx->pairs_a;
+    a.cells = make_cells(ctx);
+    a.s_posm = ctx->s_posm;
+    a.s_vel = ctx->s_vel;
+    a.s_hyd = ctx->s_hyd;
+    a.s_bal = ctx->s_bal;
+    a.active = ctx->active;
+    a.acc = ctx->acc;
+    a.du = ctx->du;
+    a.dhdt = ctx->dhdt;
+    a.next_active = ctx->next_active;
+    a.time = ctx->time;
+    a.alpha = p.viscosity_alpha;
+    a.cfl = p.cfl;
+    a.gamma = p.gamma;
+    a.dt0 = p.dt0;
+    const float dtmin1 = p.dt0 / (float)(1 << p.r_max);
+    a.dtmin = 1.f / dtmin1;
+    a.max_h_len = p.max_smoothing_length;
+    a.r_max = p.r_max;
+    const int threads = 256;
+    k_gather_hydro<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(
+        ctx->n, ctx->pairs_a, ctx->h, ctx->f, ctx->P, ctx->rho, ctx->c, ctx->B, ctx->s_hyd, ctx->s_bal);
+    SPH_HIP_TRY(hipGetLastError());
+    k_acceleration<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+    SPH_HIP_TRY(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_step(sph_ctx* ctx) {
+    const sph_params& p = ctx->prm;
+    StepArgs a;
+    a.n = ctx->n;
+    a.pos = ctx->pos;
+    a.vel = ctx->vel;
+    a.acc = ctx->acc;
+    a.rho = ctx->rho;
+    a.u = ctx->u;
+    a.h = ctx->h;
+    a.du = ctx->du;
+    a.dhdt = ctx->dhdt;
+    a.active = ctx->active;
+    a.next_active = ctx->next_active;
+    a.time = ctx->time;
+    a.dtmin1 = p.dt0 / (float)(1 << p.r_max);
+    a.max_h_len = p.max_smoothing_length;
+    a.max_rungs = p.max_rungs;
+    const int threads = 256;
+    k_step<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+    SPH_HIP_TRY(hipGetLastError());
+    if (p.max_rungs > 0) {
+        k_advance_time<<<1, 1, 0, ctx->stream>>>(ctx->time);
+        SPH_HIP_TRY(hipGetLastError());
+    }
+    return SPH_OK;
+}

@@ -1,0 +1,38 @@
+/*
+ * params.cpp -- sph_params_default(): the reference's Parameters.h values
+ * (Parameters.h:11-53) as the runtime defaults.
+ */
+#include <cstring>
+
+#include "sph_capi.h"
+#include "Parameters.h"
+
+extern "C" void sph_params_default(sph_params* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->viscosity_alpha = VISCOSITY_ALPHA;
+    o->density_eta = DENSITY_ETA;
+    o->max_smoothing_length = MAX_SMOOTHING_LENGTH;
+    o->max_nr_iterations = MAX_DENSITY_NR_ITTERATIONS;
+    o->n_neighbours_estim = N_NEIGHBOURS_ESTIM;
+    o->resolution_eta = RESOLUTION_ETA;
+    o->eos_min_rho = ANEOS_MIN_RHO;
+    o->eos_max_rho = ANEOS_MAX_RHO;
+    o->eos_min_u = ANEOS_MIN_U;
+    o->eos_max_u = ANEOS_MAX_U;
+    o->cfl = CFL;
+    o->dt0 = DT0;
+    o->gamma = GAMMA;
+    o->r_max = R_MAX;
+    o->activity_rule = SPH_ACTIVE_AS_WRITTEN;
+    o->max_rungs = 0;
+    o->gravity_eta = GRAVITY_ETA;
+    o->gravity_g = GRAVITY_G;
+    o->gravity_softening = GRAVITY_SMOOTHING_LENGTH;
+    o->gravity_max_recursion = GRAVITY_MAX_RECURSION;
+    o->max_unchecked_pointers = MAX_UNCHECKED_POINTERS;
+    o->max_children_in_leaf = MAX_CHILDREN_IN_LEAF;
+    o->max_tree_depth = MAX_TREE_DEPTH;
+    o->cell_width = CELL_WIDTH;
+    o->cell_power = CELL_POWER;
+    o->fixes = 0;
+}

@@ -1,0 +1,235 @@
+/*
+ * sort.hip -- Morton hash, stable LSD radix sort of (cell key, index) pairs,
+ * and cell start/end, for gfx950.
+ *
+ * Reference: Compute::sort (Compute.cpp:265-326) running hash / hist / scan /
+ * sum / sort x4 / initialise / findCellPositions (sort.metal:15-148).
+ *
+ * The result is bit-identical to the reference: its LSD passes are stable
+ * and start from the identity permutation, so the sorted array is ordered by
+ * (key, index).  Any stable LSD sort over the significant digits gives the
+ * same array; keys hold 3*cell_power bits (24 at CELL_POWER 8), so
+ * ceil(3*cell_power/8) passes suffice and the reference's 4th pass (the
+ * identity) is skipped.
+ *
+ * Per pass, three kernels:
+ *   k_hash_hist / k_hist  per-tile digit histogram (tile = 2048 pairs), LDS atomics
+ *   k_scan                one workgroup: exclusive scan of the digit-major
+ *                         [256][ntiles] table = global destination of each
+ *                         (digit, tile) run
+ *   k_scatter             per tile, 8 rounds of 256 consecutive pairs; a
+ *                         round ranks its pairs stably with 8 wave ballots per
+ *                         digit bit (lanes with equal digit = one peer mask) and
+ *                         a per-wave LDS count, then scatters
+ * Cell start/end (sort.metal:131-148) come from run boundaries of the sorted
+ * keys instead of 2 atomics per particle, and instead of re-initialising all
+ * 2^24 cells (128 MiB) every step, only the cells occupied after the previous
+ * sort are reset -- the table content is the same.
+ */
+#include "sph_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kRounds = 8;
+constexpr int kTile = kThreads * kRounds;
+constexpr int kWaves = kThreads / 64;
+
+__device__ __forceinline__ uint32_t hash_key(float4 p, float cellSize, int cellsPerDim) {
+    /* sort.metal:20-43 */
+    const float nx = p.x / cellSize, ny = p.y / cellSize, nz = p.z / cellSize;
+    int px = (int)floorf(nx) % cellsPerDim;
+    int py = (int)floorf(ny) % cellsPerDim;
+    int pz = (int)floorf(nz) % cellsPerDim;
+    if (px < 0) px += cellsPerDim;
+    if (py < 0) py += cellsPerDim;
+    if (pz < 0) pz += cellsPerDim;
+    return dev_expand_bits((uint32_t)px) | (dev_expand_bits((uint32_t)py) << 1) | (dev_expand_bits((uint32_t)pz) << 2);
+}
+
+__global__ __launch_bounds__(kThreads) void k_hash_hist(const float4* __restrict__ pos, int n, float cellSize,
+                                                        int cellsPerDim, uint2* __restrict__ pairs,
+                                                        uint32_t* __restrict__ tile_hist, int ntiles) {
+    __shared__ uint32_t hist[256];
+    const int t = blockIdx.x;
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = t * kTile + r * kThreads + threadIdx.x;
+        if (i < n) {
+            const uint32_t key = hash_key(pos[i], cellSize, cellsPerDim);
+            pairs[i] = make_uint2(key, (uint32_t)i);
+            atomicAdd(&hist[key & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    tile_hist[threadIdx.x * ntiles + t] = hist[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kThreads) void k_hist(const uint2* __restrict__ pairs, int n, int shift,
+                                                   uint32_t* __restrict__ tile_hist, int ntiles) {
+    __shared__ uint32_t hist[256];
+    const int t = blockIdx.x;
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = t * kTile + r * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&hist[(pairs[i].x >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    tile_hist[threadIdx.x * ntiles + t] = hist[threadIdx.x];
+}
+
+/* Exclusive scan of `count` entries in place, one workgroup of 1024. */
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ a, int count) {
+    __shared__ uint32_t part[1024];
+    const int tid = threadIdx.x;
+    const int per = (count + 1023) / 1024;
+    const int b = tid * per;
+    const int e = min(b + per, count);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += a[i];
+    part[tid] = s;
+    __syncthreads();
+    /* Hillis-Steele inclusive scan over 1024 partial sums */
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = tid == 0 ? 0u : part[tid - 1];
+    for (int i = b; i < e; ++i) {
+        const uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scatter(const uint2* __restrict__ src, uint2* __restrict__ dst, int n,
+                                                      int shift, const uint32_t* __restrict__ tile_off, int ntiles) {
+    __shared__ uint32_t goff[256];
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wave_cnt[kWaves][256];
+    __shared__ uint32_t wave_off[kWaves][256];
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    goff[tid] = tile_off[tid * ntiles + t];
+    running[tid] = 0;
+    for (int w = 0; w < kWaves; ++w) wave_cnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = t * kTile + r * kThreads + tid;
+        const bool valid = i < n;
+        uint2 pr = valid ? src[i] : make_uint2(0u, 0u);
+        const uint32_t d = (pr.x >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool set = (d >> bit) & 1u;
+            const uint64_t m = __ballot(set);
+            peers &= set ? m : ~m;
+        }
+        const uint32_t lane_rank = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && lane_rank == 0) wave_cnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t base = running[tid];
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                wave_off[w][tid] = base;
+                base += wave_cnt[w][tid];
+                wave_cnt[w][tid] = 0;
+            }
+            running[tid] = base;
+        }
+        __syncthreads();
+        if (valid) dst[goff[d] + wave_off[wave][d] + lane_rank] = pr;
+    }
+}
+
+/* Resets the cells that the previous sort occupied (sparse form of
+ * `initialise`, sort.metal:131-138). */
+__global__ void k_cells_reset(const uint2* __restrict__ prev, int n, uint2* __restrict__ cell_se) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = prev[i].x;
+    if (i == 0 || prev[i - 1].x != key) cell_se[key] = make_uint2(0xFFFFFFFFu, 0u);
+}
+
+/* findCellPositions (sort.metal:140-148) from run boundaries, fused with the
+ * gather of the Morton-ordered copies used by the neighbour passes. */
+__global__ void k_cells_gather(const uint2* __restrict__ sorted, int n, uint2* __restrict__ cell_se,
+                               const float4* __restrict__ pos, const float4* __restrict__ vel,
+                               const float* __restrict__ m, float4* __restrict__ s_posm, float4* __restrict__ s_vel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint2 pr = sorted[i];
+    const uint32_t key = pr.x;
+    if (i == 0 || sorted[i - 1].x != key) cell_se[key].x = (uint32_t)i;
+    if (i == n - 1 || sorted[i + 1].x != key) cell_se[key].y = (uint32_t)i;
+    const uint32_t j = pr.y;
+    const float4 p = pos[j];
+    const float4 v = vel[j];
+    s_posm[i] = make_float4(p.x, p.y, p.z, m[j]);
+    s_vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+}
+
+__global__ void k_fill_cells(uint2* __restrict__ cell_se, int ncells) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncells) cell_se[i] = make_uint2(0xFFFFFFFFu, 0u);
+}
+
+}  // namespace
+
+int launch_fill_cells(sph_ctx* ctx) {
+    const int threads = 256;
+    k_fill_cells<<<(ctx->ncells + threads - 1) / threads, threads, 0, ctx->stream>>>(ctx->cell_se, ctx->ncells);
+    SPH_HIP_TRY(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_sort(sph_ctx* ctx) {
+    const int n = ctx->n;
+    const int ntiles = (n + kTile - 1) / kTile;
+    if (ntiles > ctx->ntiles) return ctx_fail(ctx, SPH_E_STATE, "sort tile table too small");
+    const int bits = 3 * ctx->prm.cell_power;
+    const int passes = (bits + 7) / 8;
+    const int threads = 256;
+    const int blocks = (n + threads - 1) / threads;
+    hipStream_t s = ctx->stream;
+    if (ctx->have_prev) {
+        k_cells_reset<<<blocks, threads, 0, s>>>(ctx->prev_keys_src, n, ctx->cell_se);
+        SPH_HIP_TRY(hipGetLastError());
+    }
+    uint2* bufs[2] = {ctx->pairs_a, ctx->pairs_b};
+    k_hash_hist<<<ntiles, kThreads, 0, s>>>(ctx->pos, n, ctx->prm.cell_width, 1 << ctx->prm.cell_power, bufs[0],
+                                            ctx->tile_hist, ntiles);
+    SPH_HIP_TRY(hipGetLastError());
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        if (p > 0) {
+            k_hist<<<ntiles, kThreads, 0, s>>>(bufs[cur], n, shift, ctx->tile_hist, ntiles);
+            SPH_HIP_TRY(hipGetLastError());
+        }
+        k_scan<<<1, 1024, 0, s>>>(ctx->tile_hist, 256 * ntiles);
+        SPH_HIP_TRY(hipGetLastError());
+        k_scatter<<<ntiles, kThreads, 0, s>>>(bufs[cur], bufs[cur ^ 1], n, shift, ctx->tile_hist, ntiles);
+        SPH_HIP_TRY(hipGetLastError());
+        cur ^= 1;
+    }
+    uint2* sorted = bufs[cur];
+    k_cells_gather<<<blocks, threads, 0, s>>>(sorted, n, ctx->cell_se, ctx->pos, ctx->vel, ctx->m, ctx->s_posm,
+                                              ctx->s_vel);
+    SPH_HIP_TRY(hipGetLastError());
+    /* keep the sorted array where the next sort will not overwrite it before
+     * the reset: swap so that `pairs_a` is always the current result */
+    if (cur == 1) std::swap(ctx->pairs_a, ctx->pairs_b);
+    ctx->prev_keys_src = ctx->pairs_a;
+    ctx->have_prev = true;
+    return SPH_OK;
+}

@@ -1,0 +1,156 @@
+/*
+ * sph_internal.h -- device memory layout of one simulation context and the
+ * launchers shared between the translation units of libsphhip.so.
+ *
+ * Layout in HBM (n particles, C = 2^(3*cell_power) cells, K nodes):
+ *   particle state, original-ID order (the order the caller sees):
+ *     pos, vel, acc            float4 x n   (xyz + unused w, 16-B stride)
+ *     rho u m P h f c du B dhdt gravnorm   float x n
+ *     mat, next_active, nr_iters            int32 x n;  active uint8 x n
+ *   cells: pairs_a / pairs_b  uint2 x n (sorted (key, index), ping-pong)
+ *          cell_se            uint2 x C (first slot, last slot), 128 MiB at C = 2^24
+ *   Morton-ordered copies written once per pass and read by every
+ *   neighbour candidate (slot k = k-th sorted particle):
+ *          s_posm  float4 (x, y, z, m)        -- written after the sort
+ *          s_vel   float4 (vx, vy, vz, 0)     -- written after the sort
+ *          s_hyd   float4 (h, f P / rho^2, rho, c) -- written at the start of the acceleration pass
+ *          s_bal   float  (Balsara factor)          -- written at the start of the acceleration pass
+ *   tree (node id = pre-order node number):
+ *          mp_far  3 x float4 x K  (pos, size) (M, XX, XY, XZ) (YY, YZ, ZZ, power2)
+ *          mp_aux  3 x float4 x K  (min, minGrav) (max, active) (X, Y, Z, -) dipoles
+ *          loc     4 x float4 x K  (pos, L_M) (L_X, L_Y, L_Z, L_XX) (L_XY, L_XZ, L_YY, L_YZ) (L_ZZ, -, -, -)
+ *          node_info int4 x K (nParticles, treePointer, level, -), parent_slot int32 x K
+ *          node_child int32 x 8K (child node ids, -1)
+ *          lists a/b int32 x (max nodes of a non-last level) x max_unchecked_pointers
+ */
+#ifndef SPH_INTERNAL_H
+#define SPH_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sph_capi.h"
+
+#define SPH_HIP_TRY(expr)                                              \
+    do {                                                               \
+        hipError_t _e = (expr);                                        \
+        if (_e != hipSuccess) return ctx_fail_hip(ctx, _e, #expr);     \
+    } while (0)
+
+struct sph_tree_dev {
+    int32_t* octree = nullptr;      /* raw octreeData (current) */
+    int32_t* octree_up[2] = {nullptr, nullptr}; /* upload double buffer */
+    int64_t octree_cap = 0;         /* ints */
+    int32_t* level_tp = nullptr;    /* concatenated treeLevels (tree offsets, current) */
+    int32_t* level_tp_up[2] = {nullptr, nullptr};
+    int up_cur = 0;                 /* buffer holding the current tree */
+    hipEvent_t released[2] = {nullptr, nullptr}; /* compute stream done with buffer k */
+    int32_t* level_id = nullptr;    /* same slots, node ids */
+    int64_t level_cap = 0;
+    int4* node_info = nullptr;      /* (nParticles, treePointer, level, 0) */
+    int32_t* parent_slot = nullptr; /* slot of the parent in its level list (parentIndexes) */
+    int32_t* parent_level = nullptr;/* level of the parent (validation) */
+    int32_t* seen = nullptr;        /* times each node id occurs in the level lists (validation) */
+    int64_t n_ints = 0;             /* size of the current octreeData */
+    int32_t* node_child = nullptr;  /* 8 per node */
+    float4* mp_far = nullptr;       /* 3 per node */
+    float4* mp_aux = nullptr;       /* 3 per node */
+    float4* loc = nullptr;          /* 4 per node */
+    int64_t node_cap = 0;
+    int32_t node_count = 0;
+    std::vector<int32_t> level_counts, level_offsets;
+    int32_t max_nonlast = 0;        /* most nodes in any level but the last */
+};
+
+struct sph_ctx {
+    sph_params prm{};
+    int n = 0;
+    int ncells = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;
+    std::string err;
+
+    /* particle state, original order */
+    float4 *pos = nullptr, *vel = nullptr, *acc = nullptr;
+    float *rho = nullptr, *u = nullptr, *m = nullptr, *P = nullptr, *h = nullptr;
+    float *f = nullptr, *c = nullptr, *du = nullptr, *B = nullptr, *dhdt = nullptr, *gravnorm = nullptr;
+    int32_t *mat = nullptr, *next_active = nullptr, *nr_iters = nullptr;
+    uint8_t* active = nullptr;
+    int32_t* time = nullptr; /* [0] globalTime, [1] dt */
+
+    /* cells and sort */
+    uint2 *pairs_a = nullptr, *pairs_b = nullptr; /* sorted result always ends in pairs_a */
+    uint2* prev_keys_src = nullptr;               /* sorted pairs of the previous sort (for the sparse reset) */
+    bool have_prev = false;
+    uint2* cell_se = nullptr;
+    uint32_t* tile_hist = nullptr; /* 256 x ntiles, digit-major */
+    int ntiles = 0;
+
+    /* Morton-ordered copies */
+    float4 *s_posm = nullptr, *s_vel = nullptr, *s_hyd = nullptr;
+    float* s_bal = nullptr;
+
+    /* EOS tables: slot 0 = material 400, slot 1 = every other material */
+    float2* eos[2] = {nullptr, nullptr};
+    int eos_res = 0;
+
+    /* tree */
+    sph_tree_dev tree;
+    bool have_tree = false;
+    int32_t *list_a = nullptr, *list_b = nullptr;
+    int64_t list_cap = 0; /* entries per list buffer */
+    uint32_t* grav_flags = nullptr; /* [0] != 0: the last uploaded tree failed validation */
+    bool tree_check_pending = false;
+
+    /* octree handoff */
+    float4* snap = nullptr;
+    hipEvent_t snap_done = nullptr;
+    hipEvent_t upload_done = nullptr;
+    bool upload_pending = false;
+
+    /* timing: event pairs recorded around each pass, folded on query */
+    struct TimedPass {
+        int pass;
+        hipEvent_t start, stop;
+    };
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<TimedPass> ev_pending;
+    hipEvent_t ev_open[SPH_PASS_COUNT] = {};
+    double pass_ms[SPH_PASS_COUNT] = {};
+    int64_t pass_calls[SPH_PASS_COUNT] = {};
+};
+
+int ctx_fail(sph_ctx* ctx, int code, const std::string& msg);
+int ctx_fail_hip(sph_ctx* ctx, hipError_t e, const char* what);
+int ctx_begin_pass(sph_ctx* ctx, int pass);
+int ctx_end_pass(sph_ctx* ctx, int pass);
+int ctx_fold_times(sph_ctx* ctx);
+
+/* sort.hip */
+int launch_sort(sph_ctx* ctx);
+/* hydro.hip */
+int launch_density(sph_ctx* ctx);
+int launch_acceleration(sph_ctx* ctx);
+int launch_step(sph_ctx* ctx);
+/* gravity.hip */
+int launch_tree_convert(sph_ctx* ctx);
+int launch_gravity_up(sph_ctx* ctx);
+int launch_gravity_down(sph_ctx* ctx);
+int launch_fill_cells(sph_ctx* ctx);
+
+/* ---------------------------------------------------------- device helpers */
+/* Morton interleave of three <= 10-bit values (morton.h:16-34). */
+__device__ __forceinline__ uint32_t dev_expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+#endif

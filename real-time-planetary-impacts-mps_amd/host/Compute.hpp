@@ -1,0 +1,128 @@
+/*
+ * Compute.hpp -- host-side owner of one simulation, with the reference's
+ * method names and call contract (SPH/Compute/Compute.hpp:40-148).
+ *
+ * The reference's Compute owns Metal buffers and encodes each pass into a
+ * command buffer.  This one owns an sph_ctx (device memory + HIP stream) and
+ * enqueues each pass through the C ABI; nothing waits until Headless::step
+ * calls sph_sync, as the reference waits once per frame.
+ *
+ * The octree is still built on a CPU thread (Compute.cpp:456-516), with two
+ * differences: positions reach the thread through an explicit pinned
+ * snapshot (the reference reads the shared Metal buffer while the GPU may be
+ * writing it), and the hand-off uses atomics instead of a plain bool.
+ */
+#ifndef SPH_HOST_COMPUTE_HPP
+#define SPH_HOST_COMPUTE_HPP
+
+#include <atomic>
+#include <cstdint>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "sph_capi.h"
+#include "sph_host.h"
+
+/* Initial conditions (hdfHandler.hpp:15-26); 3-vectors with a 4-float stride. */
+struct DataStruct {
+    std::vector<float> positions;
+    std::vector<float> velocities;
+    std::vector<float> densities;
+    std::vector<float> entropies;
+    std::vector<float> internalEnergy;
+    std::vector<float> masses;
+    std::vector<int32_t> materialIDs;
+    std::vector<float> potentials;
+    std::vector<float> pressures;
+    std::vector<float> smoothingLengths;
+};
+
+/* One EOS table per material slot (ANEOSTable, ANEOS.hpp:16-29). */
+struct EOSTable {
+    int32_t materialId;
+    int32_t resolution;
+    std::vector<float> data; /* (P, c) pairs, row = density sample */
+};
+
+class Compute {
+public:
+    /* Compute::Compute + loadInitialConditions (Compute.cpp:22-84): uploads
+     * the state and the tables and builds the first tree synchronously. */
+    Compute(const sph_params& params, const DataStruct& data, const std::vector<EOSTable>& tables, int treeMode,
+            int treeThreads);
+    ~Compute();
+    Compute(const Compute&) = delete;
+    Compute& operator=(const Compute&) = delete;
+
+    /* Compute.cpp:456-508: swap in a finished tree, start the next build. */
+    void updateOctreeBuffer();
+    void sort();               /* Compute.cpp:265-326 */
+    void gravitationalPass();  /* Compute.cpp:328-377 */
+    void densityPass();        /* Compute.cpp:179-208 */
+    void accelerationPass();   /* Compute.cpp:210-240 */
+    void stepPass();           /* Compute.cpp:242-263 */
+
+    int nParticles = 0;
+    sph_ctx* ctx = nullptr;
+
+    /* first failing call (0 = none) and its message */
+    int status() const { return status_; }
+    const std::string& error() const { return error_; }
+    int64_t treesSwapped() const { return treesSwapped_; }
+    double lastBuildMs() const { return lastBuildMs_.load(); }
+
+private:
+    void check(int rc, const char* what);
+    void buildAndUpload(const float* pos4);
+    void updateOctreeData(); /* body of the octree thread (Compute.cpp:510-516) */
+    void ensureStaging(int64_t nInts, int64_t nNodes);
+
+    sph_params params_{};
+    int treeMode_ = SPH_TREE_ASYNC;
+    int treeThreads_ = 1;
+    int status_ = 0;
+    std::string error_;
+
+    /* octree hand-off */
+    float* snapshot_ = nullptr;       /* pinned, 4 floats per particle */
+    int32_t* stageOctree_ = nullptr;  /* pinned staging of octreeData */
+    int32_t* stageLevels_ = nullptr;  /* pinned staging of the level lists */
+    int64_t stageOctreeCap_ = 0, stageLevelsCap_ = 0;
+    std::vector<int32_t> stageCounts_;
+    int64_t stageInts_ = 0;
+    int32_t stageNodes_ = 0;
+    int stageBuildRc_ = 0;
+    std::thread builder_;
+    std::atomic<bool> updating_{false};
+    std::atomic<bool> treeReady_{false};
+    std::atomic<double> lastBuildMs_{0.0};
+    bool haveTree_ = false;
+    int64_t treesSwapped_ = 0;
+};
+
+/* Headless driver (Headless.cpp:19-55). */
+class Headless {
+public:
+    Headless(const sph_params& params, const sph_impact_config& config, int eosResolution, int treeMode,
+             int treeThreads);
+    ~Headless();
+    /* updateOctreeBuffer, sort, gravity, density, acceleration, step, wait */
+    void step();
+    /* the same sequence without the final wait */
+    void enqueueStep();
+
+    Compute* compute = nullptr;
+    int status() const;
+    std::string error() const;
+
+private:
+    int status_ = 0;
+    std::string error_;
+};
+
+/* Synthetic inputs of the configs. */
+std::vector<EOSTable> makeEOSTables(const sph_params& params, int resolution);
+DataStruct makeImpact(const sph_params& params, const sph_impact_config& config, const std::vector<EOSTable>& tables);
+
+#endif

@@ -1,0 +1,246 @@
+"""GPU parity: every pass of the HIP path against the CPU oracle on the same
+state, through the C ABI.
+
+Bars (stated per test): bit-exact for the sort and cell tables; FP32
+tolerance for density, acceleration, step and gravity.  Density also
+classifies Newton-Raphson "flips" -- particles whose iteration count differs
+because a successive-iterate change sits within rounding of DENSITY_ETA --
+and bounds how many there may be.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(sph, orc, ic, tables, **prm_over):
+    prm = sph.default_params(**prm_over)
+    ctx = sph.Context(prm, ic, tables)
+    st = orc.State(prm, ic, tables)
+    return prm, ctx, st
+
+
+def _rel(a, b, floor=0.0):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+# ------------------------------------------------------------------- sort
+@pytest.mark.parametrize("n,seed", [(2000, 5), (20000, 1), (100000, 2)])
+def test_sort_and_cells_bit_exact(sph, orc, tables256, impact, n, seed):
+    ic = impact(n, seed)
+    prm, ctx, st = _pair(sph, orc, ic, tables256)
+    print(f"[sort] n={n} seed={seed}", flush=True)
+    for rnd in range(3):
+        ctx.sort()
+        st.sort()
+        assert np.array_equal(ctx.download("cell_array"), st["cell_array"]), f"cell_array round {rnd}"
+        assert np.array_equal(ctx.download("cell_start"), st["cell_start"]), f"cell_start round {rnd}"
+        assert np.array_equal(ctx.download("cell_end"), st["cell_end"]), f"cell_end round {rnd}"
+        # move the particles so the next sort resets and fills different cells
+        rng = np.random.default_rng(seed + rnd)
+        pos = st["pos"].copy()
+        pos[:, :3] += rng.normal(0, 0.3, size=(n, 3)).astype(np.float32)
+        st["pos"] = pos
+        ctx.upload("pos", pos)
+
+
+def test_sort_negative_and_wrapped_coordinates(sph, orc, tables256, impact):
+    """Positions beyond the 46.08 Mm periodic cell grid and on both sides of 0."""
+    ic = impact(5000, 9)
+    ic["pos"][:, :3] = ic["pos"][:, :3] * 5.0 + np.float32(-40.0)
+    ic["pos"][::7, 0] += np.float32(46.08 * 2)
+    prm, ctx, st = _pair(sph, orc, ic, tables256)
+    ctx.sort()
+    st.sort()
+    assert np.array_equal(ctx.download("cell_array"), st["cell_array"])
+    assert np.array_equal(ctx.download("cell_start"), st["cell_start"])
+    assert np.array_equal(ctx.download("cell_end"), st["cell_end"])
+
+
+# ---------------------------------------------------------------- density
+def _density_compare(ctx, st, n, label):
+    nr_g = ctx.download("nr_iters")
+    nr_c = st["nr_iters"]
+    same = nr_g == nr_c
+    flips = int((~same).sum())
+    out = {}
+    for f in ("h", "rho", "gradterm", "pressure", "soundspeed"):
+        r = _rel(ctx.download(f), st[f], 1e-30)[same]
+        out[f] = (float(np.max(r)), float(np.median(r)))
+    b = np.abs(ctx.download("balsara").astype(np.float64) - st["balsara"])[same]
+    out["balsara_abs"] = (float(b.max()), float(np.median(b)))
+    print(f"[density {label}] n={n} flips={flips} stats={out}", flush=True)
+    return flips, out
+
+
+@pytest.mark.parametrize("n,seed", [(20000, 1), (100000, 2)])
+def test_density_parity(sph, orc, tables256, impact, n, seed):
+    ic = impact(n, seed)
+    prm, ctx, st = _pair(sph, orc, ic, tables256)
+    ctx.sort()
+    st.sort()
+    ctx.density()
+    st.density()
+    flips, s = _density_compare(ctx, st, n, "ic")
+    assert flips <= max(2, n // 1000)
+    assert s["h"][0] < 1e-4 and s["rho"][0] < 1e-4 and s["gradterm"][0] < 1e-3
+    assert s["pressure"][0] < 1e-3 and s["soundspeed"][0] < 1e-3
+    assert s["balsara_abs"][0] < 1e-3
+    assert int(ctx.download("dt")[0]) == int(st["dt"][0]) == 65536
+
+
+def test_density_cache_paths(sph, orc, tables256, impact):
+    """Particles at the h cap (h = 1) take the reference's neighbour-cache path
+    inside the NR loop; a small cache forces the rescan (and its double count)."""
+    ic = impact(20000, 3)
+    ic["h"][::5] = 1.0
+    for ne in (128, 24):
+        prm, ctx, st = _pair(sph, orc, ic, tables256, n_neighbours_estim=ne)
+        ctx.sort()
+        st.sort()
+        ctx.density()
+        st.density()
+        flips, s = _density_compare(ctx, st, 20000, f"ne={ne}")
+        assert flips <= 20
+        assert s["h"][0] < 1e-4 and s["rho"][0] < 1e-4
+
+
+# ----------------------------------------------------------- acceleration
+@pytest.mark.parametrize("n,seed", [(20000, 1), (100000, 2)])
+def test_acceleration_parity(sph, orc, tables256, impact, n, seed):
+    ic = impact(n, seed)
+    prm, ctx, st = _pair(sph, orc, ic, tables256)
+    st.sort()
+    st.density()
+    ctx.sort()
+    for f in ("h", "rho", "pressure", "gradterm", "soundspeed", "balsara", "dt"):
+        ctx.upload(f, st[f])
+    st["acc"] = 0.0
+    ctx.upload("acc", st["acc"])
+    ctx.acceleration()
+    st.acceleration()
+    ag, ac = ctx.download("acc")[:, :3].astype(np.float64), st["acc"][:, :3].astype(np.float64)
+    scale = np.sqrt(np.mean(np.sum(ac ** 2, axis=1)))
+    err = np.linalg.norm(ag - ac, axis=1) / scale
+    du_s = np.sqrt(np.mean(st["du"].astype(np.float64) ** 2))
+    dh_s = np.sqrt(np.mean(st["dhdt"].astype(np.float64) ** 2))
+    e_du = np.abs(ctx.download("du") - st["du"]) / du_s
+    e_dh = np.abs(ctx.download("dhdt") - st["dhdt"]) / dh_s
+    na_same = np.mean(ctx.download("next_active") == st["next_active"])
+    print(f"[accel] n={n} acc max={err.max():.3e} med={np.median(err):.3e} du max={e_du.max():.3e} "
+          f"dhdt max={e_dh.max():.3e} next_active same={na_same:.6f} dt={ctx.download('dt')[0]}/{st['dt'][0]}",
+          flush=True)
+    assert err.max() < 1e-4 and e_du.max() < 1e-4 and e_dh.max() < 1e-4
+    assert na_same > 0.999
+    assert int(ctx.download("dt")[0]) == int(st["dt"][0])
+
+
+# ------------------------------------------------------------------- step
+def test_step_parity(sph, orc, tables256, impact):
+    n = 20000
+    ic = impact(n, 1)
+    prm, ctx, st = _pair(sph, orc, ic, tables256)
+    st.sort()
+    st.density()
+    st.acceleration()
+    for f in ("pos", "vel", "acc", "rho", "u", "du", "dhdt", "h", "dt", "active", "next_active"):
+        ctx.upload(f, st[f])
+    ctx.step()
+    st.step()
+    for f in ("pos", "vel", "rho", "u", "h"):
+        r = _rel(ctx.download(f), st[f], 1e-12)
+        print(f"[step] {f} max rel {r.max():.3e}", flush=True)
+        assert r.max() < 1e-5, f
+    assert int(ctx.download("global_time")[0]) == int(st["global_time"][0])
+
+
+# ---------------------------------------------------------------- gravity
+def _grav_compare(ctx, st, label):
+    mg = ctx.download("multipoles")
+    mc = st.grav["multipoles"]
+    mstat = {}
+    for name, cols in (("pos", [0, 1, 2]), ("M", [12]), ("quad", [16, 17, 18, 19, 20, 21]), ("size", [3])):
+        scale = np.abs(mc[:, cols]).max(axis=1, keepdims=True) + 1e-30
+        mstat[name] = float((np.abs(mg[:, cols] - mc[:, cols]) / scale).max())
+    act_same = bool(np.array_equal(mg[:, 11], mc[:, 11]))
+    ag, ac = ctx.download("acc")[:, :3].astype(np.float64), st["acc"][:, :3].astype(np.float64)
+    mag = np.linalg.norm(ac, axis=1)
+    err = np.linalg.norm(ag - ac, axis=1) / np.maximum(mag, 1e-30)
+    q = np.quantile(err, [0.5, 0.99, 0.999, 1.0])
+    print(f"[gravity {label}] multipoles {mstat} active_same={act_same} acc rel err q50/99/99.9/max={q}",
+          flush=True)
+    return mstat, act_same, q
+
+
+@pytest.mark.parametrize("n,seed", [(20000, 1), (100000, 2)])
+def test_gravity_parity(sph, orc, tables256, impact, n, seed):
+    ic = impact(n, seed)
+    prm, ctx, st = _pair(sph, orc, ic, tables256)
+    tree = orc.build_octree(ic["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+    st.set_tree(tree)
+    ctx.upload_tree(tree)
+    # first step: gravNorm = 0, every MAC test rejects (the reference's step 1)
+    st.gravity()
+    ctx.gravity()
+    mstat, act_same, q = _grav_compare(ctx, st, "step1")
+    assert act_same and mstat["M"] < 1e-5 and mstat["pos"] < 1e-5 and mstat["quad"] < 1e-4
+    assert q[1] < 1e-3
+    # second evaluation with the first's |a| as minGrav (the MAC is live)
+    ctx.upload("gravnorm", st["gravnorm"])
+    st.gravity()
+    ctx.gravity()
+    mstat, act_same, q = _grav_compare(ctx, st, "warm")
+    assert act_same and mstat["quad"] < 1e-4
+    assert q[0] < 1e-5 and q[2] < 1e-2
+
+
+def test_gravity_small_list(sph, orc, tables256, impact):
+    """A tiny MAX_UNCHECKED_POINTERS forces the list-full accept path."""
+    ic = impact(20000, 4)
+    prm, ctx, st = _pair(sph, orc, ic, tables256, max_unchecked_pointers=16)
+    tree = orc.build_octree(ic["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+    st.set_tree(tree)
+    ctx.upload_tree(tree)
+    st.gravity()
+    ctx.upload("gravnorm", st["gravnorm"])
+    st.gravity()
+    ctx.gravity()
+    _, act_same, q = _grav_compare(ctx, st, "maxu16")
+    assert act_same and q[0] < 1e-5 and q[2] < 1e-2
+
+
+# ------------------------------------------------------- whole steps
+def test_headless_steps_match_oracle(sph, orc, tables256, impact):
+    """Five Headless steps (tree rebuilt before every step) against the oracle."""
+    n = 20000
+    prm = sph.default_params()
+    cfg = sph.impact_config(n, 1)
+    hl = sph.Headless(prm, cfg, eos_res=256, tree_mode=sph.TREE_EVERY, tree_threads=4)
+    st = orc.State(prm, impact(n, 1), tables256)
+    for s in range(5):
+        hl.step()
+        tree = orc.build_octree(st["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+        st.full_step(tree)
+        pg = hl.ctx.download("pos")[:, :3]
+        pc = st["pos"][:, :3]
+        d = np.abs(pg - pc).max()
+        print(f"[headless] step {s} max |dx| = {d:.3e} gt={hl.ctx.download('global_time')[0]}/{st['global_time'][0]}",
+              flush=True)
+    assert int(hl.ctx.download("global_time")[0]) == int(st["global_time"][0])
+    assert d < 1e-4
+
+
+def test_gpu_runs_are_deterministic(sph, impact, tables256):
+    n = 20000
+    prm = sph.default_params()
+    outs = []
+    for _ in range(2):
+        hl = sph.Headless(prm, sph.impact_config(n, 1), eos_res=256, tree_mode=sph.TREE_EVERY, tree_threads=2)
+        for _ in range(3):
+            hl.step()
+        outs.append((hl.ctx.download("pos"), hl.ctx.download("h"), hl.ctx.download("acc")))
+        hl.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
