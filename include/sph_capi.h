@@ -125,7 +125,9 @@ void sph_host_free(void* p);
  * (n x 4 floats) on the copy stream.  Replaces the CPU read of the shared
  * position buffer in Compute::updateOctreeData (Compute.cpp:511). */
 int sph_snapshot_positions(sph_ctx* ctx, float* pinned_dst4);
-/* Blocks the calling thread until the last snapshot has landed in host memory. */
+/* Blocks the calling thread until the last snapshot has landed in host memory.
+ * The one call that may come from another thread (the octree builder) while
+ * the owning thread keeps enqueuing passes. */
 int sph_wait_snapshot(sph_ctx* ctx);
 
 /* Compute::updateOctreeBuffer (Compute.cpp:456-508): takes a tree in the
@@ -176,6 +178,9 @@ enum sph_field {
     SPH_F_NR_ITERS,       /* int32 x n: NR iterations of the last density pass (diagnostic) */
     SPH_F_MULTIPOLES,     /* float x 24 x node_count, see below */
     SPH_F_LOCALS,         /* float x 16 x node_count, see below */
+    SPH_F_DENSITY_WORK,   /* int32 x 4n: neighbour candidates examined and candidates inside 2h
+                             over the NR scans, then the same two counts for the final pass,
+                             of the last density pass (diagnostic) */
     SPH_F_COUNT
 };
 /* Multipole record (24 floats, poles.h:46-55 reordered):

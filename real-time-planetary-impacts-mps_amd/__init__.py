@@ -35,7 +35,7 @@ FIELDS = [
     "pos", "vel", "acc", "rho", "u", "mass", "pressure", "h", "mat", "gradterm",
     "soundspeed", "du", "balsara", "dhdt", "active", "next_active", "gravnorm",
     "cell_array", "cell_start", "cell_end", "global_time", "dt", "nr_iters",
-    "multipoles", "locals",
+    "multipoles", "locals", "density_work",
 ]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 _F4 = {"pos", "vel", "acc"}
@@ -334,6 +334,8 @@ class Context:
             return (nbytes // 64, 16), np.float32
         if field == "cell_array":
             return (self.n, 2), np.uint32
+        if field == "density_work":
+            return (self.n, 4), np.int32
         if field in _U32:
             return (nbytes // 4,), np.uint32
         if field in _I32:

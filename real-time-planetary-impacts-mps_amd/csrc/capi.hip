@@ -73,6 +73,7 @@ void destroy_ctx(sph_ctx* ctx) {
     int32_t** i1[] = {&ctx->mat, &ctx->next_active, &ctx->nr_iters, &ctx->time, &ctx->list_a, &ctx->list_b};
     for (auto p : i1) dfree(*p);
     dfree(ctx->active);
+    dfree(ctx->density_work);
     dfree(ctx->pairs_a);
     dfree(ctx->pairs_b);
     dfree(ctx->cell_se);
@@ -109,6 +110,7 @@ int64_t field_bytes(const sph_ctx* ctx, int field) {
         case SPH_F_GLOBAL_TIME: case SPH_F_DT: return 4;
         case SPH_F_MULTIPOLES: return ctx->have_tree ? 24 * 4 * (int64_t)ctx->tree.node_count : 0;
         case SPH_F_LOCALS: return ctx->have_tree ? 16 * 4 * (int64_t)ctx->tree.node_count : 0;
+        case SPH_F_DENSITY_WORK: return 16 * n;
         default: return -1;
     }
 }
@@ -133,6 +135,7 @@ void* field_ptr(sph_ctx* ctx, int field) {
         case SPH_F_NEXT_ACTIVE: return ctx->next_active;
         case SPH_F_GRAVNORM: return ctx->gravnorm;
         case SPH_F_NR_ITERS: return ctx->nr_iters;
+        case SPH_F_DENSITY_WORK: return ctx->density_work;
         case SPH_F_CELL_ARRAY: return ctx->pairs_a;
         case SPH_F_GLOBAL_TIME: return ctx->time;
         case SPH_F_DT: return ctx->time + 1;
@@ -245,6 +248,8 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->next_active, n));
     TRY(dalloc(&ctx->nr_iters, n));
     TRY(dalloc(&ctx->active, n));
+    TRY(dalloc(&ctx->density_work, n));
+    TRY(hipMemset(ctx->density_work, 0, 16 * n));
     TRY(dalloc(&ctx->time, 4));
     TRY(dalloc(&ctx->pairs_a, n));
     TRY(dalloc(&ctx->pairs_b, n));
@@ -559,7 +564,8 @@ int sph_upload(sph_ctx* ctx, int field, const void* src) {
     const int64_t bytes = field_bytes(ctx, field);
     if (bytes < 0) return ctx_fail(ctx, SPH_E_ARG, "unknown field");
     if (field == SPH_F_CELL_ARRAY || field == SPH_F_CELL_START || field == SPH_F_CELL_END ||
-        field == SPH_F_MULTIPOLES || field == SPH_F_LOCALS || field == SPH_F_NR_ITERS)
+        field == SPH_F_MULTIPOLES || field == SPH_F_LOCALS || field == SPH_F_NR_ITERS ||
+        field == SPH_F_DENSITY_WORK)
         return ctx_fail(ctx, SPH_E_ARG, "derived field cannot be uploaded");
     void* dst = field_ptr(ctx, field);
     if (!dst) return ctx_fail(ctx, SPH_E_ARG, "field not uploadable");

@@ -120,6 +120,7 @@ struct DensityArgs {
     const float* u;
     float *rho, *h, *P, *c, *f, *B;
     int32_t* nr_iters;
+    int4* work;
     int32_t* time;
     const float2* eos0;
     const float2* eos1;
@@ -144,6 +145,7 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
         a.P[ind] = pc.x;
         a.c[ind] = pc.y;
         a.nr_iters[ind] = 0;
+        a.work[ind] = make_int4(0, 0, 0, 0);
         return;
     }
     const float4 pm = a.s_posm[index];
@@ -164,6 +166,7 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     int nN = 0;             /* cached-neighbour count of the last scan (capped) */
     float h_scan = h_i;     /* h of the last scan */
     float hs2x4 = h2x4;     /* its 4 h^2 */
+    int w_cand = 0, w_in = 0, f_cand = 0, f_in = 0;
     while (eps > a.eta && count < a.max_it) {
         density = 0.f;
         const bool use_cache = (h_i < (float)max_h) && (nN < a.ne);
@@ -181,11 +184,13 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
                     const uint32_t key = exy | (dev_expand_bits((uint32_t)(z & mask)) << 2);
                     const uint2 se = cse[key];
                     if (se.x == 0xFFFFFFFFu) continue;
+                    w_cand += (int)(se.y - se.x + 1);
                     for (uint32_t k = se.x; k <= se.y; k++) {
                         const float4 q = spm[k];
                         const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
                         const float r2 = dx * dx + dy * dy + dz * dz;
                         if (r2 > hb2) continue;
+                        w_in++;
                         const float r = sqrtf(r2);
                         if (use_cache) {
                             if (r > hsum) continue;
@@ -235,11 +240,13 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
                     const uint32_t key = exy | (dev_expand_bits((uint32_t)(z & mask)) << 2);
                     const uint2 se = cse[key];
                     if (se.x == 0xFFFFFFFFu) continue;
+                    f_cand += (int)(se.y - se.x + 1);
                     for (uint32_t k = se.x; k <= se.y; k++) {
                         const float4 q = spm[k];
                         const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
                         const float r2 = dx * dx + dy * dy + dz * dz;
                         if (r2 > hb2) continue;
+                        f_in++;
                         const float r = sqrtf(r2);
                         if (cached && r > hsum) continue;
                         const float mass = q.w;
@@ -272,6 +279,7 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     a.P[ind] = pc.x;
     a.c[ind] = pc.y;
     a.B[ind] = bal;
+    a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
 }
 
 struct AccelArgs {
@@ -477,6 +485,7 @@ int launch_density(sph_ctx* ctx) {
     a.f = ctx->f;
     a.B = ctx->B;
     a.nr_iters = ctx->nr_iters;
+    a.work = ctx->density_work;
     a.time = ctx->time;
     a.eos0 = ctx->eos[0];
     a.eos1 = ctx->eos[1];

@@ -79,6 +79,7 @@ struct sph_ctx {
     float *rho = nullptr, *u = nullptr, *m = nullptr, *P = nullptr, *h = nullptr;
     float *f = nullptr, *c = nullptr, *du = nullptr, *B = nullptr, *dhdt = nullptr, *gravnorm = nullptr;
     int32_t *mat = nullptr, *next_active = nullptr, *nr_iters = nullptr;
+    int4* density_work = nullptr; /* NR (candidates, inside 2h), final pass (candidates, inside) */
     uint8_t* active = nullptr;
     int32_t* time = nullptr; /* [0] globalTime, [1] dt */
 

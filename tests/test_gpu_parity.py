@@ -20,6 +20,14 @@ def _pair(sph, orc, ic, tables, **prm_over):
     return prm, ctx, st
 
 
+def _stir(ic, seed, sigma=1e-3):
+    """Adds a random velocity dispersion (sigma in Mm/s) so that divergence,
+    curl, viscosity and du/dt are non-trivial."""
+    rng = np.random.default_rng(seed)
+    ic["vel"][:, :3] += rng.normal(0, sigma, size=ic["vel"][:, :3].shape).astype(np.float32)
+    return ic
+
+
 def _rel(a, b, floor=0.0):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
@@ -66,8 +74,14 @@ def _density_compare(ctx, st, n, label):
     same = nr_g == nr_c
     flips = int((~same).sum())
     out = {}
-    for f in ("h", "rho", "gradterm", "pressure", "soundspeed"):
+    for f in ("h", "rho", "gradterm"):
         r = _rel(ctx.download(f), st[f], 1e-30)[same]
+        out[f] = (float(np.max(r)), float(np.median(r)))
+    # P and c come from a table lookup whose slope in rho is steep where P
+    # crosses zero: compare against the field's RMS instead of pointwise
+    for f in ("pressure", "soundspeed"):
+        ref = st[f].astype(np.float64)
+        r = (np.abs(ctx.download(f) - ref) / np.sqrt(np.mean(ref ** 2)))[same]
         out[f] = (float(np.max(r)), float(np.median(r)))
     b = np.abs(ctx.download("balsara").astype(np.float64) - st["balsara"])[same]
     out["balsara_abs"] = (float(b.max()), float(np.median(b)))
@@ -77,7 +91,7 @@ def _density_compare(ctx, st, n, label):
 
 @pytest.mark.parametrize("n,seed", [(20000, 1), (100000, 2)])
 def test_density_parity(sph, orc, tables256, impact, n, seed):
-    ic = impact(n, seed)
+    ic = _stir(impact(n, seed), seed)
     prm, ctx, st = _pair(sph, orc, ic, tables256)
     ctx.sort()
     st.sort()
@@ -85,16 +99,17 @@ def test_density_parity(sph, orc, tables256, impact, n, seed):
     st.density()
     flips, s = _density_compare(ctx, st, n, "ic")
     assert flips <= max(2, n // 1000)
-    assert s["h"][0] < 1e-4 and s["rho"][0] < 1e-4 and s["gradterm"][0] < 1e-3
-    assert s["pressure"][0] < 1e-3 and s["soundspeed"][0] < 1e-3
-    assert s["balsara_abs"][0] < 1e-3
+    assert s["h"][0] < 1e-5 and s["rho"][0] < 1e-5 and s["gradterm"][0] < 1e-4
+    assert s["pressure"][0] < 1e-4 and s["soundspeed"][0] < 1e-4
+    assert s["balsara_abs"][0] < 1e-4
+    assert 0.0 < float(np.median(st["balsara"])) < 1.0
     assert int(ctx.download("dt")[0]) == int(st["dt"][0]) == 65536
 
 
 def test_density_cache_paths(sph, orc, tables256, impact):
     """Particles at the h cap (h = 1) take the reference's neighbour-cache path
     inside the NR loop; a small cache forces the rescan (and its double count)."""
-    ic = impact(20000, 3)
+    ic = _stir(impact(20000, 3), 3)
     ic["h"][::5] = 1.0
     for ne in (128, 24):
         prm, ctx, st = _pair(sph, orc, ic, tables256, n_neighbours_estim=ne)
@@ -104,13 +119,13 @@ def test_density_cache_paths(sph, orc, tables256, impact):
         st.density()
         flips, s = _density_compare(ctx, st, 20000, f"ne={ne}")
         assert flips <= 20
-        assert s["h"][0] < 1e-4 and s["rho"][0] < 1e-4
+        assert s["h"][0] < 1e-5 and s["rho"][0] < 1e-5 and s["pressure"][0] < 1e-4
 
 
 # ----------------------------------------------------------- acceleration
 @pytest.mark.parametrize("n,seed", [(20000, 1), (100000, 2)])
 def test_acceleration_parity(sph, orc, tables256, impact, n, seed):
-    ic = impact(n, seed)
+    ic = _stir(impact(n, seed), seed)
     prm, ctx, st = _pair(sph, orc, ic, tables256)
     st.sort()
     st.density()
@@ -132,7 +147,7 @@ def test_acceleration_parity(sph, orc, tables256, impact, n, seed):
     print(f"[accel] n={n} acc max={err.max():.3e} med={np.median(err):.3e} du max={e_du.max():.3e} "
           f"dhdt max={e_dh.max():.3e} next_active same={na_same:.6f} dt={ctx.download('dt')[0]}/{st['dt'][0]}",
           flush=True)
-    assert err.max() < 1e-4 and e_du.max() < 1e-4 and e_dh.max() < 1e-4
+    assert err.max() < 1e-5 and e_du.max() < 1e-4 and e_dh.max() < 1e-4
     assert na_same > 0.999
     assert int(ctx.download("dt")[0]) == int(st["dt"][0])
 
@@ -140,7 +155,7 @@ def test_acceleration_parity(sph, orc, tables256, impact, n, seed):
 # ------------------------------------------------------------------- step
 def test_step_parity(sph, orc, tables256, impact):
     n = 20000
-    ic = impact(n, 1)
+    ic = _stir(impact(n, 1), 1)
     prm, ctx, st = _pair(sph, orc, ic, tables256)
     st.sort()
     st.density()
@@ -161,7 +176,11 @@ def _grav_compare(ctx, st, label):
     mg = ctx.download("multipoles")
     mc = st.grav["multipoles"]
     mstat = {}
-    for name, cols in (("pos", [0, 1, 2]), ("M", [12]), ("quad", [16, 17, 18, 19, 20, 21]), ("size", [3])):
+    # centre of mass relative to the node's extent plus its distance from
+    # the origin; moments relative to the node's largest moment
+    pscale = (mc[:, 3] + np.abs(mc[:, 0:3]).max(axis=1))[:, None] + 1e-30
+    mstat["pos"] = float((np.abs(mg[:, 0:3] - mc[:, 0:3]) / pscale).max())
+    for name, cols in (("M", [12]), ("quad", [16, 17, 18, 19, 20, 21]), ("size", [3])):
         scale = np.abs(mc[:, cols]).max(axis=1, keepdims=True) + 1e-30
         mstat[name] = float((np.abs(mg[:, cols] - mc[:, cols]) / scale).max())
     act_same = bool(np.array_equal(mg[:, 11], mc[:, 11]))
