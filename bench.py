@@ -162,6 +162,14 @@ def main():
         out["density_ms"] = round(per_pass["density"], 4)
         # dominant kernel: the density pass is one kernel (k_density)
         work = ctx.download("density_work")
+        nr = ctx.download("nr_iters")
+        out["nr_iterations_mean"] = round(float(nr.mean()), 4)
+        out["density_work_per_particle"] = {
+            "nr_scan_candidates": round(float(work[:, 0].mean()), 1),
+            "nr_scan_inside_2h": round(float(work[:, 1].mean()), 1),
+            "final_candidates": round(float(work[:, 2].mean()), 1),
+            "final_neighbours": round(float(work[:, 3].mean()), 1),
+        }
         flops = density_flops(work)
         t = per_pass["density"] / 1000.0
         achieved = flops / t / 1e12

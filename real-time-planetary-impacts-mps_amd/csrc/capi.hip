@@ -65,10 +65,11 @@ void destroy_ctx(sph_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
-    float4** f4[] = {&ctx->pos, &ctx->vel, &ctx->acc, &ctx->s_posm, &ctx->s_vel, &ctx->s_hyd, &ctx->snap};
+    float4** f4[] = {&ctx->pos,    &ctx->vel,    &ctx->acc,   &ctx->s_posm, &ctx->s_vel,
+                     &ctx->s_posh, &ctx->s_velb, &ctx->s_hyd, &ctx->snap};
     for (auto p : f4) dfree(*p);
     float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c,
-                    &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->s_bal};
+                    &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm};
     for (auto p : f1) dfree(*p);
     int32_t** i1[] = {&ctx->mat, &ctx->next_active, &ctx->nr_iters, &ctx->time, &ctx->list_a, &ctx->list_b};
     for (auto p : i1) dfree(*p);
@@ -240,8 +241,9 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->snap, n));
     TRY(dalloc(&ctx->s_posm, n));
     TRY(dalloc(&ctx->s_vel, n));
+    TRY(dalloc(&ctx->s_posh, n));
+    TRY(dalloc(&ctx->s_velb, n));
     TRY(dalloc(&ctx->s_hyd, n));
-    TRY(dalloc(&ctx->s_bal, n));
     float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c, &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm};
     for (auto p : f1) TRY(dalloc(p, n));
     TRY(dalloc(&ctx->mat, n));

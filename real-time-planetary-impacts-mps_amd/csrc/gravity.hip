@@ -432,35 +432,61 @@ __device__ __forceinline__ float dphi_dr(float r, float h, int fixed) {
     return r1 * r1;
 }
 
+/* transformLocal (L2L.metal:12-88) of the node's local expansion to one
+ * active child, written to the child's record (L2L.metal:90-104). */
+__device__ __forceinline__ void l2l_child(const DownArgs& a, const LocalExp& L, int cid) {
+    if (cid < 0) return;
+    if (a.t.aux[3 * cid + 1].w == 0.f) return;
+    const float4 c0 = a.t.far[3 * cid];
+    const float rx = c0.x - L.px, ry = c0.y - L.py, rz = c0.z - L.pz;
+    float nM = L.e[eM];
+    nM += rx * L.e[eX] + ry * L.e[eY] + rz * L.e[eZ];
+    nM += 0.5f * (rx * rx * L.e[eXX] + ry * ry * L.e[eYY] + rz * rz * L.e[eZZ]);
+    nM += rx * ry * L.e[eXY] + rx * rz * L.e[eXZ] + ry * rz * L.e[eYZ];
+    const float nX = L.e[eX] + (rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ]);
+    const float nY = L.e[eY] + (rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ]);
+    const float nZ = L.e[eZ] + (rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ]);
+    a.t.loc[4 * cid + 0] = make_float4(L.px + rx, L.py + ry, L.pz + rz, nM);
+    a.t.loc[4 * cid + 1] = make_float4(nX, nY, nZ, L.e[eXX]);
+    a.t.loc[4 * cid + 2] = make_float4(L.e[eXY], L.e[eXZ], L.e[eYY], L.e[eYZ]);
+    a.t.loc[4 * cid + 3] = make_float4(L.e[eZZ], 0.f, 0.f, 0.f);
+}
+
+/* L2P + intra-leaf P2P for particle k of a leaf (helpers.h:47-81). */
+__device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp& L, const int32_t* pl, int np, int k) {
+    const int p = pl[k];
+    if (!a.active[p]) return;
+    const float4 xi = a.pos[p];
+    const float rx = xi.x - L.px, ry = xi.y - L.py, rz = xi.z - L.pz;
+    /* L2P (L2P.metal:12-32) */
+    float ax = L.e[eX], ay = L.e[eY], az = L.e[eZ];
+    ax += rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ];
+    ay += rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ];
+    az += rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ];
+    for (int j = 0; j < np; ++j) {
+        if (j == k) continue;
+        const int pj = pl[j];
+        const float4 xj = a.pos[pj];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (r == 0.f) continue;
+        const float w = a.m[pj] * dphi_dr(r, a.eps, a.fix_soft);
+        ax -= w * (dx / r);
+        ay -= w * (dy / r);
+        az -= w * (dz / r);
+    }
+    a.acc[p] = make_float4(a.G * ax, a.G * ay, a.G * az, 0.f);
+    a.gravnorm[p] = sqrtf(ax * ax + ay * ay + az * az);
+}
+
+__device__ __forceinline__ void leaf_field_lanes(const DownArgs& a, const LocalExp& L, int4 inf, int lane, int width) {
+    const int32_t* pl = a.t.octree + inf.y + 2;
+    for (int k = lane; k < inf.x; k += width) leaf_particle(a, L, pl, inf.x, k);
+}
+
 /* localToLeaf (helpers.h:47-81): L2P plus the intra-leaf P2P */
 __device__ void leaf_field(const DownArgs& a, const LocalExp& L, int4 inf) {
-    const int np = inf.x;
-    const int32_t* pl = a.t.octree + inf.y + 2;
-    for (int k = 0; k < np; ++k) {
-        const int p = pl[k];
-        if (!a.active[p]) continue;
-        const float4 xi = a.pos[p];
-        const float rx = xi.x - L.px, ry = xi.y - L.py, rz = xi.z - L.pz;
-        /* L2P (L2P.metal:12-32) */
-        float ax = L.e[eX], ay = L.e[eY], az = L.e[eZ];
-        ax += rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ];
-        ay += rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ];
-        az += rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ];
-        for (int j = 0; j < np; ++j) {
-            if (j == k) continue;
-            const int pj = pl[j];
-            const float4 xj = a.pos[pj];
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            if (r == 0.f) continue;
-            const float w = a.m[pj] * dphi_dr(r, a.eps, a.fix_soft);
-            ax -= w * (dx / r);
-            ay -= w * (dy / r);
-            az -= w * (dz / r);
-        }
-        a.acc[p] = make_float4(a.G * ax, a.G * ay, a.G * az, 0.f);
-        a.gravnorm[p] = sqrtf(ax * ax + ay * ay + az * az);
-    }
+    leaf_field_lanes(a, L, inf, 0, 1);
 }
 
 /* downPass (gravity.metal:64-143) for one non-root node */
@@ -503,27 +529,133 @@ __global__ __launch_bounds__(256) void k_down(DownArgs a) {
     if (outp < maxOut && np == 0) a.out[outp] = -1;
     if (np == 0) {
         /* L2L (L2L.metal:90-104) to the active children */
-        for (int c = 0; c < 8; ++c) {
-            const int cid = a.t.child[8 * id + c];
-            if (cid < 0) continue;
-            if (a.t.aux[3 * cid + 1].w == 0.f) continue;
-            const float4 c0 = a.t.far[3 * cid];
-            const float rx = c0.x - L.px, ry = c0.y - L.py, rz = c0.z - L.pz;
-            /* transformLocal (L2L.metal:12-88) */
-            float nM = L.e[eM];
-            nM += rx * L.e[eX] + ry * L.e[eY] + rz * L.e[eZ];
-            nM += 0.5f * (rx * rx * L.e[eXX] + ry * ry * L.e[eYY] + rz * rz * L.e[eZZ]);
-            nM += rx * ry * L.e[eXY] + rx * rz * L.e[eXZ] + ry * rz * L.e[eYZ];
-            const float nX = L.e[eX] + (rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ]);
-            const float nY = L.e[eY] + (rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ]);
-            const float nZ = L.e[eZ] + (rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ]);
-            a.t.loc[4 * cid + 0] = make_float4(L.px + rx, L.py + ry, L.pz + rz, nM);
-            a.t.loc[4 * cid + 1] = make_float4(nX, nY, nZ, L.e[eXX]);
-            a.t.loc[4 * cid + 2] = make_float4(L.e[eXY], L.e[eXZ], L.e[eYY], L.e[eYZ]);
-            a.t.loc[4 * cid + 3] = make_float4(L.e[eZZ], 0.f, 0.f, 0.f);
-        }
+        for (int c = 0; c < 8; ++c) l2l_child(a, L, a.t.child[8 * id + c]);
     } else {
         leaf_field(a, L, inf);
+    }
+}
+
+
+/* Down pass for GRAVITY_MAX_RECURSION == 1 with GS lanes per node.
+ *
+ * Lanes test GS list entries at a time against the node (MAC); accepted
+ * entries are M2L'd by their lane.  The children of rejected entries are
+ * then expanded GS/8 entries at a time, one lane per child: a leaf node
+ * force-accepts every child (recursiveScan, helpers.h:83-123 at recursion 1);
+ * a branch force-accepts leaf children, M2Ls MAC-accepted branch children and
+ * pushes the rest (nonRecursiveScan, helpers.h:125-147).  Pushes are ranked
+ * with a ballot in (entry, child) order -- the reference's sequential order --
+ * so a push whose rank reaches max_unchecked_pointers is force-accepted
+ * exactly where the reference finds its list full.  Each lane keeps a
+ * partial local expansion; the group reduces them with shuffles (a fixed
+ * order, so results are reproducible), adds the incoming local expansion and
+ * runs L2L (lane c -> child c) or L2P + P2P (lane k -> particle k). */
+template <int GS>
+__global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
+    constexpr int GPB = 256 / GS;
+    constexpr int EPR = GS / 8;
+    const int tid = threadIdx.x;
+    const int lane = tid % GS;
+    const int wl = tid & 63;
+    const int gbase = wl - lane;
+    const uint64_t gmask = GS == 64 ? ~0ull : (((1ull << GS) - 1ull) << gbase);
+    const uint64_t ltmask = wl == 0 ? 0ull : (~0ull >> (64 - wl));
+    const int blk = xcd_block((unsigned)((a.count + GPB - 1) / GPB));
+    if (blk < 0) return;
+    const int s = blk * GPB + tid / GS;
+    int id = 0;
+    bool live = s < a.count && a.flags[0] == 0u;
+    if (live) {
+        id = a.level_id[s];
+        live = a.t.aux[3 * id + 1].w != 0.f;
+    }
+    const int4 inf = live ? a.t.info[id] : make_int4(0, 0, 0, 0);
+    const int np = inf.x;
+    const float4 f0 = live ? a.t.far[3 * id] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const NodeA A{f0.x, f0.y, f0.z, f0.w, live ? a.t.aux[3 * id].w : 0.f};
+    const float4 l0 = live ? a.t.loc[4 * id] : make_float4(0.f, 0.f, 0.f, 0.f);
+    LocalExp L;
+    L.px = l0.x;
+    L.py = l0.y;
+    L.pz = l0.z;
+#pragma unroll
+    for (int k = 0; k < eNT; ++k) L.e[k] = 0.f;
+    const uint32_t outbase = (uint32_t)s * (uint32_t)a.maxu;
+    int running = 0;
+    if (live && np == 0) {
+        if (lane == 0) a.out[outbase] = id;
+        running = 1;
+    }
+    const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
+    bool more = live;
+    for (int base = 0; more; base += GS) {
+        const int i = base + lane;
+        const int e = i < a.maxu ? in[i] : -1;
+        const uint64_t term = __ballot(e == -1) & gmask;
+        const uint64_t first = term & (~term + 1ull);
+        const bool valid = i < a.maxu && (term == 0ull || (1ull << wl) < first);
+        more = term == 0ull && base + GS < a.maxu;
+        bool acc = false;
+        if (valid) acc = check_add(a, A, L, e, false);
+        uint64_t rej = __ballot(valid && !acc) & gmask;
+        while (rej != 0ull) {
+            int my_src = -1;
+#pragma unroll
+            for (int r = 0; r < EPR; ++r) {
+                if (rej != 0ull) {
+                    const int b = __ffsll((long long)rej) - 1;
+                    rej &= rej - 1ull;
+                    if (r == (lane >> 3)) my_src = b;
+                }
+            }
+            const int ent = __shfl(e, (my_src < 0 ? wl : my_src) - gbase, GS);
+            const int child = my_src >= 0 ? a.t.child[8 * ent + (lane & 7)] : -1;
+            bool take = false, pushc = false;
+            float4 c0, c1, c2;
+            if (child >= 0 && child != id) {
+                c0 = a.t.far[3 * child];
+                c1 = a.t.far[3 * child + 1];
+                c2 = a.t.far[3 * child + 2];
+                if (np != 0 || a.t.info[child].x != 0) take = true;
+                else if (mac_accept(A.x, A.y, A.z, A.size, A.minGrav, c0, c1, c2, a.eta)) take = true;
+                else pushc = true;
+            }
+            const uint64_t pm = __ballot(pushc) & gmask;
+            if (pushc) {
+                const int pos = running + __popcll(pm & ltmask);
+                if (pos >= a.maxu) take = true;
+                else a.out[outbase + (uint32_t)pos] = child;
+            }
+            running += __popcll(pm);
+            if (take) m2l_add(L, c0, c1, c2, a.eps);
+        }
+    }
+    if (live && np == 0 && lane == 0 && running < a.maxu) a.out[outbase + (uint32_t)running] = -1;
+#pragma unroll
+    for (int k = 0; k < eNT; ++k) {
+        float v = L.e[k];
+#pragma unroll
+        for (int off = GS / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, GS);
+        L.e[k] = v;
+    }
+    if (!live) return;
+    {
+        const float4 l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2], l3 = a.t.loc[4 * id + 3];
+        L.e[eM] = l0.w + L.e[eM];
+        L.e[eX] = l1.x + L.e[eX];
+        L.e[eY] = l1.y + L.e[eY];
+        L.e[eZ] = l1.z + L.e[eZ];
+        L.e[eXX] = l1.w + L.e[eXX];
+        L.e[eXY] = l2.x + L.e[eXY];
+        L.e[eXZ] = l2.y + L.e[eXZ];
+        L.e[eYY] = l2.z + L.e[eYY];
+        L.e[eYZ] = l2.w + L.e[eYZ];
+        L.e[eZZ] = l3.x + L.e[eZZ];
+    }
+    if (np == 0) {
+        if (lane < 8) l2l_child(a, L, a.t.child[8 * id + lane]);
+    } else {
+        leaf_field_lanes(a, L, inf, lane, GS);
     }
 }
 
@@ -665,7 +797,13 @@ int launch_gravity_down(sph_ctx* ctx) {
         a.level_id = T.level_id + T.level_offsets[l];
         a.in = in;
         a.out = out;
-        k_down<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+        if (a.max_rec == 1) {
+            constexpr int GS = 32;
+            const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
+            k_down_group<GS><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
+        } else {
+            k_down<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+        }
         SPH_HIP_TRY(hipGetLastError());
     }
     return SPH_OK;

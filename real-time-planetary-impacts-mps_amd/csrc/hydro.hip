@@ -8,7 +8,7 @@
  *
  * One thread per Morton-sorted slot.  Neighbour candidates are read from the
  * Morton-ordered copies written after the sort (s_posm, s_vel) and at the
- * start of the acceleration pass (s_hyd, s_bal), so a cell's particles are
+ * start of the acceleration pass (s_posh, s_velb, s_hyd), so a cell's particles are
  * contiguous in HBM and neighbouring threads read overlapping cells.  Summation order per
  * particle is the reference's: cells x -> y -> z over the particle's own cell
  * box, particles of a cell in sorted order.
@@ -134,7 +134,9 @@ struct DensityArgs {
 
 /* density.metal:12-239 */
 __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
-    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = xcd_block((a.n + 255) / 256);
+    if (blk < 0) return;
+    const int index = blk * 256 + threadIdx.x;
     if (index >= a.n) return;
     if (index == 0) a.time[1] = a.dt_reset;
     const uint32_t ind = a.sorted[index].y;
@@ -282,14 +284,34 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
 }
 
+/* Morton-ordered copies of what the acceleration loop reads for every
+ * candidate, packed so that the range test needs one 16-byte load:
+ * (x, y, z, h), (v, Balsara factor), (m, f P / rho^2, rho, c).  The product
+ * f P / rho^2 is formed exactly as accelerations.metal:45,91 forms it. */
+__global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __restrict__ sorted,
+                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                      const float* __restrict__ m, const float* __restrict__ h,
+                                                      const float* __restrict__ f, const float* __restrict__ P,
+                                                      const float* __restrict__ rho, const float* __restrict__ c,
+                                                      const float* __restrict__ B, float4* __restrict__ s_posh,
+                                                      float4* __restrict__ s_velb, float4* __restrict__ s_hyd) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = sorted[k].y;
+    const float r = rho[j];
+    const float4 x = pos[j], v = vel[j];
+    s_posh[k] = make_float4(x.x, x.y, x.z, h[j]);
+    s_velb[k] = make_float4(v.x, v.y, v.z, B[j]);
+    s_hyd[k] = make_float4(m[j], f[j] * P[j] / (r * r), r, c[j]);
+}
+
 struct AccelArgs {
     int n;
     const uint2* sorted;
     Cells cells;
-    const float4* s_posm;
-    const float4* s_vel;
+    const float4* s_posh;
+    const float4* s_velb;
     const float4* s_hyd;
-    const float* s_bal;
     const uint8_t* active;
     float4* acc;
     float* du;
@@ -300,22 +322,6 @@ struct AccelArgs {
     int r_max;
 };
 
-/* Morton-ordered copy of the per-particle terms the acceleration loop reads
- * for every candidate: (h, f P / rho^2, rho, c) and the Balsara factor.  The
- * product f P / rho^2 is formed exactly as accelerations.metal:45,91 forms it. */
-__global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __restrict__ sorted,
-                                                      const float* __restrict__ h, const float* __restrict__ f,
-                                                      const float* __restrict__ P, const float* __restrict__ rho,
-                                                      const float* __restrict__ c, const float* __restrict__ B,
-                                                      float4* __restrict__ s_hyd, float* __restrict__ s_bal) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t j = sorted[k].y;
-    const float r = rho[j];
-    s_hyd[k] = make_float4(h[j], f[j] * P[j] / (r * r), r, c[j]);
-    s_bal[k] = B[j];
-}
-
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
@@ -324,16 +330,18 @@ __device__ __forceinline__ int wave_min(int v) {
 
 /* accelerations.metal:12-141 */
 __global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
-    const int index = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = xcd_block((a.n + 255) / 256);
+    if (blk < 0) return;
+    const int index = blk * 256 + threadIdx.x;
     int integerDt = 0x7FFFFFFF;
     if (index < a.n) {
         const uint32_t ind = a.sorted[index].y;
         if (a.active[ind]) {
+            const float4 xi4 = a.s_posh[index];
+            const float4 vi4 = a.s_velb[index];
             const float4 hyd = a.s_hyd[index];
-            const float h_i = hyd.x, fact_i = hyd.y, rho_i = hyd.z, c_i = hyd.w;
-            const float B_i = a.s_bal[index];
-            const float4 xi4 = a.s_posm[index];
-            const float4 vi4 = a.s_vel[index];
+            const float h_i = xi4.w, fact_i = hyd.y, rho_i = hyd.z, c_i = hyd.w;
+            const float B_i = vi4.w;
             const float hi1 = 1.f / h_i;
             const float hi2x4 = h_i * h_i * 4.f;
             float dvx = 0.f, dvy = 0.f, dvz = 0.f, du_dt = 0.f, dhdt = 0.f;
@@ -351,20 +359,20 @@ __global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
                         const uint2 se = cse[key];
                         if (se.x == 0xFFFFFFFFu) continue;
                         for (uint32_t k = se.x; k <= se.y; k++) {
-                            const float4 xj4 = a.s_posm[k];
+                            const float4 xj4 = a.s_posh[k];
                             const float dx = xi4.x - xj4.x, dy = xi4.y - xj4.y, dz = xi4.z - xj4.z;
                             const float r2 = dx * dx + dy * dy + dz * dz;
-                            const float4 hj = a.s_hyd[k];
-                            const float h_j = hj.x;
+                            const float h_j = xj4.w;
                             const float hj2x4 = h_j * h_j * 4.f;
                             if ((r2 > hi2x4 && r2 > hj2x4) || r2 < 1e-12f) continue;
                             const float r = sqrtf(r2);
                             const float r1 = 1.f / r;
-                            const float m_j = xj4.w;
+                            const float4 hj = a.s_hyd[k];
+                            const float4 vj4 = a.s_velb[k];
+                            const float m_j = hj.x;
                             const float fact_j = hj.y;
                             const float rho_j = hj.z;
                             const float c_j = hj.w;
-                            const float4 vj4 = a.s_vel[k];
                             const float hj1 = 1.f / h_j;
                             const float vx = vi4.x - vj4.x, vy = vi4.y - vj4.y, vz = vi4.z - vj4.z;
                             const float rho_ij = 0.5f * (rho_i + rho_j);
@@ -372,7 +380,7 @@ __global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
                             const float mu_ij = v_dot_r < 0.f ? v_dot_r * r1 : 0.f;
                             const float v_sigij = c_i + c_j - 3.f * mu_ij;
                             v_sigi = fmaxf(v_sigij, v_sigi);
-                            const float B_ij = 0.5f * (B_i + a.s_bal[k]);
+                            const float B_ij = 0.5f * (B_i + vj4.w);
                             const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij / rho_ij;
                             const float rx = dx * r1, ry = dy * r1, rz = dz * r1;
                             const float gi = kGradWFactor(r, hi1), gj = kGradWFactor(r, hj1);
@@ -503,7 +511,7 @@ int launch_density(sph_ctx* ctx) {
     const float dtmin1 = p.dt0 / (float)(1 << p.r_max);
     a.dt_reset = (int)(p.dt0 * (1.f / dtmin1));
     const int threads = 256;
-    k_density<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+    k_density<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a);
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }
@@ -514,10 +522,9 @@ int launch_acceleration(sph_ctx* ctx) {
     a.n = ctx->n;
     a.sorted = ctx->pairs_a;
     a.cells = make_cells(ctx);
-    a.s_posm = ctx->s_posm;
-    a.s_vel = ctx->s_vel;
+    a.s_posh = ctx->s_posh;
+    a.s_velb = ctx->s_velb;
     a.s_hyd = ctx->s_hyd;
-    a.s_bal = ctx->s_bal;
     a.active = ctx->active;
     a.acc = ctx->acc;
     a.du = ctx->du;
@@ -534,9 +541,10 @@ int launch_acceleration(sph_ctx* ctx) {
     a.r_max = p.r_max;
     const int threads = 256;
     k_gather_hydro<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(
-        ctx->n, ctx->pairs_a, ctx->h, ctx->f, ctx->P, ctx->rho, ctx->c, ctx->B, ctx->s_hyd, ctx->s_bal);
+        ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m, ctx->h, ctx->f, ctx->P, ctx->rho, ctx->c, ctx->B,
+        ctx->s_posh, ctx->s_velb, ctx->s_hyd);
     SPH_HIP_TRY(hipGetLastError());
-    k_acceleration<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+    k_acceleration<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a);
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }

@@ -80,29 +80,51 @@ __global__ __launch_bounds__(kThreads) void k_hist(const uint2* __restrict__ pai
     tile_hist[threadIdx.x * ntiles + t] = hist[threadIdx.x];
 }
 
-/* Exclusive scan of `count` entries in place, one workgroup of 1024. */
+/* Exclusive scan of `count` entries in place, one workgroup of 1024 walking
+ * the array in coalesced chunks of 4096 (4 consecutive entries per lane):
+ * lane-local scan, wave scan of the lane sums with shuffles, a scan of the
+ * 16 wave sums, and a carry from chunk to chunk. */
 __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ a, int count) {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
     const int tid = threadIdx.x;
-    const int per = (count + 1023) / 1024;
-    const int b = tid * per;
-    const int e = min(b + per, count);
-    uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += a[i];
-    part[tid] = s;
+    const int lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry_s = 0;
     __syncthreads();
-    /* Hillis-Steele inclusive scan over 1024 partial sums */
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = tid >= off ? part[tid - off] : 0u;
+    for (int base = 0; base < count; base += 4096) {
+        const int i0 = base + 4 * tid;
+        uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = i0 + k < count ? a[i0 + k] : 0u;
+        const uint32_t t = v[0] + v[1] + v[2] + v[3];
+        uint32_t incl = t;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
+        }
+        if (lane == 63) wsum[wave] = incl;
         __syncthreads();
-        part[tid] += v;
+        if (wave == 0) {
+            uint32_t w = lane < 16 ? wsum[lane] : 0u;
+            uint32_t wi = w;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                const uint32_t u = __shfl_up(wi, off, 64);
+                if (lane >= off) wi += u;
+            }
+            if (lane < 16) wsum[lane] = wi - w;
+        }
         __syncthreads();
-    }
-    uint32_t run = tid == 0 ? 0u : part[tid - 1];
-    for (int i = b; i < e; ++i) {
-        const uint32_t v = a[i];
-        a[i] = run;
-        run += v;
+        uint32_t run = carry_s + wsum[wave] + (incl - t);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < count) a[i0 + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (tid == 1023) carry_s = run;
+        __syncthreads();
     }
 }
 

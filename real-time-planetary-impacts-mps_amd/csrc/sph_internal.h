@@ -13,8 +13,11 @@
  *   neighbour candidate (slot k = k-th sorted particle):
  *          s_posm  float4 (x, y, z, m)        -- written after the sort
  *          s_vel   float4 (vx, vy, vz, 0)     -- written after the sort
- *          s_hyd   float4 (h, f P / rho^2, rho, c) -- written at the start of the acceleration pass
- *          s_bal   float  (Balsara factor)          -- written at the start of the acceleration pass
+ *   and at the start of the acceleration pass, so that a rejected candidate
+ *   costs one 16-byte load:
+ *          s_posh  float4 (x, y, z, h)
+ *          s_velb  float4 (vx, vy, vz, Balsara factor)
+ *          s_hyd   float4 (m, f P / rho^2, rho, c)
  *   tree (node id = pre-order node number):
  *          mp_far  3 x float4 x K  (pos, size) (M, XX, XY, XZ) (YY, YZ, ZZ, power2)
  *          mp_aux  3 x float4 x K  (min, minGrav) (max, active) (X, Y, Z, -) dipoles
@@ -92,8 +95,8 @@ struct sph_ctx {
     int ntiles = 0;
 
     /* Morton-ordered copies */
-    float4 *s_posm = nullptr, *s_vel = nullptr, *s_hyd = nullptr;
-    float* s_bal = nullptr;
+    float4 *s_posm = nullptr, *s_vel = nullptr;
+    float4 *s_posh = nullptr, *s_velb = nullptr, *s_hyd = nullptr;
 
     /* EOS tables: slot 0 = material 400, slot 1 = every other material */
     float2* eos[2] = {nullptr, nullptr};
@@ -145,6 +148,20 @@ int launch_gravity_down(sph_ctx* ctx);
 int launch_fill_cells(sph_ctx* ctx);
 
 /* ---------------------------------------------------------- device helpers */
+/* MI355X has 8 XCDs, each with its own L2, and hands consecutive workgroup
+ * ids to the XCDs in turn.  Launches over Morton-ordered particles or level
+ * slots remap the hardware workgroup id so that XCD x runs one contiguous
+ * 1/8 of the logical blocks: neighbouring particles then share an L2.
+ * Launch with xcd_grid(blocks) workgroups; -1 marks a padding workgroup. */
+constexpr int kXcds = 8;
+inline unsigned xcd_grid(unsigned blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ int xcd_block(unsigned blocks) {
+    const unsigned per = (blocks + kXcds - 1) / kXcds;
+    const unsigned b = blockIdx.x;
+    const unsigned logical = (b % kXcds) * per + b / kXcds;
+    return logical < blocks ? (int)logical : -1;
+}
+
 /* Morton interleave of three <= 10-bit values (morton.h:16-34). */
 __device__ __forceinline__ uint32_t dev_expand_bits(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
