@@ -42,7 +42,12 @@ extern "C" {
 #define SPH_FIX_SYNC_RANGE    (1u << 1) /* M2M.metal:150 childEnd from the parent's start  */
 #define SPH_FIX_SOFTENING     (1u << 2) /* kernels.h:93,97 integer-literal divisions       */
 #define SPH_FIX_DENSITY_CACHE (1u << 3) /* density.metal:173-199 density not reset on rescan */
-#define SPH_FIX_ALL           (0xFu)
+/* Near field (helpers.h:83-147 force-accept every MAC-rejected leaf): with
+ * this flag a branch carries a rejected leaf down its unchecked list, and a
+ * leaf opens rejected nodes recursively and sums rejected source leaves
+ * particle by particle (P2P), as a standard FMM near field does. */
+#define SPH_FIX_NEAR_FIELD    (1u << 4)
+#define SPH_FIX_ALL           (0x1Fu)
 
 /* Particle activity rule used by the gravity up pass (P2M.metal:31). */
 #define SPH_ACTIVE_AS_WRITTEN 0 /* active iff globalTime <= nextActiveTime (every particle, every step) */

@@ -91,6 +91,19 @@ def lib() -> ctypes.CDLL:
                                           ctypes.c_int32, ctypes.c_double, ctypes.c_double, _P(ctypes.c_double),
                                           _P(ctypes.c_double)]
         L.orc_direct_sum_fp64.restype = None
+        L.orc_pole_m2l.argtypes = [_P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_float]
+        L.orc_pole_m2l.restype = None
+        L.orc_pole_l2l.argtypes = [_P(ctypes.c_float)] * 3
+        L.orc_pole_l2l.restype = None
+        L.orc_pole_l2p.argtypes = [_P(ctypes.c_float)] * 3
+        L.orc_pole_l2p.restype = None
+        L.orc_pole_mac.argtypes = [_P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_float]
+        L.orc_pole_mac.restype = ctypes.c_int
+        L.orc_pole_shift_multipole.argtypes = [_P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_int,
+                                               _P(ctypes.c_float)]
+        L.orc_pole_shift_multipole.restype = None
+        L.orc_pole_derivatives.argtypes = [_P(ctypes.c_float), ctypes.c_float, _P(ctypes.c_float)]
+        L.orc_pole_derivatives.restype = None
     return _lib
 
 
@@ -101,6 +114,79 @@ def _p(a: np.ndarray, t=ctypes.c_float):
 def _pp(params) -> ctypes.c_void_p:
     """Address of an sph_params ctypes structure (the product package's SphParams)."""
     return ctypes.cast(ctypes.byref(params), ctypes.c_void_p)
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, np.float32)
+
+
+def pole_m2l(local16, mp24, eps):
+    out = f32(local16).copy()
+    lib().orc_pole_m2l(_p(out), _p(f32(mp24)), eps)
+    return out
+
+
+def pole_l2l(local16, r3):
+    out = np.zeros(16, np.float32)
+    lib().orc_pole_l2l(_p(f32(local16)), _p(f32(r3)), _p(out))
+    return out
+
+
+def pole_l2p(local16, x3):
+    out = np.zeros(3, np.float32)
+    lib().orc_pole_l2p(_p(f32(local16)), _p(f32(x3)), _p(out))
+    return out
+
+
+def pole_mac(A24, B24, eta):
+    return bool(lib().orc_pole_mac(_p(f32(A24)), _p(f32(B24)), eta))
+
+
+def pole_shift_multipole(mp24, r3, fix_zz):
+    out = np.zeros(24, np.float32)
+    lib().orc_pole_shift_multipole(_p(f32(mp24)), _p(f32(r3)), 1 if fix_zz else 0, _p(out))
+    return out
+
+
+def pole_derivatives(r3, eps):
+    out = np.zeros(10, np.float32)
+    lib().orc_pole_derivatives(_p(f32(r3)), eps, _p(out))
+    return out
+
+
+def W(r, h1):
+    return float(lib().orc_W(r, h1))
+
+
+def dW_dh(r, h1):
+    return float(lib().orc_dW_dh(r, h1))
+
+
+def gradW(x3, r, r1, h1):
+    out = np.zeros(3, np.float32)
+    lib().orc_gradW(_p(f32(x3)), r, r1, h1, _p(out))
+    return out
+
+
+def dphi_dr(x3, h, fixed):
+    return float(lib().orc_dphi_dr(_p(f32(x3)), h, 1 if fixed else 0))
+
+
+def cell_range(params, x3, h):
+    out = np.zeros(6, np.int32)
+    lib().orc_cell_range(_pp(params), _p(f32(x3)), h, _p(out, ctypes.c_int32))
+    return out[:3], out[3:]
+
+
+def eos_sample(params, table, u, rho):
+    out = np.zeros(2, np.float32)
+    t = f32(table)
+    lib().orc_eos_sample(_pp(params), _p(t), t.shape[0], u, rho, _p(out))
+    return out
+
+
+def morton3d(x, y, z):
+    return int(lib().orc_morton3d(x, y, z))
 
 
 def build_octree(pos4: np.ndarray, max_leaf: int = 4, max_depth: int = 7) -> Dict[str, object]:

@@ -559,28 +559,70 @@ void recursive_scan(const GravCtx& c, const Multipole& mp, Local& local, int toS
     }
 }
 
-/* helpers.h:125-147 */
+/* helpers.h:125-147.  With SPH_FIX_NEAR_FIELD a rejected leaf (an entry or a
+ * child) is pushed like a branch instead of being force-accepted. */
 void non_recursive_scan(const GravCtx& c, int32_t* out, const Multipole& mp, Local& local, uint32_t& outputPointer,
                         uint32_t maxOutputPointer, int toScan, int treePointer) {
     const int* tree = c.t->octree;
+    const bool near = c.prm->fixes & SPH_FIX_NEAR_FIELD;
     if (check_and_add(c, mp, local, toScan, false)) return;
+    if (near && tree[toScan] != 0) {
+        if (outputPointer == maxOutputPointer) check_and_add(c, mp, local, toScan, true);
+        else out[outputPointer++] = toScan;
+        return;
+    }
     int start = toScan + 2;
     int end = start + 8;
     for (int j = start; j < end; j++) {
         int childPointer = tree[j];
         if (childPointer == treePointer || childPointer == -1) continue;
-        if (!check_and_add(c, mp, local, childPointer, tree[childPointer] != 0 || outputPointer == maxOutputPointer)) {
+        const bool force = (!near && tree[childPointer] != 0) || outputPointer == maxOutputPointer;
+        if (!check_and_add(c, mp, local, childPointer, force)) {
             out[outputPointer] = childPointer;
             outputPointer++;
         }
     }
 }
 
-/* helpers.h:47-81 */
-void local_to_leaf(const GravCtx& c, const Local& local, int treePointer, int nParticles) {
+/* SPH_FIX_NEAR_FIELD for a leaf: accept by the MAC, else open branches down to
+ * leaves, and collect the rejected source leaves for the P2P sum. */
+void near_scan(const GravCtx& c, const Multipole& mp, Local& local, int toScan, int treePointer,
+               std::vector<int>& p2p_leaves) {
+    const int* tree = c.t->octree;
+    std::vector<int> stack{toScan};
+    while (!stack.empty()) {
+        const int node = stack.back();
+        stack.pop_back();
+        if (node == treePointer) continue;
+        if (check_and_add(c, mp, local, node, false)) continue;
+        if (tree[node] != 0) {
+            p2p_leaves.push_back(node);
+            continue;
+        }
+        for (int j = node + 9; j >= node + 2; --j)   /* children in order 0..7 */
+            if (tree[j] != -1) stack.push_back(tree[j]);
+    }
+}
+
+/* -d(potential)/dr of a unit mass for the potential whose derivatives M2L
+ * uses (derivatives.metal:12-57 with support eps): the near-field form sums
+ * every pair with this one kernel so that a pair's force does not depend on
+ * whether it is reached by P2P or by M2L. */
+inline float dphi_dr_m2l_kernel(float r, float eps) {
+    if (r >= eps) return 1.f / (r * r);
+    const float eps_inv = 1.f / eps;
+    const float u = r * eps_inv;
+    return -(eps_inv * eps_inv) * D_soft_2(u);
+}
+
+/* helpers.h:47-81; `p2p_leaves` (near-field form only) adds the particles of
+ * rejected source leaves to the intra-leaf sum. */
+void local_to_leaf(const GravCtx& c, const Local& local, int treePointer, int nParticles,
+                   const std::vector<int>& p2p_leaves) {
     const int* tree = c.t->octree;
     const orc_state* s = c.s;
     const bool fixed = c.prm->fixes & SPH_FIX_SOFTENING;
+    const bool near = c.prm->fixes & SPH_FIX_NEAR_FIELD;
     int start = treePointer + 2;
     int end = start + nParticles;
     for (int i = start; i < end; i++) {
@@ -595,7 +637,19 @@ void local_to_leaf(const GravCtx& c, const Local& local, int treePointer, int nP
             const float r = len(x_ij);
             if (r == 0) continue;
             const f3 r_hat = x_ij / r;
-            a -= (s->m[p_j] * dphi_dr(x_ij, c.prm->gravity_softening, fixed)) * r_hat;
+            const float f = near ? dphi_dr_m2l_kernel(r, c.prm->gravity_softening)
+                                 : dphi_dr(x_ij, c.prm->gravity_softening, fixed);
+            a -= (s->m[p_j] * f) * r_hat;
+        }
+        for (int leaf : p2p_leaves) {
+            for (int j = leaf + 2; j < leaf + 2 + tree[leaf]; j++) {
+                int p_j = tree[j];
+                f3 x_ij = x_i - ld3(s->pos, p_j);
+                const float r = len(x_ij);
+                if (r == 0) continue;
+                const f3 r_hat = x_ij / r;
+                a -= (s->m[p_j] * dphi_dr_m2l_kernel(r, c.prm->gravity_softening)) * r_hat;
+            }
         }
         st3(s->acc, pp, c.prm->gravity_g * a);
         s->gravnorm[pp] = len(a);
@@ -656,15 +710,21 @@ void down_pass_node(const GravCtx& c, uint32_t index, int treePointer, const int
     uint32_t startScan = parentIndex * MAXU;
     uint32_t endScan = (parentIndex + 1) * MAXU;
     Local local = load_lc(c.g->locals + 16 * (int64_t)dataPointer);
+    const bool near = c.prm->fixes & SPH_FIX_NEAR_FIELD;
+    std::vector<int> p2p_leaves;
     for (uint32_t i = startScan; i < endScan; i++) {
         int toScan = in[i];
         if (toScan == -1) break;
-        if (nParticles != 0) recursive_scan(c, mp, local, toScan, treePointer);
-        else non_recursive_scan(c, out, mp, local, outputPointer, maxOutputPointer, toScan, treePointer);
+        if (nParticles != 0) {
+            if (near) near_scan(c, mp, local, toScan, treePointer, p2p_leaves);
+            else recursive_scan(c, mp, local, toScan, treePointer);
+        } else {
+            non_recursive_scan(c, out, mp, local, outputPointer, maxOutputPointer, toScan, treePointer);
+        }
     }
     if (outputPointer < maxOutputPointer && nParticles == 0) out[outputPointer] = -1;
     if (nParticles == 0) L2L(c, local, treePointer);
-    else local_to_leaf(c, local, treePointer, nParticles);
+    else local_to_leaf(c, local, treePointer, nParticles, p2p_leaves);
 }
 
 /* octree.cpp:14-109 */
@@ -1158,6 +1218,36 @@ void orc_gravity_down(const sph_params* prm, const orc_state* s, const orc_tree*
     }
 }
 
+void orc_pole_m2l(float* local16, const float* mp24, float eps) {
+    Local l = load_lc(local16);
+    Local add = M2L(l.pos, load_mp(mp24), eps);
+    for (int j = 0; j < NT; ++j) l.e[j] += add.e[j];
+    store_lc(local16, l);
+}
+
+void orc_pole_l2l(const float* local16, const float* r3, float* out16) {
+    store_lc(out16, transform_local(load_lc(local16), mk(r3[0], r3[1], r3[2])));
+}
+
+void orc_pole_l2p(const float* local16, const float* x3, float* acc3) {
+    f3 a = L2P(load_lc(local16), mk(x3[0], x3[1], x3[2]));
+    acc3[0] = a.x; acc3[1] = a.y; acc3[2] = a.z;
+}
+
+int orc_pole_mac(const float* A24, const float* B24, float eta) {
+    return m2l_accept(load_mp(A24), load_mp(B24), eta) ? 1 : 0;
+}
+
+void orc_pole_shift_multipole(const float* mp24, const float* r3, int fix_zz, float* out24) {
+    Multipole m = transform_multipole(load_mp(mp24), mk(r3[0], r3[1], r3[2]), fix_zz != 0);
+    add_powers(m);
+    store_mp(out24, m);
+}
+
+void orc_pole_derivatives(const float* r3, float eps, float* d10) {
+    derivatives(mk(r3[0], r3[1], r3[2]), eps, d10);
+}
+
 void orc_direct_sum_fp64(const float* pos4, const float* m, int32_t n, const int32_t* targets, int32_t n_targets,
                          double G, double softening, double* acc3_out, double* phi_out) {
 #pragma omp parallel for schedule(dynamic, 8)
@@ -1173,7 +1263,14 @@ void orc_direct_sum_fp64(const float* pos4, const float* m, int32_t n, const int
             if (r2 == 0.0) continue;
             const double r = std::sqrt(r2);
             double f, p;
-            if (hh > 0 && r < 2 * hh) {
+            if (hh < 0 && r < -hh) {
+                /* the M2L softening (derivatives.metal:12-57), support |hh| */
+                const double e = -hh, u = r / e;
+                const double d1 = ((((((-3 * u + 15) * u - 28) * u + 21) * u) * u - 7) * u) * u + 3;
+                const double d2 = (((((-21 * u + 90) * u - 140) * u + 84) * u) * u - 14) * u;
+                f = -d2 / (e * e);
+                p = -d1 / e;
+            } else if (hh > 0 && r < 2 * hh) {
                 const double q = r / hh, h1 = 1 / hh;
                 if (q <= 1) {
                     f = h1 * h1 * (4.0 / 3.0 * q - 6.0 / 5.0 * q * q * q + 0.5 * q * q * q * q);

@@ -104,9 +104,23 @@ typedef struct orc_gravity_buffers {
 void orc_gravity_up(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g);
 void orc_gravity_down(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g);
 
+/* Single FMM operators on 24-float multipole / 16-float local records (for
+ * unit pins): M2L (M2L.metal:12-108) added into `local`, transformLocal
+ * (L2L.metal:12-88), L2P (L2P.metal:12-32), the acceptance test
+ * (accept.metal:14-55), transformMultipole (M2M.metal:12-73) and the
+ * derivative tensor of the softened 1/r (derivatives.metal:59-156). */
+void orc_pole_m2l(float* local16, const float* mp24, float eps);
+void orc_pole_l2l(const float* local16, const float* r3, float* out16);
+void orc_pole_l2p(const float* local16, const float* x3, float* acc3);
+int orc_pole_mac(const float* A24, const float* B24, float eta);
+void orc_pole_shift_multipole(const float* mp24, const float* r3, int fix_zz, float* out24);
+void orc_pole_derivatives(const float* r3, float eps, float* d10);
+
 /* FP64 direct sum over all n sources at the sampled targets: acceleration
  * (G included) and potential.  `softening` > 0 uses the cubic-spline softened
- * force (the corrected dphi_dr form) with that length; 0 = pure Newton. */
+ * force (the corrected dphi_dr form) with that length (support 2 softening);
+ * < 0 uses the potential whose derivatives M2L uses (derivatives.metal:12-57)
+ * with support |softening|; 0 = pure Newton. */
 void orc_direct_sum_fp64(const float* pos4, const float* m, int32_t n,
                          const int32_t* targets, int32_t n_targets, double G,
                          double softening, double* acc3_out, double* phi_out);

@@ -26,7 +26,8 @@ SPH_FIX_M2M_ZZ = 1 << 0
 SPH_FIX_SYNC_RANGE = 1 << 1
 SPH_FIX_SOFTENING = 1 << 2
 SPH_FIX_DENSITY_CACHE = 1 << 3
-SPH_FIX_ALL = 0xF
+SPH_FIX_NEAR_FIELD = 1 << 4
+SPH_FIX_ALL = 0x1F
 SPH_ACTIVE_AS_WRITTEN = 0
 SPH_ACTIVE_README = 1
 TREE_ASYNC, TREE_EVERY, TREE_FIXED = 0, 1, 2

@@ -36,11 +36,12 @@ double tillotson_pressure(const sph_tillotson& t, double rho, double u) {
         if (u >= t.Ecv) P = Pe;
         else P = ((u - t.Eiv) * Pe + (t.Ecv - u) * Pc) / (t.Ecv - t.Eiv);
     }
-    return P > 0.0 ? P : 0.0;
+    return P;
 }
 
-/* c^2 = dP/drho|_u + (P / rho^2) dP/du|_rho by central differences, floored
- * at (1% of the bulk sound speed sqrt(A/rho0))^2. */
+/* c^2 = dP/drho|_u + (P / rho^2) dP/du|_rho by central differences of the
+ * unfloored pressure (so a cold, slightly expanded state keeps its bulk sound
+ * speed), floored at (1% of the bulk sound speed sqrt(A/rho0))^2. */
 double tillotson_sound_speed(const sph_tillotson& t, double rho, double u) {
     const double dr = rho * 1e-6, du = u * 1e-6;
     const double P = tillotson_pressure(t, rho, u);
@@ -77,7 +78,7 @@ int sph_make_eos_table(const sph_params* prm, const sph_tillotson* t, int res, u
             const double u = minU * std::pow(maxU / minU, (double)j / res);
             const double P = tillotson_pressure(*t, rho, u);
             const double c = tillotson_sound_speed(*t, rho, u);
-            pc_out[2 * ((int64_t)i * res + j)] = (float)(P * 1e-18);
+            pc_out[2 * ((int64_t)i * res + j)] = (float)((P > 0.0 ? P : 0.0) * 1e-18);
             pc_out[2 * ((int64_t)i * res + j) + 1] = (float)(c * 1e-6);
         }
     }

@@ -1,0 +1,659 @@
+"""Pins for the CPU oracle (oracle/): every restated function is checked
+against something it does not share code with -- a closed form, brute force,
+quadrature, finite differences or an invariant -- chosen so that a dropped
+term, a wrong sign or index, or a transposed operand fails.
+
+The reference ships no tests, golden vectors or fixtures with expected
+outputs, so these are the oracle's pins.  CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+# ------------------------------------------------------------------ helpers
+def naive_morton(x, y, z):
+    """Bit-by-bit interleave x | y << 1 | z << 2 over 10 bits."""
+    key = 0
+    for b in range(10):
+        key |= ((x >> b) & 1) << (3 * b)
+        key |= ((y >> b) & 1) << (3 * b + 1)
+        key |= ((z >> b) & 1) << (3 * b + 2)
+    return key
+
+
+def cubic_W(r, h):
+    """M4 cubic spline, sigma = 1/pi, support 2h (Monaghan & Lattanzio 1985)."""
+    q = r / h
+    s = 1.0 / (math.pi * h ** 3)
+    if q < 1:
+        return s * (1 - 1.5 * q * q + 0.75 * q ** 3)
+    if q < 2:
+        return s * 0.25 * (2 - q) ** 3
+    return 0.0
+
+
+def spline_potential(r, h):
+    """Potential of a unit mass with cubic-spline softening of length h
+    (Hernquist & Katz 1989), normalised to -1/r beyond 2h."""
+    q = r / h
+    if q < 1:
+        return (2 / 3 * q ** 2 - 3 / 10 * q ** 4 + 1 / 10 * q ** 5 - 7 / 5) / h
+    if q < 2:
+        return (4 / 3 * q ** 2 - q ** 3 + 3 / 10 * q ** 4 - 1 / 30 * q ** 5 - 8 / 5 + 1 / (15 * q)) / h
+    return -1.0 / r
+
+
+@pytest.fixture(scope="module")
+def prm(sph):
+    return sph.default_params()
+
+
+# ------------------------------------------------------------ a1 morton
+def test_morton_exhaustive_64_cube_and_corners(orc):
+    rng = np.random.default_rng(0)
+    pts = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (255, 255, 255), (0, 0, 0), (128, 3, 77)]
+    pts += [tuple(int(v) for v in rng.integers(0, 256, 3)) for _ in range(2000)]
+    for x, y, z in pts:
+        assert orc.morton3d(x, y, z) == naive_morton(x, y, z)
+    assert orc.morton3d(1, 0, 0) == 1 and orc.morton3d(0, 1, 0) == 2 and orc.morton3d(0, 0, 1) == 4
+    assert orc.morton3d(255, 255, 255) == 0xFFFFFF
+    for x in range(0, 64, 3):
+        for y in range(64):
+            for z in range(0, 64, 5):
+                assert orc.morton3d(x, y, z) == naive_morton(x, y, z)
+
+
+# ----------------------------------------------------- a2-a9 hash and sort
+def _state(sph, orc, prm, tables, n, seed, scale=1.0, shift=0.0):
+    ic = sph.make_impact(prm, sph.impact_config(n, seed), tables)
+    ic["pos"][:, :3] = ic["pos"][:, :3] * np.float32(scale) + np.float32(shift)
+    return orc.State(prm, ic, tables), ic
+
+
+def test_hash_closed_form_with_negative_and_wrapped_coordinates(sph, orc, prm, tables256):
+    st, ic = _state(sph, orc, prm, tables256, 3000, 11, scale=6.0, shift=-30.0)
+    st.hash()
+    pos = st["pos"][:, :3]
+    cs = np.float32(prm.cell_width)
+    c = np.floor(pos / cs).astype(np.int64) % 256      # python modulo: already in [0, 256)
+    want = np.array([naive_morton(*map(int, v)) for v in c], np.uint32)
+    assert np.array_equal(st["cell_array"][:, 0], want)
+    assert np.array_equal(st["cell_array"][:, 1], np.arange(3000, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("n,seed", [(1, 1), (257, 2), (5000, 3)])
+def test_radix_sort_is_stable_sort_by_key(sph, orc, prm, tables256, n, seed):
+    if n == 1:
+        ic = {k: v[:1].copy() for k, v in sph.make_impact(prm, sph.impact_config(10, seed), tables256).items()}
+        st = orc.State(prm, ic, tables256)
+    else:
+        st, _ = _state(sph, orc, prm, tables256, n, seed, scale=3.0)
+    st.hash()
+    keys = st["cell_array"][:, 0].copy()
+    st.radix_sort()
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(st["cell_array"][:, 0], keys[order])
+    assert np.array_equal(st["cell_array"][:, 1], order.astype(np.uint32))
+
+
+def test_cell_start_end_by_linear_scan(sph, orc, prm, tables256):
+    st, _ = _state(sph, orc, prm, tables256, 4000, 5)
+    st.sort()
+    keys = st["cell_array"][:, 0]
+    start = np.full(1 << 24, 0xFFFFFFFF, np.uint32)
+    end = np.zeros(1 << 24, np.uint32)
+    for i, k in enumerate(keys):
+        if start[k] == 0xFFFFFFFF:
+            start[k] = i
+        end[k] = i
+    assert np.array_equal(st["cell_start"], start)
+    assert np.array_equal(st["cell_end"], end)
+
+
+# --------------------------------------------------------- a10 cell range
+def test_cell_range_covers_the_2h_ball(orc, prm):
+    rng = np.random.default_rng(4)
+    cs = prm.cell_width
+    for _ in range(300):
+        x = rng.uniform(-20, 20, 3).astype(np.float32)
+        h = np.float32(rng.uniform(0.05, 1.0))
+        mn, mx = orc.cell_range(prm, x, h)
+        d = rng.normal(size=(200, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        pts = x + d * (2 * h) * rng.uniform(0, 0.999, (200, 1))
+        c = np.floor(pts / cs).astype(np.int64)
+        assert np.all(c >= mn) and np.all(c <= mx)
+        # and the box is no bigger than the ball's bounding cells
+        assert np.all(mx - mn <= np.ceil(4 * h / cs) + 1)
+
+
+# ------------------------------------------------------------ a11 kernels
+def test_W_normalised_and_matches_closed_form(orc):
+    for h in (0.1, 0.37, 1.0):
+        r = np.linspace(0, 2 * h, 4001)
+        w = np.array([orc.W(float(v), 1.0 / h) for v in r])
+        integral = np.trapezoid(4 * math.pi * r ** 2 * w, r)
+        assert abs(integral - 1.0) < 1e-5
+        for v, wv in zip(r[::97], w[::97]):
+            assert abs(wv - cubic_W(v, h)) <= 2e-6 * cubic_W(0, h)
+    assert orc.W(2.5, 1.0) == 0.0
+
+
+def test_gradW_and_dWdh_are_derivatives_of_W(orc):
+    h = 0.3
+    for r in np.linspace(0.01, 0.59, 40):
+        d = 1e-4
+        dWdr = (cubic_W(r + d, h) - cubic_W(r - d, h)) / (2 * d)
+        x = np.array([r * 0.6, -r * 0.8, 0.0])
+        g = orc.gradW(x, float(r), float(1 / r), 1.0 / h)
+        assert np.allclose(g, dWdr * x / r, rtol=2e-3, atol=1e-3 * abs(dWdr) + 1e-6)
+        dWdh = (cubic_W(r, h + d) - cubic_W(r, h - d)) / (2 * d)
+        assert abs(orc.dW_dh(float(r), 1.0 / h) - dWdh) <= 2e-3 * abs(dWdh) + 1e-3
+
+
+def test_dphi_dr_corrected_form_is_the_spline_force(orc):
+    eps = 0.04
+    for r in np.linspace(0.002, 0.12, 60):
+        d = 1e-6
+        force = (spline_potential(r + d, eps) - spline_potential(r - d, eps)) / (2 * d)
+        got = orc.dphi_dr(np.array([r, 0.0, 0.0]), eps, True)
+        assert abs(got - force) <= 1e-3 * abs(force) + 1e-2
+    assert orc.dphi_dr(np.array([0.2, 0, 0]), eps, True) == pytest.approx(1 / 0.2 ** 2, rel=1e-6)
+
+
+def test_dphi_dr_as_written_uses_integer_fractions(orc):
+    """kernels.h:93,97: (4/3)=1, (6/5)=1, (8/3)=2, (1/6)=0, (1/15)=0."""
+    eps = 0.04
+    h1 = 1 / eps
+    for q in (0.3, 0.9, 1.2, 1.9):
+        r = q * eps
+        got = orc.dphi_dr(np.array([0.0, r, 0.0]), eps, False)
+        want = h1 * h1 * ((q - q ** 3 + 0.5 * q ** 4) if q <= 1 else (2 * q - 3 * q * q + q ** 3))
+        assert got == pytest.approx(want, rel=1e-5)
+    # as written the "force" is repulsive for 1 < q < 2
+    assert orc.dphi_dr(np.array([1.5 * eps, 0, 0]), eps, False) < 0
+
+
+# ---------------------------------------------------------------- a13 EOS
+def test_eos_sample_reproduces_a_bilinear_table(orc, prm):
+    res = 16
+    i = np.arange(res)
+    P = (3.0 * i[:, None] + 0.5 * i[None, :] + 1.0).astype(np.float32)   # row = rho index, col = u index
+    C = (2.0 * i[:, None] - 1.0 * i[None, :] + 40.0).astype(np.float32)
+    tab = np.stack([P, C], axis=-1)
+    rng = np.random.default_rng(2)
+    lu = math.log(prm.eos_max_u / prm.eos_min_u)
+    lr = math.log(prm.eos_max_rho / prm.eos_min_rho)
+    for _ in range(300):
+        tx, ty = rng.uniform(0.0, res - 1.0, 2)   # texel-space coordinate between the first and last centre
+        u = prm.eos_min_u * math.exp((tx + 0.5) / res * lu)
+        rho = prm.eos_min_rho * math.exp((ty + 0.5) / res * lr)
+        got = orc.eos_sample(prm, tab, u, rho)
+        assert got[0] == pytest.approx(3.0 * ty + 0.5 * tx + 1.0, rel=2e-4, abs=2e-3)
+        assert got[1] == pytest.approx(2.0 * ty - 1.0 * tx + 40.0, rel=2e-4, abs=2e-3)
+    # clamp to edge: far outside the axes gives the corner texels
+    assert np.allclose(orc.eos_sample(prm, tab, 1e-30, 1e-30), tab[0, 0])
+    assert np.allclose(orc.eos_sample(prm, tab, 1e9, 1e9), tab[-1, -1])
+    # a non-positive energy reads the first column
+    assert np.allclose(orc.eos_sample(prm, tab, -1.0, 1e-30), tab[0, 0])
+
+
+# ------------------------------------------------------------- a12 density
+@pytest.fixture(scope="module")
+def small_density(sph, orc, prm, tables256):
+    st, ic = _state(sph, orc, prm, tables256, 2000, 21)
+    rng = np.random.default_rng(21)
+    st["vel"] = st["vel"] + np.pad(rng.normal(0, 1e-3, (2000, 3)), ((0, 0), (0, 1))).astype(np.float32)
+    st.sort()
+    st.density()
+    return st
+
+
+def test_density_equals_brute_force_sum(small_density):
+    st = small_density
+    pos = st["pos"][:, :3].astype(np.float64)
+    m = st["mass"].astype(np.float64)
+    h = st["h"].astype(np.float64)
+    d = np.linalg.norm(pos[:, None, :] - pos[None, :, :], axis=2)
+    q = d / h[:, None]
+    w = np.where(q < 1, 1 - 1.5 * q ** 2 + 0.75 * q ** 3, np.where(q < 2, 0.25 * (2 - q) ** 3, 0.0))
+    w /= (math.pi * h ** 3)[:, None]
+    rho = (w * m[None, :]).sum(axis=1)
+    assert np.max(np.abs(rho - st["rho"]) / rho) < 2e-5
+
+
+def test_density_fixed_point_invariant(small_density, prm):
+    """At exit, the damped update of the final h is within DENSITY_ETA unless
+    the iteration cap was reached (density.metal:131-142)."""
+    st = small_density
+    h = st["h"].astype(np.float64)
+    target = np.minimum(prm.resolution_eta * (st["mass"] / st["rho"].astype(np.float64)) ** (1 / 3),
+                        prm.max_smoothing_length)
+    eps = np.abs(0.5 * (h + target) - h) / h
+    ok = (eps <= prm.density_eta * (1 + 1e-5)) | (st["nr_iters"] >= prm.max_nr_iterations)
+    assert ok.all()
+    assert st["nr_iters"].max() > 0
+
+
+def test_gradient_term_and_balsara_by_brute_force(small_density):
+    st = small_density
+    pos = st["pos"][:, :3].astype(np.float64)
+    vel = st["vel"][:, :3].astype(np.float64)
+    m = st["mass"].astype(np.float64)
+    h = st["h"].astype(np.float64)
+    rho = st["rho"].astype(np.float64)
+    x = pos[None, :, :] - pos[:, None, :]            # x_j - x_i
+    r = np.linalg.norm(x, axis=2)
+    q = r / h[:, None]
+    # dW/dh and dW/dr of the cubic spline
+    sig = 1 / math.pi
+    dwdq = np.where(q < 1, -3 * q + 2.25 * q * q, np.where(q < 2, -0.75 * (2 - q) ** 2, 0.0))
+    wq = np.where(q < 1, 1 - 1.5 * q ** 2 + 0.75 * q ** 3, np.where(q < 2, 0.25 * (2 - q) ** 3, 0.0))
+    h_ = h[:, None]
+    dwdh = sig * (-3 * wq / h_ ** 4 - dwdq * q / h_ ** 4)
+    dwdr = sig * dwdq / h_ ** 4
+    density_h = (m[None, :] * dwdh).sum(axis=1)
+    f = 1 / (1 + h / (3 * rho) * density_h)
+    assert np.max(np.abs(f - st["gradterm"]) / np.abs(f)) < 1e-3
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rhat = np.where(r[..., None] > 0, x / r[..., None], 0.0)
+    g = dwdr[..., None] * rhat
+    v = vel[None, :, :] - vel[:, None, :]
+    div = np.abs((m[None, :, None] * v * g).sum(axis=(1, 2))) / rho
+    curl = np.linalg.norm((m[None, :, None] * np.cross(v, g)).sum(axis=1), axis=1) / rho
+    bal = div / (div + curl + 1e-4 * st["soundspeed"] / h)
+    assert np.max(np.abs(bal - st["balsara"])) < 2e-3
+
+
+def test_density_cache_overflow_counts_twice(sph, orc, prm, tables256):
+    """density.metal:173-199: with a cache smaller than the neighbour count the
+    rescan adds to the NR density again; the fixed form does not."""
+    ic = sph.make_impact(prm, sph.impact_config(2000, 22), tables256)
+    base = orc.State(prm, ic, tables256)
+    base.sort()
+    base.density()
+    small = sph.default_params(n_neighbours_estim=4)
+    st = orc.State(small, ic, tables256)
+    st.sort()
+    st.density()
+    assert np.allclose(st["rho"], 2 * base["rho"], rtol=1e-5)
+    fixed = sph.default_params(n_neighbours_estim=4, fixes=sph.SPH_FIX_DENSITY_CACHE)
+    st2 = orc.State(fixed, ic, tables256)
+    st2.sort()
+    st2.density()
+    assert np.allclose(st2["rho"], base["rho"], rtol=1e-6)
+
+
+# -------------------------------------------------------- a14 acceleration
+def test_acceleration_brute_force_and_momentum(small_density, prm):
+    st = small_density
+    n = st.n
+    st["acc"] = 0.0
+    st.acceleration()
+    pos = st["pos"][:, :3].astype(np.float64)
+    vel = st["vel"][:, :3].astype(np.float64)
+    m = st["mass"].astype(np.float64)
+    h = st["h"].astype(np.float64)
+    rho = st["rho"].astype(np.float64)
+    P = st["pressure"].astype(np.float64)
+    f = st["gradterm"].astype(np.float64)
+    c = st["soundspeed"].astype(np.float64)
+    B = st["balsara"].astype(np.float64)
+    xij = pos[:, None, :] - pos[None, :, :]
+    r = np.linalg.norm(xij, axis=2)
+    np.fill_diagonal(r, np.inf)
+
+    def gradw(hh):
+        q = r / hh
+        dwdq = np.where(q < 1, -3 * q + 2.25 * q * q, np.where(q < 2, -0.75 * (2 - q) ** 2, 0.0))
+        return (dwdq / (math.pi * hh ** 4))[..., None] * xij / r[..., None]
+
+    gi = gradw(h[:, None])
+    gj = gradw(h[None, :])
+    # candidates come only from particle i's own cell box of 2 h_i
+    # (accelerations.metal:60), although a pair is kept when r <= 2 h_j too
+    cs = np.float32(prm.cell_width)
+    p32 = st["pos"][:, :3]
+    cell = np.floor(p32 / cs).astype(np.int64)
+    hb = np.minimum(st["h"], np.float32(prm.max_smoothing_length))
+    lo = np.floor((p32 - (np.float32(2) * hb)[:, None]) / cs).astype(np.int64)
+    hi = np.floor((p32 + (np.float32(2) * hb)[:, None]) / cs).astype(np.int64)
+    inbox = np.all((cell[None, :, :] >= lo[:, None, :]) & (cell[None, :, :] <= hi[:, None, :]), axis=2)
+    use = ((r <= 2 * h[:, None]) | (r <= 2 * h[None, :])) & inbox
+    vij = vel[:, None, :] - vel[None, :, :]
+    vdr = (vij * xij).sum(axis=2)
+    mu = np.where(vdr < 0, vdr / r, 0.0)
+    vsig = c[:, None] + c[None, :] - 3 * mu
+    nu = -0.5 * prm.viscosity_alpha * 0.5 * (B[:, None] + B[None, :]) * mu * vsig / (0.5 * (rho[:, None] + rho[None, :]))
+    fi = (f * P / rho ** 2)[:, None, None]
+    fj = (f * P / rho ** 2)[None, :, None]
+    gm = 0.5 * (gi + gj)
+    terms = m[None, :, None] * (fi * gi + fj * gj + nu[..., None] * gm)
+    a = -(np.where(use[..., None], terms, 0.0)).sum(axis=1)
+    scale = np.sqrt(np.mean(np.sum(a ** 2, axis=1)))
+    assert np.max(np.linalg.norm(a - st["acc"][:, :3], axis=1)) / scale < 1e-4
+    du = (np.where(use, m[None, :] * (fi[..., 0] * (vij * gi).sum(2) + 0.5 * nu * (vij * gm).sum(2)), 0.0)).sum(1)
+    assert np.max(np.abs(du - st["du"])) / np.sqrt(np.mean(du ** 2)) < 1e-4
+
+
+def test_momentum_conserved_with_uniform_h(sph, orc, prm, tables256):
+    """With equal smoothing lengths the pair sets are symmetric, so sum m a = 0."""
+    ic = sph.make_impact(prm, sph.impact_config(3000, 23), tables256)
+    st = orc.State(prm, ic, tables256)
+    st.sort()
+    st.density()
+    st["h"] = np.float32(np.median(st["h"]))
+    st["acc"] = 0.0
+    st.acceleration()
+    a = st["acc"][:, :3].astype(np.float64)
+    m = st["mass"].astype(np.float64)
+    p = (m[:, None] * a).sum(axis=0)
+    assert np.linalg.norm(p) < 1e-5 * (m[:, None] * np.abs(a)).sum()
+
+
+def test_timestep_is_power_of_two_rung(small_density, prm):
+    st = small_density
+    dt = int(st["dt"][0])
+    assert dt > 0 and (dt & (dt - 1)) == 0 and dt <= 65536
+    steps = st["next_active"] - st["global_time"][0]
+    assert np.all((steps & (steps - 1)) == 0) and steps.min() == dt
+
+
+# ---------------------------------------------------------------- a15 step
+def test_step_closed_form(sph, orc, prm, tables256):
+    ic = sph.make_impact(prm, sph.impact_config(500, 24), tables256)
+    st = orc.State(prm, ic, tables256)
+    rng = np.random.default_rng(24)
+    st["acc"] = rng.normal(0, 1e-6, (500, 4)).astype(np.float32)
+    st["du"] = rng.normal(0, 1e-9, 500).astype(np.float32)
+    st["dhdt"] = rng.normal(0, 1e-3, 500).astype(np.float32)
+    st["dt"] = 4096
+    before = {k: st[k].astype(np.float64).copy() for k in ("pos", "vel", "acc", "h", "rho", "u", "du", "dhdt")}
+    st.step()
+    dt = 4096 * prm.dt0 / 2 ** prm.r_max
+    x = before["pos"][:, :3] + dt * before["vel"][:, :3] + 0.5 * before["acc"][:, :3] * dt * dt
+    assert np.allclose(st["pos"][:, :3], x, rtol=1e-6, atol=1e-6)
+    assert np.allclose(st["vel"][:, :3], before["vel"][:, :3] + dt * before["acc"][:, :3], rtol=1e-6, atol=1e-9)
+    g = before["dhdt"] * dt / before["h"]
+    assert np.allclose(st["h"], np.minimum(before["h"] * np.exp(g), 1.0), rtol=1e-5)
+    assert np.allclose(st["rho"], before["rho"] * np.exp(-3 * g), rtol=1e-5)
+    assert np.allclose(st["u"], before["u"] + dt * before["du"], rtol=1e-6)
+    assert st["global_time"][0] == 4096
+
+
+# -------------------------------------------------------------- a16 octree
+def _walk(tree, n):
+    oc = tree["octree"]
+    seen = np.zeros(n, np.int32)
+    depth_of = {}
+    order = []
+
+    def rec(tp, depth):
+        order.append(tp)
+        depth_of[tp] = depth
+        if oc[tp] == 0:
+            for c in oc[tp + 2:tp + 10]:
+                if c >= 0:
+                    rec(int(c), depth + 1)
+        else:
+            seen[oc[tp + 2:tp + 2 + oc[tp]]] += 1
+    rec(0, 0)
+    return seen, depth_of, order
+
+
+@pytest.mark.parametrize("n,seed", [(1, 1), (5, 2), (3000, 3)])
+def test_octree_partition_and_layout(sph, orc, prm, tables256, n, seed):
+    ic = sph.make_impact(prm, sph.impact_config(max(n, 10), seed), tables256)
+    pos = ic["pos"][:n]
+    tree = orc.build_octree(pos, 4, 7)
+    seen, depth_of, order = _walk(tree, n)
+    assert np.all(seen == 1)                                # each particle in exactly one leaf
+    oc = tree["octree"]
+    ids = [oc[tp + 1] for tp in order]
+    assert ids == list(range(tree["node_count"]))           # node numbers are pre-order
+    off = 0
+    for lvl, cnt in enumerate(tree["level_counts"]):
+        lv = tree["level_nodes"][off:off + cnt]
+        assert all(depth_of[int(tp)] == lvl for tp in lv)
+        assert list(lv) == [tp for tp in order if depth_of[tp] == lvl]
+        off += cnt
+    for tp in order:                                        # leaf rule
+        if oc[tp] != 0:
+            assert oc[tp] <= 4 or depth_of[tp] == 7
+        else:
+            kids = [c for c in oc[tp + 2:tp + 10] if c >= 0]
+            assert depth_of[tp] < 7 and len(kids) >= 1
+
+
+def test_octree_split_is_at_the_mean_with_strict_greater(orc):
+    # 8 particles at the corners of a cube around 0 plus one at the mean
+    pos = np.zeros((9, 4), np.float32)
+    corners = np.array([[sx, sy, sz] for sz in (-1, 1) for sy in (-1, 1) for sx in (-1, 1)], np.float32)
+    pos[:8, :3] = corners
+    tree = orc.build_octree(pos, 4, 7)
+    oc = tree["octree"]
+    assert oc[0] == 0
+    # particle 8 sits exactly at the mean: strict '>' puts it in octant 0 with corner 0
+    child0 = oc[2]
+    assert oc[child0] == 2 and set(oc[child0 + 2:child0 + 4]) == {0, 8}
+    for c in range(1, 8):
+        tp = oc[2 + c]
+        assert oc[tp] == 1 and oc[tp + 2] == c
+
+
+# ------------------------------------------------------------- a18-a24 FMM
+def _mp_record(pos, m):
+    """24-float multipole record (pos, size, min, minGrav, max, active, M,
+    dipoles, quadrupoles) of a particle set by direct FP64 sums."""
+    pos = np.asarray(pos, np.float64)
+    m = np.asarray(m, np.float64)
+    M = m.sum()
+    com = (m[:, None] * pos).sum(0) / M
+    r = pos - com
+    q = lambda a, b: (m * r[:, a] * r[:, b]).sum()
+    rec = np.zeros(24)
+    rec[0:3] = com
+    rec[3] = (pos.max(0) - pos.min(0)).max()
+    rec[4:7] = pos.min(0)
+    rec[8:11] = pos.max(0)
+    rec[11] = 1
+    rec[12] = M
+    rec[16:22] = [0.5 * q(0, 0), q(0, 1), q(0, 2), 0.5 * q(1, 1), q(1, 2), 0.5 * q(2, 2)]
+    rec[22] = math.sqrt(rec[21] ** 2 + 0.5 * rec[20] ** 2 + rec[19] ** 2 + 0.5 * rec[18] ** 2 +
+                        0.5 * rec[17] ** 2 + 0.5 * rec[16] ** 2)
+    return rec
+
+
+def test_derivatives_match_finite_differences_of_softened_potential(orc):
+    eps = 0.04
+    rng = np.random.default_rng(7)
+    for r in (0.01, 0.03, 0.05, 0.3, 2.0):
+        v = rng.normal(size=3)
+        v = v / np.linalg.norm(v) * r
+        d = orc.pole_derivatives(v, eps).astype(np.float64)
+        # D0 is 1/r outside eps; the first derivatives are the gradient of D0
+        hstep = 1e-4 * max(r, eps)
+        grad = []
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = hstep
+            grad.append((orc.pole_derivatives(v + e, eps)[0] - orc.pole_derivatives(v - e, eps)[0]) / (2 * hstep))
+        assert np.allclose(d[1:4], grad, rtol=2e-2, atol=2e-2 * np.abs(grad).max())
+        if r > eps:
+            assert d[0] == pytest.approx(1 / r, rel=1e-6)
+            hess = 3 * np.outer(v, v) / r ** 5 - np.eye(3) / r ** 3
+            assert np.allclose(d[4:10], [hess[0, 0], hess[0, 1], hess[0, 2], hess[1, 1], hess[1, 2], hess[2, 2]],
+                               rtol=1e-4, atol=1e-6 / r ** 3)
+
+
+def test_m2l_of_a_separated_cluster_converges_like_a3(orc):
+    rng = np.random.default_rng(8)
+    src = rng.normal(0, 0.1, (50, 3))
+    m = rng.uniform(0.5, 1.5, 50)
+    rec = _mp_record(src, m)
+    errs, perrs = [], []
+    for dist in (2.0, 4.0, 8.0):
+        x = np.array([dist, 0.3 * dist, -0.2 * dist])
+        local = np.zeros(16, np.float32)
+        local[0:3] = x
+        local = orc.pole_m2l(local, rec, 0.04)
+        acc = local[5:8].astype(np.float64)      # L_X, L_Y, L_Z = field gradient of sum m/r
+        d = x - src
+        exact = -(m[:, None] * d / np.linalg.norm(d, axis=1)[:, None] ** 3).sum(0)
+        errs.append(np.linalg.norm(acc - exact) / np.linalg.norm(exact))
+        phi_exact = (m / np.linalg.norm(d, axis=1)).sum()
+        perrs.append(abs(local[4] - phi_exact) / phi_exact)
+    # at P = 2 the potential carries the quadrupole (error ~ (a/r)^3, x8 per
+    # doubling of r) and the field at the expansion centre only the monopole
+    # derivative (error ~ (a/r)^2, x4 per doubling)
+    assert perrs[0] / perrs[1] > 6 and perrs[1] / perrs[2] > 6
+    assert 3 < errs[0] / errs[1] < 6 and 3 < errs[1] / errs[2] < 6
+    assert errs[2] < 1e-3
+
+
+def test_l2l_is_exact_for_a_quadratic_field(orc):
+    rng = np.random.default_rng(9)
+    local = np.zeros(16, np.float32)
+    local[0:3] = rng.normal(size=3)
+    local[4:14] = rng.normal(size=10)
+    r = rng.normal(0, 0.3, 3).astype(np.float32)
+    shifted = orc.pole_l2l(local, r)
+    assert np.allclose(shifted[0:3], local[0:3] + r, atol=1e-6)
+    for _ in range(10):
+        x = local[0:3] + rng.normal(0, 0.5, 3).astype(np.float32)
+        assert np.allclose(orc.pole_l2p(local, x), orc.pole_l2p(shifted, x), rtol=1e-4, atol=1e-5)
+
+
+def test_shift_multipole_parallel_axis_and_zz_quirk(orc):
+    rng = np.random.default_rng(10)
+    pts = rng.normal(0, 0.2, (30, 3))
+    m = rng.uniform(0.5, 1.5, 30)
+    rec = _mp_record(pts, m)
+    r = np.array([0.3, -0.2, 0.4])
+    moved = _mp_record(pts - rec[0:3] + rec[0:3] - r, m)     # same cloud seen from a centre offset by r
+    fixed = orc.pole_shift_multipole(rec, r, True)
+    # quadrupoles about the new centre = parallel-axis theorem
+    want = _mp_record(np.vstack([pts]), m)[16:22].copy()
+    pa = np.array([0.5 * r[0] ** 2, r[0] * r[1], r[0] * r[2], 0.5 * r[1] ** 2, r[1] * r[2], 0.5 * r[2] ** 2])
+    assert np.allclose(fixed[16:22], want + m.sum() * pa, rtol=1e-5)
+    as_written = orc.pole_shift_multipole(rec, r, False)
+    assert np.allclose(as_written[16:21], fixed[16:21], rtol=1e-6)
+    # M2M.metal:29 multiplies the ZZ shift by the dipole (0 about the COM)
+    assert as_written[21] == pytest.approx(rec[21], rel=1e-5)
+    assert moved is not None
+
+
+def test_mac_special_cases(orc):
+    A = np.zeros(24, np.float32)
+    A[3] = 0.1
+    A[7] = 1.0
+    B = A.copy()
+    B[12] = 1.0
+    far = B.copy()
+    far[0] = 100.0
+    assert orc.pole_mac(A, far, 0.005)              # tiny, far and light: accept
+    near = B.copy()
+    near[0] = 0.15
+    assert not orc.pole_mac(A, near, 0.005)         # overlapping: reject
+    A0 = A.copy()
+    A0[7] = 0.0
+    assert not orc.pole_mac(A0, far, 0.005)         # minGrav 0: nothing accepted
+    ok = [orc.pole_mac(A, np.concatenate([[d], far[1:]]), 0.005) for d in np.geomspace(0.3, 100, 40)]
+    assert ok == sorted(ok)                          # acceptance is monotone in distance
+
+
+def test_direct_sum_two_body_and_shell(orc):
+    pos = np.zeros((2, 4), np.float32)
+    pos[1, 0] = 2.0
+    m = np.array([3.0, 5.0], np.float32)
+    acc, phi = orc.direct_sum(pos, m, np.array([0, 1]), 1.0, 0.0)
+    assert np.allclose(acc[0], [5.0 / 4, 0, 0]) and np.allclose(acc[1], [-3.0 / 4, 0, 0])
+    assert np.allclose(phi, [-5.0 / 2, -3.0 / 2])
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(4000, 3))
+    shell = np.zeros((4001, 4), np.float32)
+    shell[:4000, :3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    shell[4000, :3] = [3.0, 0, 0]
+    mm = np.ones(4001, np.float32)
+    acc, _ = orc.direct_sum(shell, mm, np.array([4000]), 1.0, 0.0)
+    assert acc[0, 0] == pytest.approx(-4000 / 9.0, rel=1e-2)
+
+
+@pytest.fixture(scope="module")
+def fmm_state(sph, orc, tables256):
+    prm = sph.default_params(fixes=sph.SPH_FIX_M2M_ZZ | sph.SPH_FIX_SOFTENING | sph.SPH_FIX_NEAR_FIELD)
+    ic = sph.make_impact(prm, sph.impact_config(3000, 31), tables256)
+    st = orc.State(prm, ic, tables256)
+    st.set_tree(orc.build_octree(ic["pos"], 4, 7))
+    st.gravity()
+    st.gravity()            # second evaluation with the first's |a| in the MAC
+    return st, prm
+
+
+def test_root_multipole_equals_direct_moments_when_fixed(fmm_state):
+    st, _ = fmm_state
+    want = _mp_record(st["pos"][:, :3], st["mass"])
+    root = st.grav["multipoles"][0].astype(np.float64)
+    assert np.allclose(root[0:3], want[0:3], atol=1e-5)
+    assert root[12] == pytest.approx(want[12], rel=1e-6)
+    assert np.allclose(root[16:22], want[16:22], rtol=2e-4, atol=1e-6 * np.abs(want[16:22]).max())
+
+
+def test_root_multipole_zz_differs_as_written(sph, orc, tables256):
+    prm = sph.default_params()
+    ic = sph.make_impact(prm, sph.impact_config(3000, 31), tables256)
+    st = orc.State(prm, ic, tables256)
+    st.set_tree(orc.build_octree(ic["pos"], 4, 7))
+    st.gravity_up()
+    want = _mp_record(st["pos"][:, :3], st["mass"])
+    root = st.grav["multipoles"][0].astype(np.float64)
+    assert np.allclose(root[16:21], want[16:21], rtol=2e-4, atol=1e-6 * np.abs(want[16:22]).max())
+    assert abs(root[21] - want[21]) > 0.1 * abs(want[21])      # M2M.metal:29 (Q2)
+
+
+def test_fmm_against_fp64_direct_sum(fmm_state, orc):
+    """Corrected near field: every pair uses the M2L softening, so the only
+    error left is the P = 2 truncation at the reference's MAC tolerance."""
+    st, prm = fmm_state
+    targets = np.arange(0, st.n, 3)
+    acc, _ = orc.direct_sum(st["pos"], st["mass"], targets, prm.gravity_g, -prm.gravity_softening)
+    a = st["acc"][targets, :3].astype(np.float64)
+    rel = np.linalg.norm(a - acc, axis=1) / np.linalg.norm(acc, axis=1)
+    rms = math.sqrt(np.mean(rel ** 2))
+    assert rms < 2e-2, rms
+    assert np.median(rel) < 1e-2 and rel.max() < 0.2
+
+
+def test_fmm_error_shrinks_with_the_mac_tolerance(sph, orc, tables256):
+    fx = sph.SPH_FIX_M2M_ZZ | sph.SPH_FIX_SOFTENING | sph.SPH_FIX_NEAR_FIELD
+    ic = sph.make_impact(sph.default_params(), sph.impact_config(3000, 32), tables256)
+    tree = orc.build_octree(ic["pos"], 4, 7)
+    meds = []
+    for eta in (0.02, 0.005, 0.001):
+        prm = sph.default_params(fixes=fx, gravity_eta=eta)
+        st = orc.State(prm, ic, tables256)
+        st.set_tree(tree)
+        st.gravity()
+        st.gravity()
+        tg = np.arange(0, 3000, 2)
+        acc, _ = orc.direct_sum(st["pos"], st["mass"], tg, prm.gravity_g, -prm.gravity_softening)
+        rel = np.linalg.norm(st["acc"][tg, :3] - acc, axis=1) / np.linalg.norm(acc, axis=1)
+        meds.append(float(np.median(rel)))
+    assert meds[0] > meds[1] > meds[2]
+
+
+def test_as_written_near_field_is_inaccurate(sph, orc, tables256):
+    """Documents quirk Q5/Q4: force-accepted neighbouring leaves and the two
+    different softening kernels leave O(10%) errors in the as-written FMM."""
+    prm = sph.default_params()
+    ic = sph.make_impact(prm, sph.impact_config(3000, 31), tables256)
+    st = orc.State(prm, ic, tables256)
+    st.set_tree(orc.build_octree(ic["pos"], 4, 7))
+    st.gravity()
+    st.gravity()
+    tg = np.arange(0, 3000, 3)
+    acc, _ = orc.direct_sum(st["pos"], st["mass"], tg, prm.gravity_g, -prm.gravity_softening)
+    rel = np.linalg.norm(st["acc"][tg, :3] - acc, axis=1) / np.linalg.norm(acc, axis=1)
+    assert math.sqrt(np.mean(rel ** 2)) > 0.05
