@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--tree-threads", type=int, default=8)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--no-timing", action="store_true", help="do not record per-pass HIP events")
+    ap.add_argument("--fixes", type=lambda v: int(v, 0), default=0,
+                    help="SPH_FIX_* bit mask (0 = the reference step as written, 0x1f = every correction)")
     return ap.parse_args()
 
 
@@ -102,7 +104,7 @@ def main():
               "the multi-GPU decomposition of this path is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
     sph = importlib.import_module(PKG)
-    prm = sph.default_params()
+    prm = sph.default_params(fixes=args.fixes)
     tables = sph.make_eos_tables(prm, 256)
     cfg = sph.impact_config(args.n, args.seed)
     tree_mode = {"async": sph.TREE_ASYNC, "every": sph.TREE_EVERY, "fixed": sph.TREE_FIXED}[args.tree]
@@ -146,8 +148,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"two-body 45-degree impact, {args.n} particles (89% target / 11% impactor), plain "
-                         "Newton-Raphson smoothing length, FP32, reference step as written, CPU octree "
-                         f"rebuilt on a thread ({args.tree})"),
+                         "Newton-Raphson smoothing length, FP32, "
+                         + ("reference step as written" if args.fixes == 0 else f"corrections 0x{args.fixes:x}")
+                         + f", CPU octree rebuilt on a thread ({args.tree})"),
+            "fixes": args.fixes,
             "n_particles": args.n,
             "eos_table": "256x256 Tillotson per material",
             "tree_mode": args.tree,

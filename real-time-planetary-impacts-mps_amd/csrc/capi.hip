@@ -54,6 +54,7 @@ void free_tree(sph_tree_dev& T) {
     dfree(T.parent_slot);
     dfree(T.parent_level);
     dfree(T.seen);
+    dfree(T.subtree_end);
     dfree(T.node_child);
     dfree(T.mp_far);
     dfree(T.mp_aux);
@@ -386,6 +387,7 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const i
             dfree(T.parent_slot);
             dfree(T.parent_level);
             dfree(T.seen);
+            dfree(T.subtree_end);
             dfree(T.node_child);
             dfree(T.mp_far);
             dfree(T.mp_aux);
@@ -394,6 +396,7 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const i
             SPH_HIP_TRY(dalloc(&T.parent_slot, (size_t)cap));
             SPH_HIP_TRY(dalloc(&T.parent_level, (size_t)cap));
             SPH_HIP_TRY(dalloc(&T.seen, (size_t)cap));
+            SPH_HIP_TRY(dalloc(&T.subtree_end, (size_t)cap));
             SPH_HIP_TRY(dalloc(&T.node_child, (size_t)cap * 8));
             SPH_HIP_TRY(dalloc(&T.mp_far, (size_t)cap * 3));
             SPH_HIP_TRY(dalloc(&T.mp_aux, (size_t)cap * 3));
@@ -459,6 +462,8 @@ int sph_sort(sph_ctx* ctx) {
 int sph_gravity(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
     if (!ctx->have_tree) return ctx_fail(ctx, SPH_E_NOTREE, "gravity before any tree upload");
+    /* the near-field form checks the node numbering in the up pass */
+    if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ctx->tree_check_pending = true;
     int rc = ctx_begin_pass(ctx, SPH_PASS_GRAVITY_UP);
     if (rc == SPH_OK) rc = launch_gravity_up(ctx);
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_GRAVITY_UP);
@@ -504,7 +509,10 @@ int sph_sync(sph_ctx* ctx) {
         ctx->tree_check_pending = false;
         if (flag != 0) {
             ctx->have_tree = false;
-            return ctx_fail(ctx, SPH_E_ARG, "uploaded octree is malformed; gravity was skipped");
+            return ctx_fail(ctx, SPH_E_ARG,
+                            flag == 2u ? "uploaded octree's node ids are not in pre-order (required by the "
+                                         "near-field form); gravity was skipped"
+                                       : "uploaded octree is malformed; gravity was skipped");
         }
     }
     return SPH_OK;

@@ -34,6 +34,7 @@ struct Tree {
     const int4* info;        /* (nParticles, treePointer, level, 0) */
     const int32_t* child;
     const int32_t* parent_slot;
+    int32_t* sub_end;        /* node id + pre-order subtree size */
     float4* far;             /* 3 per node */
     float4* aux;             /* 3 per node */
     float4* loc;             /* 4 per node */
@@ -131,7 +132,7 @@ __global__ void k_tree_check(int node_count, const int4* __restrict__ info, cons
 /* ---------------------------------------------------------------- up pass */
 struct UpArgs {
     Tree t;
-    const uint32_t* flags;
+    uint32_t* flags;
     const int32_t* level_id;
     int count;
     const float4* pos;
@@ -143,6 +144,7 @@ struct UpArgs {
     int readme_rule;
     int fix_zz;
     int fix_range;
+    int need_preorder; /* the near-field down pass walks subtrees as id ranges */
 };
 
 __device__ __forceinline__ float power2(float XX, float XY, float XZ, float YY, float YZ, float ZZ) {
@@ -275,8 +277,25 @@ __global__ __launch_bounds__(256) void k_up(UpArgs a) {
     if (s >= a.count || a.flags[0] != 0u) return;
     const int id = a.level_id[s];
     const int4 inf = a.t.info[id];
-    if (inf.x > 0) p2m(a, id, inf.x, inf.y);
-    else m2m(a, id, inf.y);
+    if (inf.x > 0) {
+        p2m(a, id, inf.x, inf.y);
+        a.t.sub_end[id] = id + 1;
+        return;
+    }
+    m2m(a, id, inf.y);
+    /* Pre-order numbering (octree.cpp:14-109 numbers nodes as it recurses,
+     * children in slot order): the first child is id + 1 and each later
+     * child starts where the previous child's subtree ends. */
+    int end = id + 1;
+    bool ordered = true;
+    for (int c = 0; c < 8; ++c) {
+        const int cid = a.t.child[8 * id + c];
+        if (cid < 0) continue;
+        ordered = ordered && cid == end;
+        end = a.t.sub_end[cid];
+    }
+    a.t.sub_end[id] = end;
+    if (!ordered && a.need_preorder) a.flags[0] = 2u;
 }
 
 /* -------------------------------------------------------------- down pass */
@@ -293,6 +312,7 @@ struct DownArgs {
     float eps;        /* GRAVITY_SMOOTHING_LENGTH */
     float G;
     int fix_soft;
+    int near;         /* near-field form: P2P with every rejected source leaf, one softened kernel */
     const float4* pos;
     const float* m;
     const uint8_t* active;
@@ -432,6 +452,19 @@ __device__ __forceinline__ float dphi_dr(float r, float h, int fixed) {
     return r1 * r1;
 }
 
+/* -d(potential)/dr of a unit mass for the softened potential whose
+ * derivatives M2L uses (derivatives.metal:12-57, support eps).  The
+ * near-field form sums every pair with this kernel, so a pair's force does
+ * not depend on whether it is reached by M2L or by P2P. */
+__device__ __forceinline__ float dphi_dr_m2l(float r, float eps) {
+    if (r >= eps) return 1.f / (r * r);
+    const float eps_inv = 1.f / eps;
+    const float u = r * eps_inv;
+    float s2 = -21.f * u + 90.f;
+    s2 = s2 * u - 140.f; s2 = s2 * u + 84.f; s2 = s2 * u; s2 = s2 * u - 14.f; s2 = s2 * u;
+    return -(eps_inv * eps_inv) * s2;
+}
+
 /* transformLocal (L2L.metal:12-88) of the node's local expansion to one
  * active child, written to the child's record (L2L.metal:90-104). */
 __device__ __forceinline__ void l2l_child(const DownArgs& a, const LocalExp& L, int cid) {
@@ -452,8 +485,10 @@ __device__ __forceinline__ void l2l_child(const DownArgs& a, const LocalExp& L, 
     a.t.loc[4 * cid + 3] = make_float4(L.e[eZZ], 0.f, 0.f, 0.f);
 }
 
-/* L2P + intra-leaf P2P for particle k of a leaf (helpers.h:47-81). */
-__device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp& L, const int32_t* pl, int np, int k) {
+/* L2P + intra-leaf P2P for particle k of a leaf (helpers.h:47-81); (nx, ny,
+ * nz) is the near-field form's sum over rejected source leaves (else 0). */
+__device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp& L, const int32_t* pl, int np, int k,
+                                              float nx = 0.f, float ny = 0.f, float nz = 0.f) {
     const int p = pl[k];
     if (!a.active[p]) return;
     const float4 xi = a.pos[p];
@@ -470,11 +505,14 @@ __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp&
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float r = sqrtf(dx * dx + dy * dy + dz * dz);
         if (r == 0.f) continue;
-        const float w = a.m[pj] * dphi_dr(r, a.eps, a.fix_soft);
+        const float w = a.m[pj] * (a.near ? dphi_dr_m2l(r, a.eps) : dphi_dr(r, a.eps, a.fix_soft));
         ax -= w * (dx / r);
         ay -= w * (dy / r);
         az -= w * (dz / r);
     }
+    ax += nx;
+    ay += ny;
+    az += nz;
     a.acc[p] = make_float4(a.G * ax, a.G * ay, a.G * az, 0.f);
     a.gravnorm[p] = sqrtf(ax * ax + ay * ay + az * az);
 }
@@ -536,6 +574,107 @@ __global__ __launch_bounds__(256) void k_down(DownArgs a) {
 }
 
 
+/* Group reduction of one value over GS lanes (fixed order: reproducible). */
+template <int GS>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int off = GS / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, GS);
+    return v;
+}
+
+/* Near-field form at a leaf (GS lanes, one leaf).  Each lane walks the
+ * subtrees of its list entries in pre-order without a stack: a node is
+ * M2L'd if the MAC accepts it (skip its subtree), a rejected leaf is summed
+ * particle by particle (P2P), a rejected branch is opened (next id).  The
+ * targets are the leaf's particles, NEAR_K at a time in registers; the M2L
+ * part runs in the first pass only.  Partial sums are reduced over the group,
+ * the incoming local expansion is added, and lane k finishes particle k. */
+template <int GS>
+__device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA& A, LocalExp& L, float4 l0,
+                               const int32_t* in, int lane, int wl, uint64_t gmask) {
+    constexpr int NEAR_K = 4;
+    const int np = inf.x;
+    const int32_t* pl = a.t.octree + inf.y + 2;
+    for (int k0 = 0; k0 < np; k0 += NEAR_K) {
+        float tx[NEAR_K], ty[NEAR_K], tz[NEAR_K], gx[NEAR_K], gy[NEAR_K], gz[NEAR_K];
+#pragma unroll
+        for (int kk = 0; kk < NEAR_K; ++kk) {
+            const float4 x = k0 + kk < np ? a.pos[pl[k0 + kk]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            tx[kk] = x.x;
+            ty[kk] = x.y;
+            tz[kk] = x.z;
+            gx[kk] = gy[kk] = gz[kk] = 0.f;
+        }
+        bool more = true;
+        for (int base = 0; more; base += GS) {
+            const int i = base + lane;
+            const int e = i < a.maxu ? in[i] : -1;
+            const uint64_t term = __ballot(e == -1) & gmask;
+            const uint64_t first = term & (~term + 1ull);
+            const bool valid = i < a.maxu && (term == 0ull || (1ull << wl) < first);
+            more = term == 0ull && base + GS < a.maxu;
+            if (!valid) continue;
+            int cur = e;
+            const int end = a.t.sub_end[e];
+            while (cur < end) {
+                if (cur == id) {
+                    ++cur;
+                    continue;
+                }
+                const float4 b0 = a.t.far[3 * cur], b1 = a.t.far[3 * cur + 1], b2 = a.t.far[3 * cur + 2];
+                if (mac_accept(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.eta)) {
+                    if (k0 == 0) m2l_add(L, b0, b1, b2, a.eps);
+                    cur = a.t.sub_end[cur];
+                    continue;
+                }
+                const int4 ci = a.t.info[cur];
+                ++cur;
+                if (ci.x == 0) continue;
+                const int32_t* src = a.t.octree + ci.y + 2;
+                for (int q = 0; q < ci.x; ++q) {
+                    const int pj = src[q];
+                    const float4 xj = a.pos[pj];
+                    const float mj = a.m[pj];
+#pragma unroll
+                    for (int kk = 0; kk < NEAR_K; ++kk) {
+                        const float dx = tx[kk] - xj.x, dy = ty[kk] - xj.y, dz = tz[kk] - xj.z;
+                        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+                        if (k0 + kk >= np || r == 0.f) continue;
+                        const float w = mj * dphi_dr_m2l(r, a.eps);
+                        gx[kk] -= w * (dx / r);
+                        gy[kk] -= w * (dy / r);
+                        gz[kk] -= w * (dz / r);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < NEAR_K; ++kk) {
+            gx[kk] = group_sum<GS>(gx[kk]);
+            gy[kk] = group_sum<GS>(gy[kk]);
+            gz[kk] = group_sum<GS>(gz[kk]);
+        }
+        if (k0 == 0) {
+#pragma unroll
+            for (int k = 0; k < eNT; ++k) L.e[k] = group_sum<GS>(L.e[k]);
+            const float4 l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2], l3 = a.t.loc[4 * id + 3];
+            L.e[eM] = l0.w + L.e[eM];
+            L.e[eX] = l1.x + L.e[eX];
+            L.e[eY] = l1.y + L.e[eY];
+            L.e[eZ] = l1.z + L.e[eZ];
+            L.e[eXX] = l1.w + L.e[eXX];
+            L.e[eXY] = l2.x + L.e[eXY];
+            L.e[eXZ] = l2.y + L.e[eXZ];
+            L.e[eYY] = l2.z + L.e[eYY];
+            L.e[eYZ] = l2.w + L.e[eYZ];
+            L.e[eZZ] = l3.x + L.e[eZZ];
+        }
+#pragma unroll
+        for (int kk = 0; kk < NEAR_K; ++kk)
+            if (lane == kk && k0 + kk < np) leaf_particle(a, L, pl, np, k0 + kk, gx[kk], gy[kk], gz[kk]);
+    }
+}
+
 /* Down pass for GRAVITY_MAX_RECURSION == 1 with GS lanes per node.
  *
  * Lanes test GS list entries at a time against the node (MAC); accepted
@@ -550,7 +689,7 @@ __global__ __launch_bounds__(256) void k_down(DownArgs a) {
  * partial local expansion; the group reduces them with shuffles (a fixed
  * order, so results are reproducible), adds the incoming local expansion and
  * runs L2L (lane c -> child c) or L2P + P2P (lane k -> particle k). */
-template <int GS>
+template <int GS, bool NEAR>
 __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     constexpr int GPB = 256 / GS;
     constexpr int EPR = GS / 8;
@@ -580,13 +719,19 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     L.pz = l0.z;
 #pragma unroll
     for (int k = 0; k < eNT; ++k) L.e[k] = 0.f;
+    const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
+    if constexpr (NEAR) {
+        if (live && np != 0) {
+            down_near_leaf<GS>(a, id, inf, A, L, l0, in, lane, wl, gmask);
+            return;
+        }
+    }
     const uint32_t outbase = (uint32_t)s * (uint32_t)a.maxu;
     int running = 0;
     if (live && np == 0) {
         if (lane == 0) a.out[outbase] = id;
         running = 1;
     }
-    const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
     bool more = live;
     for (int base = 0; more; base += GS) {
         const int i = base + lane;
@@ -609,14 +754,20 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
                 }
             }
             const int ent = __shfl(e, (my_src < 0 ? wl : my_src) - gbase, GS);
-            const int child = my_src >= 0 ? a.t.child[8 * ent + (lane & 7)] : -1;
+            /* near-field form: a rejected leaf entry is pushed itself (by the
+             * first lane of its eight) */
+            const bool leaf_ent = NEAR && my_src >= 0 && a.t.info[ent].x != 0;
+            int child = -1;
+            if (leaf_ent) child = (lane & 7) == 0 ? ent : -1;
+            else if (my_src >= 0) child = a.t.child[8 * ent + (lane & 7)];
             bool take = false, pushc = false;
             float4 c0, c1, c2;
             if (child >= 0 && child != id) {
                 c0 = a.t.far[3 * child];
                 c1 = a.t.far[3 * child + 1];
                 c2 = a.t.far[3 * child + 2];
-                if (np != 0 || a.t.info[child].x != 0) take = true;
+                if (leaf_ent) pushc = true;
+                else if (!NEAR && (np != 0 || a.t.info[child].x != 0)) take = true;
                 else if (mac_accept(A.x, A.y, A.z, A.size, A.minGrav, c0, c1, c2, a.eta)) take = true;
                 else pushc = true;
             }
@@ -632,12 +783,7 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     }
     if (live && np == 0 && lane == 0 && running < a.maxu) a.out[outbase + (uint32_t)running] = -1;
 #pragma unroll
-    for (int k = 0; k < eNT; ++k) {
-        float v = L.e[k];
-#pragma unroll
-        for (int off = GS / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, GS);
-        L.e[k] = v;
-    }
+    for (int k = 0; k < eNT; ++k) L.e[k] = group_sum<GS>(L.e[k]);
     if (!live) return;
     {
         const float4 l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2], l3 = a.t.loc[4 * id + 3];
@@ -695,6 +841,7 @@ Tree make_tree(sph_ctx* ctx) {
     t.info = ctx->tree.node_info;
     t.child = ctx->tree.node_child;
     t.parent_slot = ctx->tree.parent_slot;
+    t.sub_end = ctx->tree.subtree_end;
     t.far = ctx->tree.mp_far;
     t.aux = ctx->tree.mp_aux;
     t.loc = ctx->tree.loc;
@@ -754,6 +901,7 @@ int launch_gravity_up(sph_ctx* ctx) {
     a.readme_rule = ctx->prm.activity_rule == SPH_ACTIVE_README;
     a.fix_zz = (ctx->prm.fixes & SPH_FIX_M2M_ZZ) ? 1 : 0;
     a.fix_range = (ctx->prm.fixes & SPH_FIX_SYNC_RANGE) ? 1 : 0;
+    a.need_preorder = (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ? 1 : 0;
     const int threads = 256;
     for (int l = (int)T.level_counts.size() - 1; l >= 0; --l) {
         a.count = T.level_counts[l];
@@ -777,6 +925,7 @@ int launch_gravity_down(sph_ctx* ctx) {
     a.eps = p.gravity_softening;
     a.G = p.gravity_g;
     a.fix_soft = (p.fixes & SPH_FIX_SOFTENING) ? 1 : 0;
+    a.near = (p.fixes & SPH_FIX_NEAR_FIELD) ? 1 : 0;
     a.pos = ctx->pos;
     a.m = ctx->m;
     a.active = ctx->active;
@@ -797,10 +946,12 @@ int launch_gravity_down(sph_ctx* ctx) {
         a.level_id = T.level_id + T.level_offsets[l];
         a.in = in;
         a.out = out;
-        if (a.max_rec == 1) {
+        /* the near-field form has no recursion limit (leaves walk whole subtrees) */
+        if (a.max_rec == 1 || a.near) {
             constexpr int GS = 32;
             const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
-            k_down_group<GS><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
+            if (a.near) k_down_group<GS, true><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
+            else k_down_group<GS, false><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
         } else {
             k_down<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
         }

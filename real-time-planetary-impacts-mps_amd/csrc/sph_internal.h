@@ -23,6 +23,7 @@
  *          mp_aux  3 x float4 x K  (min, minGrav) (max, active) (X, Y, Z, -) dipoles
  *          loc     4 x float4 x K  (pos, L_M) (L_X, L_Y, L_Z, L_XX) (L_XY, L_XZ, L_YY, L_YZ) (L_ZZ, -, -, -)
  *          node_info int4 x K (nParticles, treePointer, level, -), parent_slot int32 x K
+ *          subtree_end int32 x K (first node id after the node's pre-order subtree)
  *          node_child int32 x 8K (child node ids, -1)
  *          lists a/b int32 x (max nodes of a non-last level) x max_unchecked_pointers
  */
@@ -57,6 +58,7 @@ struct sph_tree_dev {
     int32_t* parent_slot = nullptr; /* slot of the parent in its level list (parentIndexes) */
     int32_t* parent_level = nullptr;/* level of the parent (validation) */
     int32_t* seen = nullptr;        /* times each node id occurs in the level lists (validation) */
+    int32_t* subtree_end = nullptr; /* node id + pre-order subtree size (written by the up pass) */
     int64_t n_ints = 0;             /* size of the current octreeData */
     int32_t* node_child = nullptr;  /* 8 per node */
     float4* mp_far = nullptr;       /* 3 per node */

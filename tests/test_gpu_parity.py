@@ -230,6 +230,97 @@ def test_gravity_small_list(sph, orc, tables256, impact):
     assert act_same and q[0] < 1e-5 and q[2] < 1e-2
 
 
+def _near_fixes(sph):
+    return sph.SPH_FIX_M2M_ZZ | sph.SPH_FIX_SOFTENING | sph.SPH_FIX_NEAR_FIELD
+
+
+@pytest.mark.parametrize("n,seed", [(20000, 7), (100000, 8)])
+def test_gravity_near_field_parity(sph, orc, tables256, impact, n, seed):
+    """Near-field form (rejected leaves summed pair by pair with the M2L
+    softening) against the oracle, cold (gravNorm = 0) and warm."""
+    ic = impact(n, seed)
+    prm, ctx, st = _pair(sph, orc, ic, tables256, fixes=_near_fixes(sph))
+    tree = orc.build_octree(ic["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+    st.set_tree(tree)
+    ctx.upload_tree(tree)
+    st.gravity()
+    ctx.gravity()
+    ctx.sync()
+    mstat, act_same, q = _grav_compare(ctx, st, f"near cold n={n}")
+    assert act_same and mstat["M"] < 1e-5 and mstat["quad"] < 1e-4
+    assert q[0] < 1e-5 and q[2] < 1e-3
+    ctx.upload("gravnorm", st["gravnorm"])
+    st.gravity()
+    ctx.gravity()
+    ctx.sync()
+    _, act_same, q = _grav_compare(ctx, st, f"near warm n={n}")
+    assert act_same and q[0] < 1e-5 and q[2] < 1e-3
+
+
+def test_gravity_near_field_small_list(sph, orc, tables256, impact):
+    """List-full path of the near-field form (leaf entries pushed themselves)."""
+    ic = impact(20000, 9)
+    prm, ctx, st = _pair(sph, orc, ic, tables256, fixes=_near_fixes(sph), max_unchecked_pointers=16)
+    tree = orc.build_octree(ic["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+    st.set_tree(tree)
+    ctx.upload_tree(tree)
+    st.gravity()
+    ctx.upload("gravnorm", st["gravnorm"])
+    st.gravity()
+    ctx.gravity()
+    ctx.sync()
+    _, act_same, q = _grav_compare(ctx, st, "near maxu16")
+    assert act_same and q[0] < 1e-5 and q[2] < 1e-3
+
+
+def test_gravity_accuracy_against_fp64_direct_sum(sph, orc, tables256, impact):
+    """GPU gravity at 1e5 particles against an FP64 direct sum at 4096
+    sampled particles.  The as-written form mixes two softening kernels (the
+    P2P spline of support 2 eps and the M2L potential of support eps), which
+    leaves large errors for close pairs in different leaves; the near-field
+    form sums every pair with the M2L softening and is left with the P = 2
+    truncation error at GRAVITY_ETA."""
+    n = 100000
+    ic = impact(n, 10)
+    tree = orc.build_octree(ic["pos"], 4, 7)
+    targets = np.random.default_rng(0).choice(n, 4096, replace=False).astype(np.int32)
+    out = {}
+    for name, fx, soft_sign in (("as-written", 0, 1.0), ("near", _near_fixes(sph), -1.0)):
+        prm = sph.default_params(fixes=fx)
+        ctx = sph.Context(prm, ic, tables256)
+        ctx.upload_tree(tree)
+        ctx.gravity()
+        ctx.gravity()          # warm: minGrav from the first evaluation
+        ctx.sync()
+        a = ctx.download("acc")[targets, :3].astype(np.float64)
+        ref, _ = orc.direct_sum(ic["pos"], ic["mass"], targets, prm.gravity_g, soft_sign * prm.gravity_softening)
+        rel = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        out[name] = (float(np.sqrt(np.mean(rel ** 2))), float(np.median(rel)), float(rel.max()))
+        print(f"[gravity accuracy {name}] rms/median/max = {out[name]}", flush=True)
+        ctx.close()
+    rms, med, mx = out["near"]
+    assert rms < 2e-2 and med < 1e-2 and mx < 0.2
+    assert out["as-written"][0] > 5 * rms
+
+
+def test_near_field_rejects_trees_not_in_pre_order(sph, orc, tables256, impact):
+    ic = impact(5000, 11)
+    tree = orc.build_octree(ic["pos"], 4, 7)
+    oct_ = tree["octree"]
+    leaves = [tp for tp in tree["level_nodes"] if oct_[tp] > 0][:2]
+    oct_[leaves[0] + 1], oct_[leaves[1] + 1] = oct_[leaves[1] + 1], oct_[leaves[0] + 1]
+    ctx = sph.Context(sph.default_params(fixes=_near_fixes(sph)), ic, tables256)
+    ctx.upload_tree(tree)
+    ctx.gravity()
+    with pytest.raises(sph.SphError, match="pre-order"):
+        ctx.sync()
+    # the as-written form does not need the numbering
+    ctx2 = sph.Context(sph.default_params(), ic, tables256)
+    ctx2.upload_tree(tree)
+    ctx2.gravity()
+    ctx2.sync()
+
+
 # ------------------------------------------------------- whole steps
 def test_headless_steps_match_oracle(sph, orc, tables256, impact):
     """Five Headless steps (tree rebuilt before every step) against the oracle."""
