@@ -47,3 +47,23 @@ def impact(sph, tables256):
             _IC_CACHE[key] = sph.make_impact(prm, sph.impact_config(n, seed, headon=headon), tables256)
         return {k: v.copy() for k, v in _IC_CACHE[key].items()}
     return make
+
+
+@pytest.fixture(scope="session")
+def demo_ic_raw():
+    """The reference's demo initial conditions (SPH/demo_impact_n40.hdf5,
+    10,766 particles), converted by tests/golden/make_demo_ic.py."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "demo_impact_n40.npz"))
+    return {k: d[k] for k in d.files}
+
+
+@pytest.fixture
+def demo_ic(demo_ic_raw):
+    """Demo IC in the sph_ic_view layout (16-byte float3 stride)."""
+    n = demo_ic_raw["mass"].shape[0]
+    ic = {k: v.copy() for k, v in demo_ic_raw.items() if k not in ("pos", "vel")}
+    for k in ("pos", "vel"):
+        a = np.zeros((n, 4), np.float32)
+        a[:, :3] = demo_ic_raw[k]
+        ic[k] = a
+    return ic
