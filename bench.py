@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--tree-threads", type=int, default=8)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--no-timing", action="store_true", help="do not record per-pass HIP events")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES bytes 5 times (calibrates the "
+                         "HBM byte counters of a profiler against a kernel of known traffic) and exit")
     ap.add_argument("--fixes", type=lambda v: int(v, 0), default=0,
                     help="SPH_FIX_* bit mask (0 = the reference step as written, 0x1f = every correction)")
     return ap.parse_args()
@@ -96,8 +99,24 @@ def cpu_baseline(sph, oracle, hl, prm, tables, steps):
     }
 
 
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size: reads nbytes, writes nbytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for i in range(5):
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy {i} done")
+    print(json.dumps({"copy_probe_bytes": src.numel() * 4, "launches": 5}))
+
+
 def main():
     args = parse()
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
