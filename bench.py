@@ -57,9 +57,27 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes 5 times (calibrates the "
                          "HBM byte counters of a profiler against a kernel of known traffic) and exit")
+    ap.add_argument("--learned-steps", type=int, default=2000,
+                    help="Adam steps of the learned-h network (0 = skip the learned-h comparison)")
+    ap.add_argument("--learned-eta", type=float, default=1e-5,
+                    help="Newton-Raphson tolerance |dh/h| of the learned-h comparison")
     ap.add_argument("--fixes", type=lambda v: int(v, 0), default=0,
                     help="SPH_FIX_* bit mask (0 = the reference step as written, 0x1f = every correction)")
     return ap.parse_args()
+
+
+def profiled_traffic():
+    """HBM bytes per k_density launch at the default workload, measured by
+    two rocprofv3 counter passes and recorded in profiles/."""
+    path = os.path.join(ROOT, "profiles", "k_density_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def density_flops(work):
@@ -97,6 +115,82 @@ def cpu_baseline(sph, oracle, hl, prm, tables, steps):
                    f"(sort, gravity, density, acceleration, step), OpenMP over {threads} threads; the serial "
                    f"octree build ({t_tree:.2f} s) is excluded as it runs on a separate thread in the GPU path"),
     }
+
+
+def learned_comparison(sph, args):
+    """Learned-h against plain Newton-Raphson at a tightened tolerance.
+
+    Samples (features, converged h) are dumped by plain runs of seeded 1e5
+    (seed 11) and N-particle (seed 12) impacts, the network is trained on the
+    GPU (Adam, batch 65536), then the density pass of the benchmark's own
+    impact (seed --seed, after 3 steps) runs twice from the same state:
+    plain, and from the network's guess.  Both converge to |dh/h| <=
+    --learned-eta; the pass time includes the feature and inference kernels.
+    """
+    prm = sph.default_params(density_eta=args.learned_eta, fixes=args.fixes)
+    dump_steps = 8
+    sizes = ((100_000, 11), (args.n, 12))
+    smp = sph.Samples(sum(n for n, _ in sizes) * dump_steps)
+    t0 = time.perf_counter()
+    for n, seed in sizes:
+        hl = sph.Headless(prm, sph.impact_config(n, seed), eos_res=256, tree_mode=sph.TREE_EVERY,
+                          tree_threads=args.tree_threads)
+        hl.step()
+        hl.ctx.set_sample_dump(smp)
+        for _ in range(dump_steps):
+            hl.step()
+        hl.ctx.set_sample_dump(None)
+        hl.close()
+    t_dump = time.perf_counter() - t0
+    log(f"learned: {len(smp)} samples dumped in {t_dump:.1f} s")
+    net = sph.Mlp(seed=13)
+    net.fit_norm(smp)
+    t0 = time.perf_counter()
+    losses = net.train(smp, steps=args.learned_steps, batch=65536, lr=2e-3, seed=14)
+    t_train = time.perf_counter() - t0
+    log(f"learned: {args.learned_steps} Adam steps in {t_train:.2f} s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+
+    hl = sph.Headless(prm, sph.impact_config(args.n, args.seed), eos_res=256, tree_mode=sph.TREE_EVERY,
+                      tree_threads=args.tree_threads)
+    for _ in range(3):
+        hl.step()
+    ctx = hl.ctx
+    ctx.sort()
+    ctx.sync()
+    keep = {k: ctx.download(k) for k in ("h", "rho", "global_time", "dt")}
+    reps = 5
+    out = {}
+    for mode in ("plain", "learned"):
+        ctx.set_learned_guess(net if mode == "learned" else None)
+        ctx.reset_pass_times()
+        ctx.set_timing(True)
+        for _ in range(reps):
+            for k, v in keep.items():
+                ctx.upload(k, v)
+            ctx.density()
+        t = ctx.pass_times()["density"]
+        ctx.set_timing(False)
+        out[mode] = {"ms": t["ms"] / max(t["calls"], 1), "h": ctx.download("h").astype(np.float64),
+                     "it": float(ctx.download("nr_iters").mean())}
+    ctx.set_learned_guess(None)
+    hl.close()
+    dh = np.abs(out["learned"]["h"] / out["plain"]["h"] - 1)
+    res = {
+        "nr_tolerance": args.learned_eta,
+        "density_ms_plain": round(out["plain"]["ms"], 4),
+        "density_ms_learned": round(out["learned"]["ms"], 4),
+        "density_speedup": round(out["plain"]["ms"] / out["learned"]["ms"], 4),
+        "nr_iterations_plain": round(out["plain"]["it"], 4),
+        "nr_iterations_learned": round(out["learned"]["it"], 4),
+        "converged_h_rel_diff_max": float(dh.max()),
+        "converged_h_rel_diff_q999": float(np.quantile(dh, 0.999)),
+        "train": {"samples": len(smp), "adam_steps": args.learned_steps, "batch": 65536, "lr": 2e-3,
+                  "seconds": round(t_train, 3), "loss_first": float(losses[0]),
+                  "loss_last10_mean": float(np.mean(losses[-10:])), "dump_seconds": round(t_dump, 2)},
+    }
+    smp.close()
+    net.close()
+    return res
 
 
 def copy_probe(nbytes):
@@ -194,6 +288,7 @@ def main():
             "final_neighbours": round(float(work[:, 3].mean()), 1),
         }
         flops = density_flops(work)
+        traffic_bytes = profiled_traffic() if args.n == 1_000_000 and args.fixes == 0 else None
         t = per_pass["density"] / 1000.0
         achieved = flops / t / 1e12
         bytes_alg = DENSITY_BYTES_PER_PARTICLE * args.n
@@ -205,12 +300,18 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
-            "traffic": None,
+            "traffic": traffic_bytes,
+            "traffic_source": "profiles/k_density_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
+                              "copy-calibrated)" if traffic_bytes else None,
             "algorithmic_flop_per_launch": flops,
             "algorithmic_bytes_per_launch": bytes_alg,
             "hbm_frac_of_algorithmic_bytes": round(bytes_alg / t / 1e9 / PEAK_HBM_GBS, 6),
             "avg_launch_ms": round(per_pass["density"], 4),
         }
+    if args.learned_steps > 0:
+        log("learned-h comparison")
+        out["learned_h"] = learned_comparison(sph, args)
+        log("learned-h comparison done")
     if args.cpu_steps > 0:
         import oracle
         log("cpu baseline")

@@ -81,7 +81,13 @@ typedef struct sph_params {
     int32_t cell_power;         /* CELL_POWER: 2^(3*cell_power) cells, 1..8 */
     /* Corrections and forms */
     uint32_t fixes;             /* SPH_FIX_* bit set */
-    int32_t reserved[8];
+    /* Precision form (no counterpart in the reference; needs
+     * SPH_FIX_NEAR_FIELD): compensated (Neumaier) sums in the M2L, L2P and
+     * P2P accumulators, centres of mass summed relative to a member in
+     * P2M/M2M, and the gravitational potential per particle (SPH_F_POTENTIAL)
+     * for the FP64 conservation sums of sph_conservation. */
+    int32_t precision;
+    int32_t reserved[7];
 } sph_params;
 
 /* One EOS table: res x res float2 (pressure, sound speed) in code units.
@@ -186,6 +192,8 @@ enum sph_field {
     SPH_F_DENSITY_WORK,   /* int32 x 4n: neighbour candidates examined and candidates inside 2h
                              over the NR scans, then the same two counts for the final pass,
                              of the last density pass (diagnostic) */
+    SPH_F_POTENTIAL,      /* float x n: gravitational potential (G included) of the last
+                             gravity pass in the precision form, else 0 */
     SPH_F_COUNT
 };
 /* Multipole record (24 floats, poles.h:46-55 reordered):
@@ -211,6 +219,82 @@ int sph_set_timing(sph_ctx* ctx, int enabled);
  * `ms` has SPH_PASS_COUNT entries; `calls` (optional) counts the passes. */
 int sph_pass_times(sph_ctx* ctx, double* ms, int64_t* calls);
 int sph_reset_pass_times(sph_ctx* ctx);
+
+/* ------------------------------------------------------------ diagnostics */
+/* FP64 sums over all particles, reduced on the device (no counterpart in
+ * the reference): out[0] kinetic energy sum m v^2 / 2, out[1] internal
+ * energy sum m u, out[2] potential energy sum m phi / 2 (precision form;
+ * 0 otherwise), out[3..5] momentum sum m v, out[6..8] angular momentum
+ * sum m x cross v about the origin, out[9] total mass. */
+#define SPH_CONSERVATION_COUNT 10
+int sph_conservation(sph_ctx* ctx, double* out);
+/* Simulation time in seconds: the integer tick counter (ticks of
+ * DT_MIN1 = DT0 / 2^R_MAX, exact) times DT_MIN1, in FP64. */
+int sph_time(sph_ctx* ctx, double* seconds);
+
+/* ------------------------------------------------ learned smoothing length */
+/* None of this has a counterpart in the reference.  A small network
+ * (12 -> 64 -> 64 -> 1, SiLU, BF16 matrix-core products with FP32
+ * accumulation) predicts d = log(h_converged) - log(h_start) from 12
+ * features of a particle at the start of a density pass; the density pass
+ * then starts its Newton-Raphson iteration (density.metal:73-143) from
+ * h_start exp(d) instead of h_start, and falls back to h_start when the
+ * residual grows.  Convergence is still decided by sph_params.density_eta.
+ * Features: 0 log h, 1 (dh/dt) dt / h (dt = DT_MIN1 x the last tick step),
+ * 2 log rho, 3-7 log(1 + count) over the 27 cells around the particle's cell
+ * (all, own, 6 faces, 12 edges, 8 corners), 8 |sum count x offset| /
+ * (n27 + 1), 9 (max - min count) / (n27 + 1), 10-11 material one-hot
+ * (400, other).  Parameters (5057 floats): W1[64][12], b1[64], W2[64][64],
+ * b2[64], w3[64], b3.  Normalisation (26 floats): feature means[12],
+ * feature scales[12], target mean, target scale.  Sample record (16 floats):
+ * 12 features, d, weight, 2 unused. */
+#define SPH_MLP_INPUTS 12
+#define SPH_MLP_HIDDEN 64
+#define SPH_MLP_PARAMS 5057
+#define SPH_MLP_NORM 26
+#define SPH_SAMPLE_FLOATS 16
+
+typedef struct sph_mlp sph_mlp;
+typedef struct sph_samples sph_samples;
+
+/* Network on the current device, weights drawn from
+ * Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) with a splitmix64 stream. */
+int sph_mlp_create(uint64_t seed, sph_mlp** out);
+void sph_mlp_destroy(sph_mlp* m);
+const char* sph_mlp_last_error(const sph_mlp* m);
+/* Parameters / normalisation in and out (either pointer may be NULL).
+ * Setting parameters resets the Adam state. */
+int sph_mlp_get(sph_mlp* m, float* params, float* norm);
+int sph_mlp_set(sph_mlp* m, const float* params, const float* norm);
+/* Prediction d for n feature rows (host arrays, n x 12 -> n). */
+int sph_mlp_forward(sph_mlp* m, const float* x, int64_t n, float* d_out);
+/* Loss mean_s w_s (y_s - (d_s - mean) / scale)^2 over the n samples and its
+ * gradient (5057 floats), computed by the training kernels. */
+int sph_mlp_loss_grad(sph_mlp* m, const float* x, const float* d, const float* w, int64_t n,
+                      float* loss, float* grad);
+/* Normalisation from the weighted samples (means and standard deviations). */
+int sph_mlp_fit_norm(sph_mlp* m, const sph_samples* s);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on batches of `batch`
+ * samples (multiple of 32) drawn with a counter hash of `seed`; `losses`
+ * (optional, `steps` floats) receives the loss of every step. */
+int sph_mlp_train(sph_mlp* m, const sph_samples* s, int steps, int batch, float lr, uint64_t seed,
+                  float* losses);
+
+/* Device buffer of sample records. */
+int sph_samples_create(int64_t capacity, sph_samples** out);
+void sph_samples_destroy(sph_samples* s);
+int64_t sph_samples_count(const sph_samples* s);
+int sph_samples_read(const sph_samples* s, int64_t first, int64_t count, float* dst);
+int sph_samples_append(sph_samples* s, const float* src, int64_t count);
+int sph_samples_clear(sph_samples* s);
+
+/* With a sample buffer set, every density pass appends one record per
+ * particle (weight 1 if active, else 0); NULL switches dumping off. */
+int sph_set_sample_dump(sph_ctx* ctx, sph_samples* s);
+/* With a network set, density passes start from its guess; NULL = plain. */
+int sph_set_learned_guess(sph_ctx* ctx, sph_mlp* m);
+/* Features of the current state (n x 12, original order); needs a sort. */
+int sph_features(sph_ctx* ctx, float* dst);
 
 #ifdef __cplusplus
 }

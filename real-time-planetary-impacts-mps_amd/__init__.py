@@ -36,7 +36,7 @@ FIELDS = [
     "pos", "vel", "acc", "rho", "u", "mass", "pressure", "h", "mat", "gradterm",
     "soundspeed", "du", "balsara", "dhdt", "active", "next_active", "gravnorm",
     "cell_array", "cell_start", "cell_end", "global_time", "dt", "nr_iters",
-    "multipoles", "locals", "density_work",
+    "multipoles", "locals", "density_work", "potential",
 ]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 _F4 = {"pos", "vel", "acc"}
@@ -58,7 +58,7 @@ class SphParams(ctypes.Structure):
         ("gravity_max_recursion", ctypes.c_int32), ("max_unchecked_pointers", ctypes.c_int32),
         ("max_children_in_leaf", ctypes.c_int32), ("max_tree_depth", ctypes.c_int32),
         ("cell_width", ctypes.c_float), ("cell_power", ctypes.c_int32),
-        ("fixes", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 8),
+        ("fixes", ctypes.c_uint32), ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -104,6 +104,10 @@ CAPI_SYMBOLS = [
     "sph_tree_upload_done", "sph_sort", "sph_gravity", "sph_density", "sph_acceleration", "sph_step",
     "sph_sync", "sph_field_bytes", "sph_download", "sph_upload", "sph_set_timing", "sph_pass_times",
     "sph_reset_pass_times",
+    "sph_mlp_create", "sph_mlp_destroy", "sph_mlp_last_error", "sph_mlp_get", "sph_mlp_set", "sph_mlp_forward",
+    "sph_mlp_loss_grad", "sph_mlp_fit_norm", "sph_mlp_train", "sph_samples_create", "sph_samples_destroy",
+    "sph_samples_count", "sph_samples_read", "sph_samples_append", "sph_samples_clear", "sph_set_sample_dump",
+    "sph_set_learned_guess", "sph_features", "sph_conservation", "sph_time",
 ]
 HOST_SYMBOLS = [
     "sph_tillotson_granite", "sph_tillotson_iron", "sph_make_eos_table", "sph_impact_config_default",
@@ -160,6 +164,32 @@ def libs():
         h.sph_host_alloc.restype = ctypes.c_void_p
         h.sph_host_free.argtypes = [ctypes.c_void_p]
         h.sph_host_free.restype = None
+        F = ctypes.POINTER(ctypes.c_float)
+        h.sph_mlp_create.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
+        h.sph_mlp_destroy.argtypes = [ctypes.c_void_p]
+        h.sph_mlp_destroy.restype = None
+        h.sph_mlp_last_error.argtypes = [ctypes.c_void_p]
+        h.sph_mlp_last_error.restype = ctypes.c_char_p
+        h.sph_mlp_get.argtypes = [ctypes.c_void_p, F, F]
+        h.sph_mlp_set.argtypes = [ctypes.c_void_p, F, F]
+        h.sph_mlp_forward.argtypes = [ctypes.c_void_p, F, ctypes.c_int64, F]
+        h.sph_mlp_loss_grad.argtypes = [ctypes.c_void_p, F, F, F, ctypes.c_int64, F, F]
+        h.sph_mlp_fit_norm.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        h.sph_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_uint64, F]
+        h.sph_samples_create.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]
+        h.sph_samples_destroy.argtypes = [ctypes.c_void_p]
+        h.sph_samples_destroy.restype = None
+        h.sph_samples_count.argtypes = [ctypes.c_void_p]
+        h.sph_samples_count.restype = ctypes.c_int64
+        h.sph_samples_read.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, F]
+        h.sph_samples_append.argtypes = [ctypes.c_void_p, F, ctypes.c_int64]
+        h.sph_samples_clear.argtypes = [ctypes.c_void_p]
+        h.sph_set_sample_dump.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        h.sph_set_learned_guess.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        h.sph_features.argtypes = [ctypes.c_void_p, F]
+        h.sph_conservation.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        h.sph_time.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         o.sph_tillotson_granite.argtypes = [ctypes.POINTER(SphTillotson)]
         o.sph_tillotson_iron.argtypes = [ctypes.POINTER(SphTillotson)]
         o.sph_make_eos_table.argtypes = [ctypes.POINTER(SphParams), ctypes.POINTER(SphTillotson), ctypes.c_int,
@@ -373,6 +403,143 @@ class Context:
 
     def reset_pass_times(self):
         self._check(self._h.sph_reset_pass_times(self.ptr), "sph_reset_pass_times")
+
+    # learned smoothing-length guess
+    def set_sample_dump(self, samples: Optional["Samples"]):
+        self._check(self._h.sph_set_sample_dump(self.ptr, samples.ptr if samples is not None else None), "sph_set_sample_dump")
+
+    def set_learned_guess(self, mlp: Optional["Mlp"]):
+        self._check(self._h.sph_set_learned_guess(self.ptr, mlp.ptr if mlp is not None else None), "sph_set_learned_guess")
+
+    def conservation(self) -> Dict[str, object]:
+        """FP64 device sums: energies, momentum, angular momentum, mass."""
+        out = (ctypes.c_double * 10)()
+        self._check(self._h.sph_conservation(self.ptr, out), "sph_conservation")
+        v = list(out)
+        return {"kinetic": v[0], "internal": v[1], "potential": v[2], "momentum": np.array(v[3:6]),
+                "angular_momentum": np.array(v[6:9]), "mass": v[9]}
+
+    def time(self) -> float:
+        t = ctypes.c_double()
+        self._check(self._h.sph_time(self.ptr, ctypes.byref(t)), "sph_time")
+        return float(t.value)
+
+    def features(self) -> np.ndarray:
+        out = np.zeros((self.n, MLP_INPUTS), np.float32)
+        self._check(self._h.sph_features(self.ptr, _fp(out)), "sph_features")
+        return out
+
+
+MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS, MLP_NORM, SAMPLE_FLOATS = 12, 64, 5057, 26, 16
+
+
+class Samples:
+    """Device buffer of (features, log(h_converged / h_start), weight) records."""
+
+    def __init__(self, capacity: int):
+        h, _ = libs()
+        self._h = h
+        ptr = ctypes.c_void_p()
+        rc = h.sph_samples_create(int(capacity), ctypes.byref(ptr))
+        if rc != SPH_OK:
+            raise SphError(f"sph_samples_create failed: {rc}")
+        self.ptr = ptr
+
+    def __len__(self):
+        return int(self._h.sph_samples_count(self.ptr))
+
+    def read(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        count = len(self) - first if count is None else count
+        out = np.zeros((count, SAMPLE_FLOATS), np.float32)
+        rc = self._h.sph_samples_read(self.ptr, first, count, _fp(out))
+        if rc != SPH_OK:
+            raise SphError(f"sph_samples_read failed: {rc}")
+        return out
+
+    def append(self, rec: np.ndarray):
+        rec = np.ascontiguousarray(rec, np.float32).reshape(-1, SAMPLE_FLOATS)
+        rc = self._h.sph_samples_append(self.ptr, _fp(rec), rec.shape[0])
+        if rc != SPH_OK:
+            raise SphError(f"sph_samples_append failed: {rc}")
+
+    def clear(self):
+        self._h.sph_samples_clear(self.ptr)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self._h.sph_samples_destroy(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mlp:
+    """The learned smoothing-length network (BF16 MFMA forward/backward, Adam)."""
+
+    def __init__(self, seed: int = 13):
+        h, _ = libs()
+        self._h = h
+        ptr = ctypes.c_void_p()
+        rc = h.sph_mlp_create(int(seed), ctypes.byref(ptr))
+        if rc != SPH_OK:
+            raise SphError(f"sph_mlp_create failed: {rc}")
+        self.ptr = ptr
+
+    def _check(self, rc, what):
+        if rc != SPH_OK:
+            raise SphError(f"{what} failed ({rc}): {self._h.sph_mlp_last_error(self.ptr).decode()}")
+
+    def get(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        nm = np.zeros(MLP_NORM, np.float32)
+        self._check(self._h.sph_mlp_get(self.ptr, _fp(p), _fp(nm)), "sph_mlp_get")
+        return p, nm
+
+    def set(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        nm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        self._check(self._h.sph_mlp_set(self.ptr, None if p is None else _fp(p), None if nm is None else _fp(nm)),
+                    "sph_mlp_set")
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        out = np.zeros(x.shape[0], np.float32)
+        self._check(self._h.sph_mlp_forward(self.ptr, _fp(x), x.shape[0], _fp(out)), "sph_mlp_forward")
+        return out
+
+    def loss_grad(self, x, d, w=None):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        d = np.ascontiguousarray(d, np.float32)
+        wa = None if w is None else np.ascontiguousarray(w, np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        self._check(self._h.sph_mlp_loss_grad(self.ptr, _fp(x), _fp(d), None if wa is None else _fp(wa), x.shape[0],
+                                              _fp(loss), _fp(g)), "sph_mlp_loss_grad")
+        return float(loss[0]), g
+
+    def fit_norm(self, samples: Samples):
+        self._check(self._h.sph_mlp_fit_norm(self.ptr, samples.ptr), "sph_mlp_fit_norm")
+
+    def train(self, samples: Samples, steps: int, batch: int = 65536, lr: float = 1e-3, seed: int = 14):
+        losses = np.zeros(max(steps, 1), np.float32)
+        self._check(self._h.sph_mlp_train(self.ptr, samples.ptr, steps, batch, lr, seed, _fp(losses)),
+                    "sph_mlp_train")
+        return losses[:steps]
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self._h.sph_mlp_destroy(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Headless:

@@ -37,6 +37,7 @@ bool valid_params(const sph_params* p, std::string& why) {
     if (p->max_rungs < 0 || p->max_rungs > 24) { why = "max_rungs must be 0..24"; return false; }
     if (p->n_neighbours_estim < 0) { why = "n_neighbours_estim must be >= 0"; return false; }
     if (p->max_nr_iterations < 0) { why = "max_nr_iterations must be >= 0"; return false; }
+    if (p->precision && !(p->fixes & SPH_FIX_NEAR_FIELD)) { why = "the precision form needs SPH_FIX_NEAR_FIELD"; return false; }
     return true;
 }
 
@@ -70,7 +71,7 @@ void destroy_ctx(sph_ctx* ctx) {
                      &ctx->s_posh, &ctx->s_velb, &ctx->s_hyd, &ctx->snap};
     for (auto p : f4) dfree(*p);
     float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c,
-                    &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm};
+                    &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->phi};
     for (auto p : f1) dfree(*p);
     int32_t** i1[] = {&ctx->mat, &ctx->next_active, &ctx->nr_iters, &ctx->time, &ctx->list_a, &ctx->list_b};
     for (auto p : i1) dfree(*p);
@@ -81,6 +82,8 @@ void destroy_ctx(sph_ctx* ctx) {
     dfree(ctx->cell_se);
     dfree(ctx->tile_hist);
     dfree(ctx->grav_flags);
+    dfree(ctx->feats);
+    dfree(ctx->guess);
     dfree(ctx->eos[0]);
     dfree(ctx->eos[1]);
     free_tree(ctx->tree);
@@ -105,7 +108,7 @@ int64_t field_bytes(const sph_ctx* ctx, int field) {
         case SPH_F_RHO: case SPH_F_U: case SPH_F_MASS: case SPH_F_PRESSURE: case SPH_F_H:
         case SPH_F_MAT: case SPH_F_GRADTERM: case SPH_F_SOUNDSPEED: case SPH_F_DU:
         case SPH_F_BALSARA: case SPH_F_DHDT: case SPH_F_NEXT_ACTIVE: case SPH_F_GRAVNORM:
-        case SPH_F_NR_ITERS: return 4 * n;
+        case SPH_F_NR_ITERS: case SPH_F_POTENTIAL: return 4 * n;
         case SPH_F_ACTIVE: return n;
         case SPH_F_CELL_ARRAY: return 8 * n;
         case SPH_F_CELL_START: case SPH_F_CELL_END: return 4 * (int64_t)ctx->ncells;
@@ -138,6 +141,7 @@ void* field_ptr(sph_ctx* ctx, int field) {
         case SPH_F_GRAVNORM: return ctx->gravnorm;
         case SPH_F_NR_ITERS: return ctx->nr_iters;
         case SPH_F_DENSITY_WORK: return ctx->density_work;
+        case SPH_F_POTENTIAL: return ctx->phi;
         case SPH_F_CELL_ARRAY: return ctx->pairs_a;
         case SPH_F_GLOBAL_TIME: return ctx->time;
         case SPH_F_DT: return ctx->time + 1;
@@ -245,8 +249,9 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->s_posh, n));
     TRY(dalloc(&ctx->s_velb, n));
     TRY(dalloc(&ctx->s_hyd, n));
-    float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c, &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm};
+    float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c, &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->phi};
     for (auto p : f1) TRY(dalloc(p, n));
+    TRY(hipMemset(ctx->phi, 0, 4 * n));
     TRY(dalloc(&ctx->mat, n));
     TRY(dalloc(&ctx->next_active, n));
     TRY(dalloc(&ctx->nr_iters, n));
@@ -477,7 +482,9 @@ int sph_density(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
     if (!ctx->have_prev) return ctx_fail(ctx, SPH_E_STATE, "density before any sort");
     int rc = ctx_begin_pass(ctx, SPH_PASS_DENSITY);
+    if (rc == SPH_OK) rc = launch_density_prologue(ctx);
     if (rc == SPH_OK) rc = launch_density(ctx);
+    if (rc == SPH_OK) rc = launch_density_epilogue(ctx);
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_DENSITY);
     return rc;
 }
@@ -575,7 +582,7 @@ int sph_upload(sph_ctx* ctx, int field, const void* src) {
     if (bytes < 0) return ctx_fail(ctx, SPH_E_ARG, "unknown field");
     if (field == SPH_F_CELL_ARRAY || field == SPH_F_CELL_START || field == SPH_F_CELL_END ||
         field == SPH_F_MULTIPOLES || field == SPH_F_LOCALS || field == SPH_F_NR_ITERS ||
-        field == SPH_F_DENSITY_WORK)
+        field == SPH_F_DENSITY_WORK || field == SPH_F_POTENTIAL)
         return ctx_fail(ctx, SPH_E_ARG, "derived field cannot be uploaded");
     void* dst = field_ptr(ctx, field);
     if (!dst) return ctx_fail(ctx, SPH_E_ARG, "field not uploadable");

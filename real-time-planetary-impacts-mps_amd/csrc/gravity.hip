@@ -145,6 +145,7 @@ struct UpArgs {
     int fix_zz;
     int fix_range;
     int need_preorder; /* the near-field down pass walks subtrees as id ranges */
+    int prec;          /* precision form: centres of mass summed relative to the first member */
 };
 
 __device__ __forceinline__ float power2(float XX, float XY, float XZ, float YY, float YZ, float ZZ) {
@@ -166,15 +167,18 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
     float minGrav = FLT_MAX;
     bool act = false;
     float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+    /* precision form: sum m (x - x_0) instead of m x, x_0 = first particle */
+    const float4 x0 = a.prec ? a.pos[pl[0]] : make_float4(0.f, 0.f, 0.f, 0.f);
     for (int k = 0; k < np; ++k) {
         const int p = pl[k];
         const float pm = a.m[p];
-        const float4 x = a.pos[p];
+        const float4 xa = a.pos[p];
+        const float4 x = make_float4(xa.x - x0.x, xa.y - x0.y, xa.z - x0.z, 0.f);
         const int na = a.next_active[p];
         const bool isActive = a.readme_rule ? (na <= gt) : (gt <= na);
         act = act || isActive;
-        mxx = fmaxf(mxx, x.x); mxy = fmaxf(mxy, x.y); mxz = fmaxf(mxz, x.z);
-        mnx = fminf(mnx, x.x); mny = fminf(mny, x.y); mnz = fminf(mnz, x.z);
+        mxx = fmaxf(mxx, xa.x); mxy = fmaxf(mxy, xa.y); mxz = fmaxf(mxz, xa.z);
+        mnx = fminf(mnx, xa.x); mny = fminf(mny, xa.y); mnz = fminf(mnz, xa.z);
         M += pm;
         px += x.x * pm;
         py += x.y * pm;
@@ -182,9 +186,9 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
         if (isActive) minGrav = fminf(minGrav, a.gravnorm[p]);
     }
     for (int k = 0; k < np; ++k) a.active[pl[k]] = act ? 1 : 0;
-    px /= M;
-    py /= M;
-    pz /= M;
+    px = px / M + x0.x;
+    py = py / M + x0.y;
+    pz = pz / M + x0.z;
     float X = 0.f, Y = 0.f, Z = 0.f, XX = 0.f, XY = 0.f, XZ = 0.f, YY = 0.f, YZ = 0.f, ZZ = 0.f;
     for (int k = 0; k < np; ++k) {
         const int p = pl[k];
@@ -219,10 +223,22 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
     float minGrav = FLT_MAX, mass = 0.f, px = 0.f, py = 0.f, pz = 0.f;
     bool act = false, allLeaves = true;
     float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+    /* precision form: centre of mass relative to the first child's */
+    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (a.prec)
+        for (int c = 0; c < 8; ++c)
+            if (ch[c] >= 0) {
+                r0 = a.t.far[3 * ch[c]];
+                break;
+            }
     for (int c = 0; c < 8; ++c) {
         const int cid = ch[c];
         if (cid < 0) continue;
-        const float4 f0 = a.t.far[3 * cid], f1 = a.t.far[3 * cid + 1];
+        float4 f0 = a.t.far[3 * cid];
+        const float4 f1 = a.t.far[3 * cid + 1];
+        f0.x -= r0.x;
+        f0.y -= r0.y;
+        f0.z -= r0.z;
         const float4 x0 = a.t.aux[3 * cid], x1 = a.t.aux[3 * cid + 1];
         minGrav = fminf(x0.w, minGrav);
         mass += f1.x;
@@ -235,9 +251,9 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
         mnx = fminf(mnx, x0.x); mny = fminf(mny, x0.y); mnz = fminf(mnz, x0.z);
     }
     const float size = fmaxf(fmaxf(mxx - mnx, mxy - mny), mxz - mnz);
-    px /= mass;
-    py /= mass;
-    pz /= mass;
+    px = px / mass + r0.x;
+    py = py / mass + r0.y;
+    pz = pz / mass + r0.z;
     float M = 0.f, XX = 0.f, XY = 0.f, XZ = 0.f, YY = 0.f, YZ = 0.f, ZZ = 0.f;
     for (int c = 0; c < 8; ++c) {
         const int cid = ch[c];
@@ -313,6 +329,7 @@ struct DownArgs {
     float G;
     int fix_soft;
     int near;         /* near-field form: P2P with every rejected source leaf, one softened kernel */
+    float* phi;       /* precision form: potential per particle (G included), else null */
     const float4* pos;
     const float* m;
     const uint8_t* active;
@@ -344,8 +361,10 @@ __device__ __forceinline__ bool mac_accept(float ax, float ay, float az, float r
     return E < eta * minGrav * r2 * r2;
 }
 
-/* M2L.metal:12-108 + derivatives.metal:59-156 at P = 2, accumulated. */
-__device__ __forceinline__ void m2l_add(LocalExp& L, float4 b0, float4 b1, float4 b2, float eps) {
+/* M2L.metal:12-108 + derivatives.metal:59-156 at P = 2: the ten local
+ * coefficients one source node adds at the expansion centre. */
+__device__ __forceinline__ void m2l_terms(const LocalExp& L, float4 b0, float4 b1, float4 b2, float eps,
+                                          float T[eNT]) {
     const float vx = L.px - b0.x, vy = L.py - b0.y, vz = L.pz - b0.z;
     const float r2 = vx * vx + vy * vy + vz * vz;
     const float r = sqrtf(r2);
@@ -378,16 +397,51 @@ __device__ __forceinline__ void m2l_add(LocalExp& L, float4 b0, float4 b1, float
     float lm = M0 * Dt_1;
     lm += b1.y * DXX + b2.x * DYY + b2.z * DZZ;
     lm += b1.z * DXY + b1.w * DXZ + b2.y * DYZ;
-    L.e[eM] += lm;
-    L.e[eX] += M0 * DX;
-    L.e[eY] += M0 * DY;
-    L.e[eZ] += M0 * DZ;
-    L.e[eXX] += M0 * DXX;
-    L.e[eXY] += M0 * DXY;
-    L.e[eXZ] += M0 * DXZ;
-    L.e[eYY] += M0 * DYY;
-    L.e[eYZ] += M0 * DYZ;
-    L.e[eZZ] += M0 * DZZ;
+    T[eM] = lm;
+    T[eX] = M0 * DX;
+    T[eY] = M0 * DY;
+    T[eZ] = M0 * DZ;
+    T[eXX] = M0 * DXX;
+    T[eXY] = M0 * DXY;
+    T[eXZ] = M0 * DXZ;
+    T[eYY] = M0 * DYY;
+    T[eYZ] = M0 * DYZ;
+    T[eZZ] = M0 * DZZ;
+}
+
+__device__ __forceinline__ void m2l_add(LocalExp& L, float4 b0, float4 b1, float4 b2, float eps) {
+    float T[eNT];
+    m2l_terms(L, b0, b1, b2, eps, T);
+#pragma unroll
+    for (int k = 0; k < eNT; ++k) L.e[k] += T[k];
+}
+
+/* Neumaier's compensated sum (Neumaier 1974): s + c carries the sum of the
+ * added terms to about twice the working precision. */
+__device__ __forceinline__ void kadd(float& s, float& c, float x) {
+    const float t = s + x;
+    if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ void m2l_add_comp(LocalExp& L, float Lc[eNT], float4 b0, float4 b1, float4 b2,
+                                             float eps) {
+    float T[eNT];
+    m2l_terms(L, b0, b1, b2, eps, T);
+#pragma unroll
+    for (int k = 0; k < eNT; ++k) kadd(L.e[k], Lc[k], T[k]);
+}
+
+/* Softened 1/r whose derivatives M2L uses (derivatives.metal:12-57,
+ * support eps): the pair potential of the near-field form per unit mass. */
+__device__ __forceinline__ float pot_m2l(float r, float eps) {
+    if (r >= eps) return 1.f / r;
+    const float eps_inv = 1.f / eps;
+    const float u = r * eps_inv;
+    float s1 = -3.f * u + 15.f;
+    s1 = s1 * u - 28.f; s1 = s1 * u + 21.f; s1 = s1 * u; s1 = s1 * u - 7.f; s1 = s1 * u; s1 = s1 * u + 3.f;
+    return eps_inv * s1;
 }
 
 struct NodeA {
@@ -486,9 +540,13 @@ __device__ __forceinline__ void l2l_child(const DownArgs& a, const LocalExp& L, 
 }
 
 /* L2P + intra-leaf P2P for particle k of a leaf (helpers.h:47-81); (nx, ny,
- * nz) is the near-field form's sum over rejected source leaves (else 0). */
+ * nz) is the near-field form's sum over rejected source leaves (else 0).
+ * PREC (precision form, near field only): compensated sums, and the
+ * potential -G (local expansion at x + sum of the pair potentials) with
+ * `nphi` the rejected leaves' part. */
+template <bool PREC>
 __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp& L, const int32_t* pl, int np, int k,
-                                              float nx = 0.f, float ny = 0.f, float nz = 0.f) {
+                                              float nx = 0.f, float ny = 0.f, float nz = 0.f, float nphi = 0.f) {
     const int p = pl[k];
     if (!a.active[p]) return;
     const float4 xi = a.pos[p];
@@ -498,6 +556,12 @@ __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp&
     ax += rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ];
     ay += rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ];
     az += rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ];
+    float cx = 0.f, cy = 0.f, cz = 0.f, ps = 0.f, pc = 0.f;
+    if (PREC) {
+        ps = L.e[eM] + rx * L.e[eX] + ry * L.e[eY] + rz * L.e[eZ];
+        ps += 0.5f * (rx * rx * L.e[eXX] + ry * ry * L.e[eYY] + rz * rz * L.e[eZZ]);
+        ps += rx * ry * L.e[eXY] + rx * rz * L.e[eXZ] + ry * rz * L.e[eYZ];
+    }
     for (int j = 0; j < np; ++j) {
         if (j == k) continue;
         const int pj = pl[j];
@@ -505,26 +569,47 @@ __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp&
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float r = sqrtf(dx * dx + dy * dy + dz * dz);
         if (r == 0.f) continue;
-        const float w = a.m[pj] * (a.near ? dphi_dr_m2l(r, a.eps) : dphi_dr(r, a.eps, a.fix_soft));
-        ax -= w * (dx / r);
-        ay -= w * (dy / r);
-        az -= w * (dz / r);
+        const float mj = a.m[pj];
+        const float w = mj * (a.near ? dphi_dr_m2l(r, a.eps) : dphi_dr(r, a.eps, a.fix_soft));
+        if (PREC) {
+            kadd(ax, cx, -(w * (dx / r)));
+            kadd(ay, cy, -(w * (dy / r)));
+            kadd(az, cz, -(w * (dz / r)));
+            kadd(ps, pc, mj * pot_m2l(r, a.eps));
+        } else {
+            ax -= w * (dx / r);
+            ay -= w * (dy / r);
+            az -= w * (dz / r);
+        }
     }
-    ax += nx;
-    ay += ny;
-    az += nz;
+    if (PREC) {
+        kadd(ax, cx, nx);
+        kadd(ay, cy, ny);
+        kadd(az, cz, nz);
+        kadd(ps, pc, nphi);
+        ax += cx;
+        ay += cy;
+        az += cz;
+        a.phi[p] = -a.G * (ps + pc);
+    } else {
+        ax += nx;
+        ay += ny;
+        az += nz;
+    }
     a.acc[p] = make_float4(a.G * ax, a.G * ay, a.G * az, 0.f);
     a.gravnorm[p] = sqrtf(ax * ax + ay * ay + az * az);
 }
 
+template <bool PREC>
 __device__ __forceinline__ void leaf_field_lanes(const DownArgs& a, const LocalExp& L, int4 inf, int lane, int width) {
     const int32_t* pl = a.t.octree + inf.y + 2;
-    for (int k = lane; k < inf.x; k += width) leaf_particle(a, L, pl, inf.x, k);
+    for (int k = lane; k < inf.x; k += width) leaf_particle<PREC>(a, L, pl, inf.x, k);
 }
 
 /* localToLeaf (helpers.h:47-81): L2P plus the intra-leaf P2P */
 __device__ void leaf_field(const DownArgs& a, const LocalExp& L, int4 inf) {
-    leaf_field_lanes(a, L, inf, 0, 1);
+    if (a.phi) leaf_field_lanes<true>(a, L, inf, 0, 1);
+    else leaf_field_lanes<false>(a, L, inf, 0, 1);
 }
 
 /* downPass (gravity.metal:64-143) for one non-root node */
@@ -589,14 +674,19 @@ __device__ __forceinline__ float group_sum(float v) {
  * targets are the leaf's particles, NEAR_K at a time in registers; the M2L
  * part runs in the first pass only.  Partial sums are reduced over the group,
  * the incoming local expansion is added, and lane k finishes particle k. */
-template <int GS>
+template <int GS, bool PREC>
 __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA& A, LocalExp& L, float4 l0,
                                const int32_t* in, int lane, int wl, uint64_t gmask) {
     constexpr int NEAR_K = 4;
     const int np = inf.x;
     const int32_t* pl = a.t.octree + inf.y + 2;
+    float Lc[eNT];
+#pragma unroll
+    for (int k = 0; k < eNT; ++k) Lc[k] = 0.f;
     for (int k0 = 0; k0 < np; k0 += NEAR_K) {
         float tx[NEAR_K], ty[NEAR_K], tz[NEAR_K], gx[NEAR_K], gy[NEAR_K], gz[NEAR_K];
+        /* precision form: compensations and the pair potential */
+        float cx[NEAR_K], cy[NEAR_K], cz[NEAR_K], gp[NEAR_K], cp[NEAR_K];
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk) {
             const float4 x = k0 + kk < np ? a.pos[pl[k0 + kk]] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -604,6 +694,7 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
             ty[kk] = x.y;
             tz[kk] = x.z;
             gx[kk] = gy[kk] = gz[kk] = 0.f;
+            cx[kk] = cy[kk] = cz[kk] = gp[kk] = cp[kk] = 0.f;
         }
         bool more = true;
         for (int base = 0; more; base += GS) {
@@ -623,7 +714,10 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
                 }
                 const float4 b0 = a.t.far[3 * cur], b1 = a.t.far[3 * cur + 1], b2 = a.t.far[3 * cur + 2];
                 if (mac_accept(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.eta)) {
-                    if (k0 == 0) m2l_add(L, b0, b1, b2, a.eps);
+                    if (k0 == 0) {
+                        if constexpr (PREC) m2l_add_comp(L, Lc, b0, b1, b2, a.eps);
+                        else m2l_add(L, b0, b1, b2, a.eps);
+                    }
                     cur = a.t.sub_end[cur];
                     continue;
                 }
@@ -641,9 +735,16 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
                         const float r = sqrtf(dx * dx + dy * dy + dz * dz);
                         if (k0 + kk >= np || r == 0.f) continue;
                         const float w = mj * dphi_dr_m2l(r, a.eps);
-                        gx[kk] -= w * (dx / r);
-                        gy[kk] -= w * (dy / r);
-                        gz[kk] -= w * (dz / r);
+                        if constexpr (PREC) {
+                            kadd(gx[kk], cx[kk], -(w * (dx / r)));
+                            kadd(gy[kk], cy[kk], -(w * (dy / r)));
+                            kadd(gz[kk], cz[kk], -(w * (dz / r)));
+                            kadd(gp[kk], cp[kk], mj * pot_m2l(r, a.eps));
+                        } else {
+                            gx[kk] -= w * (dx / r);
+                            gy[kk] -= w * (dy / r);
+                            gz[kk] -= w * (dz / r);
+                        }
                     }
                 }
             }
@@ -653,10 +754,19 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
             gx[kk] = group_sum<GS>(gx[kk]);
             gy[kk] = group_sum<GS>(gy[kk]);
             gz[kk] = group_sum<GS>(gz[kk]);
+            if constexpr (PREC) {
+                gx[kk] += group_sum<GS>(cx[kk]);
+                gy[kk] += group_sum<GS>(cy[kk]);
+                gz[kk] += group_sum<GS>(cz[kk]);
+                gp[kk] = group_sum<GS>(gp[kk]) + group_sum<GS>(cp[kk]);
+            }
         }
         if (k0 == 0) {
 #pragma unroll
-            for (int k = 0; k < eNT; ++k) L.e[k] = group_sum<GS>(L.e[k]);
+            for (int k = 0; k < eNT; ++k) {
+                L.e[k] = group_sum<GS>(L.e[k]);
+                if constexpr (PREC) L.e[k] += group_sum<GS>(Lc[k]);
+            }
             const float4 l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2], l3 = a.t.loc[4 * id + 3];
             L.e[eM] = l0.w + L.e[eM];
             L.e[eX] = l1.x + L.e[eX];
@@ -671,7 +781,7 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
         }
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk)
-            if (lane == kk && k0 + kk < np) leaf_particle(a, L, pl, np, k0 + kk, gx[kk], gy[kk], gz[kk]);
+            if (lane == kk && k0 + kk < np) leaf_particle<PREC>(a, L, pl, np, k0 + kk, gx[kk], gy[kk], gz[kk], gp[kk]);
     }
 }
 
@@ -689,7 +799,7 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
  * partial local expansion; the group reduces them with shuffles (a fixed
  * order, so results are reproducible), adds the incoming local expansion and
  * runs L2L (lane c -> child c) or L2P + P2P (lane k -> particle k). */
-template <int GS, bool NEAR>
+template <int GS, bool NEAR, bool PREC>
 __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     constexpr int GPB = 256 / GS;
     constexpr int EPR = GS / 8;
@@ -717,12 +827,13 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     L.px = l0.x;
     L.py = l0.y;
     L.pz = l0.z;
+    float Lc[eNT]; /* compensations (precision form) */
 #pragma unroll
-    for (int k = 0; k < eNT; ++k) L.e[k] = 0.f;
+    for (int k = 0; k < eNT; ++k) L.e[k] = Lc[k] = 0.f;
     const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
     if constexpr (NEAR) {
         if (live && np != 0) {
-            down_near_leaf<GS>(a, id, inf, A, L, l0, in, lane, wl, gmask);
+            down_near_leaf<GS, PREC>(a, id, inf, A, L, l0, in, lane, wl, gmask);
             return;
         }
     }
@@ -741,7 +852,15 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
         const bool valid = i < a.maxu && (term == 0ull || (1ull << wl) < first);
         more = term == 0ull && base + GS < a.maxu;
         bool acc = false;
-        if (valid) acc = check_add(a, A, L, e, false);
+        if (valid) {
+            if constexpr (PREC) {
+                const float4 b0 = a.t.far[3 * e], b1 = a.t.far[3 * e + 1], b2 = a.t.far[3 * e + 2];
+                acc = mac_accept(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.eta);
+                if (acc) m2l_add_comp(L, Lc, b0, b1, b2, a.eps);
+            } else {
+                acc = check_add(a, A, L, e, false);
+            }
+        }
         uint64_t rej = __ballot(valid && !acc) & gmask;
         while (rej != 0ull) {
             int my_src = -1;
@@ -778,12 +897,18 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
                 else a.out[outbase + (uint32_t)pos] = child;
             }
             running += __popcll(pm);
-            if (take) m2l_add(L, c0, c1, c2, a.eps);
+            if (take) {
+                if constexpr (PREC) m2l_add_comp(L, Lc, c0, c1, c2, a.eps);
+                else m2l_add(L, c0, c1, c2, a.eps);
+            }
         }
     }
     if (live && np == 0 && lane == 0 && running < a.maxu) a.out[outbase + (uint32_t)running] = -1;
 #pragma unroll
-    for (int k = 0; k < eNT; ++k) L.e[k] = group_sum<GS>(L.e[k]);
+    for (int k = 0; k < eNT; ++k) {
+        L.e[k] = group_sum<GS>(L.e[k]);
+        if constexpr (PREC) L.e[k] += group_sum<GS>(Lc[k]);
+    }
     if (!live) return;
     {
         const float4 l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2], l3 = a.t.loc[4 * id + 3];
@@ -801,7 +926,7 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     if (np == 0) {
         if (lane < 8) l2l_child(a, L, a.t.child[8 * id + lane]);
     } else {
-        leaf_field_lanes(a, L, inf, lane, GS);
+        leaf_field_lanes<PREC>(a, L, inf, lane, GS);
     }
 }
 
@@ -902,6 +1027,7 @@ int launch_gravity_up(sph_ctx* ctx) {
     a.fix_zz = (ctx->prm.fixes & SPH_FIX_M2M_ZZ) ? 1 : 0;
     a.fix_range = (ctx->prm.fixes & SPH_FIX_SYNC_RANGE) ? 1 : 0;
     a.need_preorder = (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ? 1 : 0;
+    a.prec = ctx->prm.precision ? 1 : 0;
     const int threads = 256;
     for (int l = (int)T.level_counts.size() - 1; l >= 0; --l) {
         a.count = T.level_counts[l];
@@ -926,6 +1052,9 @@ int launch_gravity_down(sph_ctx* ctx) {
     a.G = p.gravity_g;
     a.fix_soft = (p.fixes & SPH_FIX_SOFTENING) ? 1 : 0;
     a.near = (p.fixes & SPH_FIX_NEAR_FIELD) ? 1 : 0;
+    /* the precision form (near field required, checked at creation) also
+     * writes the potential */
+    a.phi = (p.precision && a.near) ? ctx->phi : nullptr;
     a.pos = ctx->pos;
     a.m = ctx->m;
     a.active = ctx->active;
@@ -950,8 +1079,9 @@ int launch_gravity_down(sph_ctx* ctx) {
         if (a.max_rec == 1 || a.near) {
             constexpr int GS = 32;
             const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
-            if (a.near) k_down_group<GS, true><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
-            else k_down_group<GS, false><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
+            if (a.phi) k_down_group<GS, true, true><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
+            else if (a.near) k_down_group<GS, true, false><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
+            else k_down_group<GS, false, false><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
         } else {
             k_down<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
         }

@@ -130,7 +130,10 @@ struct DensityArgs {
     int max_it, ne;
     int fix_cache;
     int dt_reset;
+    const float* guess;   /* learned starting h per sorted slot, or null */
 };
+
+constexpr float kGuessTrust = 0.05f;
 
 /* density.metal:12-239 */
 __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
@@ -158,7 +161,15 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     const float cs = a.cells.cell_size;
     const int mask = a.cells.mask;
 
-    float h_i = a.h[ind];
+    /* With a learned starting guess the iteration starts there and restarts
+     * from the step's h once the residual grows, or when the first residual
+     * exceeds kGuessTrust (a guess that far off is not worth iterating
+     * from).  No counterpart in the reference, whose iteration always
+     * starts from h. */
+    const float h_start = a.h[ind];
+    float h_i = a.guess ? a.guess[index] : h_start;
+    bool guided = a.guess != nullptr;
+    float eps_last = 3.0e38f;
     int max_h = 0;
     float h2x4 = 4.f * h_i * h_i;
     float h1 = 1.f / h_i;
@@ -212,6 +223,19 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
         float newH = fminf(a.res_eta * powf(mass_i / density, 1.f / 3.f), a.max_h_len);
         newH = 0.5f * (h_i + newH);
         eps = fabsf(newH - h_i) / h_i;
+        if (guided) {
+            if (eps > eps_last || eps > kGuessTrust) {
+                guided = false;
+                h_i = h_start;
+                h1 = 1.f / h_i;
+                h2x4 = 4.f * h_i * h_i;
+                max_h = 0;
+                eps = 1.f;
+                count++;
+                continue;
+            }
+            eps_last = eps;
+        }
         if (eps > a.eta) {
             max_h = (int)fmaxf((float)max_h, h_i);
             h_i = newH;
@@ -510,6 +534,7 @@ int launch_density(sph_ctx* ctx) {
     a.fix_cache = (p.fixes & SPH_FIX_DENSITY_CACHE) ? 1 : 0;
     const float dtmin1 = p.dt0 / (float)(1 << p.r_max);
     a.dt_reset = (int)(p.dt0 * (1.f / dtmin1));
+    a.guess = ctx->learned ? ctx->guess : nullptr;
     const int threads = 256;
     k_density<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a);
     SPH_HIP_TRY(hipGetLastError());

@@ -83,6 +83,7 @@ struct sph_ctx {
     float4 *pos = nullptr, *vel = nullptr, *acc = nullptr;
     float *rho = nullptr, *u = nullptr, *m = nullptr, *P = nullptr, *h = nullptr;
     float *f = nullptr, *c = nullptr, *du = nullptr, *B = nullptr, *dhdt = nullptr, *gravnorm = nullptr;
+    float* phi = nullptr; /* potential (precision form) */
     int32_t *mat = nullptr, *next_active = nullptr, *nr_iters = nullptr;
     int4* density_work = nullptr; /* NR (candidates, inside 2h), final pass (candidates, inside) */
     uint8_t* active = nullptr;
@@ -129,6 +130,12 @@ struct sph_ctx {
     hipEvent_t ev_open[SPH_PASS_COUNT] = {};
     double pass_ms[SPH_PASS_COUNT] = {};
     int64_t pass_calls[SPH_PASS_COUNT] = {};
+
+    /* learned smoothing-length guess (learned.hip) */
+    float4* feats = nullptr;        /* 3 float4 (12 features) per sorted slot */
+    float* guess = nullptr;         /* starting h per sorted slot */
+    sph_mlp* learned = nullptr;     /* not owned */
+    sph_samples* dump = nullptr;    /* not owned */
 };
 
 int ctx_fail(sph_ctx* ctx, int code, const std::string& msg);
@@ -143,6 +150,9 @@ int launch_sort(sph_ctx* ctx);
 int launch_density(sph_ctx* ctx);
 int launch_acceleration(sph_ctx* ctx);
 int launch_step(sph_ctx* ctx);
+/* learned.hip */
+int launch_density_prologue(sph_ctx* ctx);
+int launch_density_epilogue(sph_ctx* ctx);
 /* gravity.hip */
 int launch_tree_convert(sph_ctx* ctx);
 int launch_gravity_up(sph_ctx* ctx);
