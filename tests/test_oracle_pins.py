@@ -581,6 +581,33 @@ def test_direct_sum_two_body_and_shell(orc):
     assert acc[0, 0] == pytest.approx(-4000 / 9.0, rel=1e-2)
 
 
+@pytest.mark.parametrize("soft", [-0.04, 0.04])
+def test_direct_sum_softened_force_is_minus_potential_gradient(orc, soft):
+    """Both softened kernels: the pair force equals the radial derivative of
+    the pair potential (central differences), and both are continuous, with
+    the Newtonian values, where the softening ends."""
+    eps = abs(soft)
+    edge = eps if soft < 0 else 2 * eps
+
+    def pair(r):
+        pos = np.zeros((2, 4), np.float32)
+        pos[1, 0] = r
+        acc, phi = orc.direct_sum(pos, np.array([1.0, 1.0], np.float32), np.array([0]), 1.0, soft)
+        return acc[0, 0], phi[0]
+
+    for r in np.linspace(0.1, 1.5, 8) * edge:
+        dr = 1e-3 * edge
+        f, _ = pair(r)
+        _, pp = pair(r + dr)
+        _, pm = pair(r - dr)
+        # particle 0 is pulled towards +x, down the pair potential: a_x = d(phi)/dr
+        assert f == pytest.approx((pp - pm) / (2 * dr), rel=5e-3, abs=1e-6)
+    fi, pi = pair(edge * (1 - 1e-6))
+    fo, po = pair(edge * (1 + 1e-6))
+    assert fi == pytest.approx(fo, rel=1e-4) and pi == pytest.approx(po, rel=1e-4)
+    assert fo == pytest.approx(1 / edge ** 2, rel=1e-4) and po == pytest.approx(-1 / edge, rel=1e-4)
+
+
 @pytest.fixture(scope="module")
 def fmm_state(sph, orc, tables256):
     prm = sph.default_params(fixes=sph.SPH_FIX_M2M_ZZ | sph.SPH_FIX_SOFTENING | sph.SPH_FIX_NEAR_FIELD)
