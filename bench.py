@@ -61,6 +61,8 @@ def parse():
                     help="Adam steps of the learned-h network (0 = skip the learned-h comparison)")
     ap.add_argument("--learned-eta", type=float, default=1e-5,
                     help="Newton-Raphson tolerance |dh/h| of the learned-h comparison")
+    ap.add_argument("--density-lanes", type=int, default=None,
+                    help="lanes per particle in the density pass (0, 8, 16, 32, 64; default: the library's)")
     ap.add_argument("--fixes", type=lambda v: int(v, 0), default=0,
                     help="SPH_FIX_* bit mask (0 = the reference step as written, 0x1f = every correction)")
     return ap.parse_args()
@@ -128,6 +130,8 @@ def learned_comparison(sph, args):
     --learned-eta; the pass time includes the feature and inference kernels.
     """
     prm = sph.default_params(density_eta=args.learned_eta, fixes=args.fixes)
+    if args.density_lanes is not None:
+        prm.density_lanes = args.density_lanes
     dump_steps = 8
     sizes = ((100_000, 11), (args.n, 12))
     smp = sph.Samples(sum(n for n, _ in sizes) * dump_steps)
@@ -218,6 +222,8 @@ def main():
         sys.exit(2)
     sph = importlib.import_module(PKG)
     prm = sph.default_params(fixes=args.fixes)
+    if args.density_lanes is not None:
+        prm.density_lanes = args.density_lanes
     tables = sph.make_eos_tables(prm, 256)
     cfg = sph.impact_config(args.n, args.seed)
     tree_mode = {"async": sph.TREE_ASYNC, "every": sph.TREE_EVERY, "fixed": sph.TREE_FIXED}[args.tree]
@@ -269,6 +275,7 @@ def main():
             "eos_table": "256x256 Tillotson per material",
             "tree_mode": args.tree,
             "tree_threads": args.tree_threads,
+            "density_lanes": int(prm.density_lanes),
             "trees_swapped_in_timed_region": trees1 - trees0,
             "last_tree_build_ms": round(build_ms, 2),
         },

@@ -87,7 +87,11 @@ typedef struct sph_params {
      * P2M/M2M, and the gravitational potential per particle (SPH_F_POTENTIAL)
      * for the FP64 conservation sums of sph_conservation. */
     int32_t precision;
-    int32_t reserved[7];
+    /* Lanes per particle in the density pass: 0 = one thread per particle
+     * (the reference's summation order), 8/16/32/64 = the particle's cell
+     * box split over that many lanes (same terms, tree-ordered sums). */
+    int32_t density_lanes;
+    int32_t reserved[6];
 } sph_params;
 
 /* One EOS table: res x res float2 (pressure, sound speed) in code units.

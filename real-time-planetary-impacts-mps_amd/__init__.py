@@ -58,7 +58,8 @@ class SphParams(ctypes.Structure):
         ("gravity_max_recursion", ctypes.c_int32), ("max_unchecked_pointers", ctypes.c_int32),
         ("max_children_in_leaf", ctypes.c_int32), ("max_tree_depth", ctypes.c_int32),
         ("cell_width", ctypes.c_float), ("cell_power", ctypes.c_int32),
-        ("fixes", ctypes.c_uint32), ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+        ("fixes", ctypes.c_uint32), ("precision", ctypes.c_int32), ("density_lanes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

@@ -38,6 +38,8 @@ bool valid_params(const sph_params* p, std::string& why) {
     if (p->n_neighbours_estim < 0) { why = "n_neighbours_estim must be >= 0"; return false; }
     if (p->max_nr_iterations < 0) { why = "max_nr_iterations must be >= 0"; return false; }
     if (p->precision && !(p->fixes & SPH_FIX_NEAR_FIELD)) { why = "the precision form needs SPH_FIX_NEAR_FIELD"; return false; }
+    if (p->density_lanes != 0 && p->density_lanes != 8 && p->density_lanes != 16 && p->density_lanes != 32 &&
+        p->density_lanes != 64) { why = "density_lanes must be 0, 8, 16, 32 or 64"; return false; }
     return true;
 }
 

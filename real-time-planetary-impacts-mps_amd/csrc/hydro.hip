@@ -308,6 +308,212 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
 }
 
+/* Group sums over G lanes in a fixed butterfly order (reproducible). */
+template <int G>
+__device__ __forceinline__ float gsum(float v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+template <int G>
+__device__ __forceinline__ int gsum_i(int v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+/* density.metal:12-239 with G lanes per particle.  The lanes of a group
+ * split the particle's cell box (lane l takes cells l, l + G, ... in the
+ * box's x -> y -> z order) and sum their cells' candidates; the group sums
+ * the partial results with a butterfly, so every lane holds the particle's
+ * sums and the Newton-Raphson control flow stays uniform over the group.
+ * The terms are the reference's; only the order of the additions differs
+ * (per-cell partial sums, then a tree), which changes results at the
+ * rounding level. */
+template <int G>
+__global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
+    constexpr int PPB = 256 / G;
+    const int blk = xcd_block((a.n + PPB - 1) / PPB);
+    if (blk < 0) return;
+    const int lane = threadIdx.x % G;
+    const int index = blk * PPB + (int)threadIdx.x / G;
+    if (index >= a.n) return;
+    if (index == 0 && lane == 0) a.time[1] = a.dt_reset;
+    const uint32_t ind = a.sorted[index].y;
+    const float2* table = a.mat[ind] == 400 ? a.eos0 : a.eos1;
+    if (!a.active[ind]) {
+        if (lane == 0) {
+            const float rho = a.rho[ind];
+            const float2 pc = eos_sample(table, a.eos_res, a.u[ind], rho, a.min_u, a.max_u, a.min_rho, a.max_rho);
+            a.P[ind] = pc.x;
+            a.c[ind] = pc.y;
+            a.nr_iters[ind] = 0;
+            a.work[ind] = make_int4(0, 0, 0, 0);
+        }
+        return;
+    }
+    const float4 pm = a.s_posm[index];
+    const float3 position = make_float3(pm.x, pm.y, pm.z);
+    const float mass_i = pm.w;
+    const uint2* __restrict__ cse = a.cells.cell_se;
+    const float4* __restrict__ spm = a.s_posm;
+    const float cs = a.cells.cell_size;
+    const int mask = a.cells.mask;
+
+    const float h_start = a.h[ind];
+    float h_i = a.guess ? a.guess[index] : h_start;
+    bool guided = a.guess != nullptr;
+    float eps_last = 3.0e38f;
+    int max_h = 0;
+    float h2x4 = 4.f * h_i * h_i;
+    float h1 = 1.f / h_i;
+    float density = 0.f;
+    float eps = 1.f;
+    int count = 0;
+    int nN = 0;
+    float h_scan = h_i;
+    float hs2x4 = h2x4;
+    int w_cand = 0, w_in = 0, f_cand = 0, f_in = 0;
+    while (eps > a.eta && count < a.max_it) {
+        const bool use_cache = (h_i < (float)max_h) && (nN < a.ne);
+        const float hb = use_cache ? h_scan : h_i;
+        const float hb2 = use_cache ? hs2x4 : h2x4;
+        const float hsum = h_i + h_i;
+        int mn[3], mx[3];
+        cell_box(position, cs, hb, mask + 1, mn, mx);
+        const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
+        const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
+        float part = 0.f;
+        int cnt = 0;
+        for (int c = lane; c < ncell; c += G) {
+            const int iz = c % nz, t = c / nz;
+            const int iy = t % ny, ix = t / ny;
+            const uint32_t key = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
+                                 (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1) |
+                                 (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
+            const uint2 se = cse[key];
+            if (se.x == 0xFFFFFFFFu) continue;
+            w_cand += (int)(se.y - se.x + 1);
+            for (uint32_t k = se.x; k <= se.y; k++) {
+                const float4 q = spm[k];
+                const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > hb2) continue;
+                w_in++;
+                const float r = sqrtf(r2);
+                if (use_cache) {
+                    if (r > hsum) continue;
+                } else {
+                    cnt++;
+                }
+                part += kW(r, h1) * q.w;
+            }
+        }
+        density = gsum<G>(part);
+        if (!use_cache) {
+            nN = min(gsum_i<G>(cnt), a.ne);
+            h_scan = h_i;
+            hs2x4 = h2x4;
+        }
+        float newH = fminf(a.res_eta * powf(mass_i / density, 1.f / 3.f), a.max_h_len);
+        newH = 0.5f * (h_i + newH);
+        eps = fabsf(newH - h_i) / h_i;
+        if (guided) {
+            if (eps > eps_last || eps > kGuessTrust) {
+                guided = false;
+                h_i = h_start;
+                h1 = 1.f / h_i;
+                h2x4 = 4.f * h_i * h_i;
+                max_h = 0;
+                eps = 1.f;
+                count++;
+                continue;
+            }
+            eps_last = eps;
+        }
+        if (eps > a.eta) {
+            max_h = (int)fmaxf((float)max_h, h_i);
+            h_i = newH;
+            h1 = 1.f / h_i;
+            h2x4 = 4.f * h_i * h_i;
+            count++;
+        }
+    }
+
+    /* second pass (density.metal:146-213) */
+    float density_h = 0.f, velDiv = 0.f, dpart = 0.f;
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    const float4 vi4 = a.s_vel[index];
+    const bool cached = nN < a.ne;
+    {
+        const float hb = cached ? h_scan : h_i;
+        const float hb2 = cached ? hs2x4 : h2x4;
+        const float hsum = h_i + h_i;
+        int mn[3], mx[3];
+        cell_box(position, cs, hb, mask + 1, mn, mx);
+        const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
+        const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
+        for (int c = lane; c < ncell; c += G) {
+            const int iz = c % nz, t = c / nz;
+            const int iy = t % ny, ix = t / ny;
+            const uint32_t key = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
+                                 (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1) |
+                                 (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
+            const uint2 se = cse[key];
+            if (se.x == 0xFFFFFFFFu) continue;
+            f_cand += (int)(se.y - se.x + 1);
+            for (uint32_t k = se.x; k <= se.y; k++) {
+                const float4 q = spm[k];
+                const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > hb2) continue;
+                f_in++;
+                const float r = sqrtf(r2);
+                if (cached && r > hsum) continue;
+                const float mass = q.w;
+                if (!cached) dpart += kW(r, h1) * mass;
+                density_h += mass * kdWdh(r, h1);
+                if (r > 1e-12f) {
+                    const float r1 = 1.f / r;
+                    const float4 vj = a.s_vel[k];
+                    const float vx = vj.x - vi4.x, vy = vj.y - vi4.y, vz = vj.z - vi4.z;
+                    const float g = kGradWFactor(r, h1);
+                    const float gx = g * (dx * r1), gy = g * (dy * r1), gz = g * (dz * r1);
+                    velDiv += mass * (vx * gx + vy * gy + vz * gz);
+                    cx += mass * (vy * gz - vz * gy);
+                    cy += mass * (vz * gx - vx * gz);
+                    cz += mass * (vx * gy - vy * gx);
+                }
+            }
+        }
+    }
+    /* as written, a rescan adds to the last NR density (Q7) unless fixed */
+    if (!cached) density = (a.fix_cache ? 0.f : density) + gsum<G>(dpart);
+    density_h = gsum<G>(density_h);
+    velDiv = gsum<G>(velDiv);
+    cx = gsum<G>(cx);
+    cy = gsum<G>(cy);
+    cz = gsum<G>(cz);
+    w_cand = gsum_i<G>(w_cand);
+    w_in = gsum_i<G>(w_in);
+    f_cand = gsum_i<G>(f_cand);
+    f_in = gsum_i<G>(f_in);
+    if (lane != 0) return;
+    a.nr_iters[ind] = count;
+    velDiv = fabsf(velDiv) / density;
+    const float absCurl = sqrtf(cx * cx + cy * cy + cz * cz) / density;
+    const float fgrad = 1.f / (1.f + (h_i / (3.f * density)) * density_h);
+    const float2 pc = eos_sample(table, a.eos_res, a.u[ind], density, a.min_u, a.max_u, a.min_rho, a.max_rho);
+    const float bal = velDiv / (velDiv + absCurl + 1e-4f * (pc.y / h_i));
+    a.h[ind] = h_i;
+    a.rho[ind] = density;
+    a.f[ind] = fgrad;
+    a.P[ind] = pc.x;
+    a.c[ind] = pc.y;
+    a.B[ind] = bal;
+    a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
+}
+
 /* Morton-ordered copies of what the acceleration loop reads for every
  * candidate, packed so that the range test needs one 16-byte load:
  * (x, y, z, h), (v, Balsara factor), (m, f P / rho^2, rho, c).  The product
@@ -536,7 +742,13 @@ int launch_density(sph_ctx* ctx) {
     a.dt_reset = (int)(p.dt0 * (1.f / dtmin1));
     a.guess = ctx->learned ? ctx->guess : nullptr;
     const int threads = 256;
-    k_density<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a);
+    switch (p.density_lanes) {
+        case 16: k_density_group<16><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
+        case 32: k_density_group<32><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
+        case 64: k_density_group<64><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
+        case 8: k_density_group<8><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
+        default: k_density<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a); break;
+    }
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }

@@ -106,6 +106,26 @@ def test_density_parity(sph, orc, tables256, impact, n, seed):
     assert int(ctx.download("dt")[0]) == int(st["dt"][0]) == 65536
 
 
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
+def test_density_lanes_parity(sph, orc, tables256, impact, lanes):
+    """The group-cooperative density pass (cell box split over `lanes` lanes
+    per particle, tree-ordered sums) against the oracle, including the
+    neighbour-cache paths (h at the cap, small cache)."""
+    n = 100000
+    ic = _stir(impact(n, 2), 2)
+    ic["h"][::97] = 1.0
+    prm, ctx, st = _pair(sph, orc, ic, tables256, density_lanes=lanes, n_neighbours_estim=64)
+    ctx.sort()
+    st.sort()
+    ctx.density()
+    st.density()
+    flips, s = _density_compare(ctx, st, n, f"lanes={lanes}")
+    assert flips <= max(2, n // 1000)
+    assert s["h"][0] < 1e-5 and s["rho"][0] < 1e-5 and s["gradterm"][0] < 1e-4
+    assert s["pressure"][0] < 1e-4 and s["soundspeed"][0] < 1e-4
+    assert s["balsara_abs"][0] < 1e-4
+
+
 def test_density_cache_paths(sph, orc, tables256, impact):
     """Particles at the h cap (h = 1) take the reference's neighbour-cache path
     inside the NR loop; a small cache forces the rescan (and its double count)."""

@@ -58,7 +58,7 @@ def test_default_params_are_the_reference_values(sph):
 
 
 def test_params_struct_layout_matches_c(sph):
-    # 26 four-byte fields, precision, 7 reserved ints
+    # 26 four-byte fields, precision, density_lanes, 6 reserved ints
     assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 8)
 
 
