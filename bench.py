@@ -63,9 +63,28 @@ def parse():
                     help="Newton-Raphson tolerance |dh/h| of the learned-h comparison")
     ap.add_argument("--density-lanes", type=int, default=None,
                     help="lanes per particle in the density pass (0, 8, 16, 32, 64; default: the library's)")
+    ap.add_argument("--accel-lanes", type=int, default=None,
+                    help="lanes per particle in the acceleration pass (0, 8, 16, 32, 64; default: the library's)")
+    ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
+                    help="override one sph_params field (repeatable), e.g. --param gravity_eta=2e-5")
+    ap.add_argument("--headon", action="store_true",
+                    help="head-on impact variant (impact parameter 0.3, 12 km/s, u0 5e6 J/kg)")
     ap.add_argument("--fixes", type=lambda v: int(v, 0), default=0,
                     help="SPH_FIX_* bit mask (0 = the reference step as written, 0x1f = every correction)")
     return ap.parse_args()
+
+
+def apply_overrides(prm, args):
+    """--density-lanes, --accel-lanes and --param NAME=VALUE on an sph_params."""
+    if args.density_lanes is not None:
+        prm.density_lanes = args.density_lanes
+    if args.accel_lanes is not None:
+        prm.accel_lanes = args.accel_lanes
+    for kv in args.param:
+        name, value = kv.split("=", 1)
+        cur = getattr(prm, name)
+        setattr(prm, name, type(cur)(float(value)) if isinstance(cur, float) else int(value, 0))
+    return prm
 
 
 def profiled_traffic():
@@ -129,15 +148,14 @@ def learned_comparison(sph, args):
     plain, and from the network's guess.  Both converge to |dh/h| <=
     --learned-eta; the pass time includes the feature and inference kernels.
     """
-    prm = sph.default_params(density_eta=args.learned_eta, fixes=args.fixes)
-    if args.density_lanes is not None:
-        prm.density_lanes = args.density_lanes
+    prm = apply_overrides(sph.default_params(fixes=args.fixes), args)
+    prm.density_eta = args.learned_eta
     dump_steps = 8
     sizes = ((100_000, 11), (args.n, 12))
     smp = sph.Samples(sum(n for n, _ in sizes) * dump_steps)
     t0 = time.perf_counter()
     for n, seed in sizes:
-        hl = sph.Headless(prm, sph.impact_config(n, seed), eos_res=256, tree_mode=sph.TREE_EVERY,
+        hl = sph.Headless(prm, sph.impact_config(n, seed, headon=args.headon), eos_res=256, tree_mode=sph.TREE_EVERY,
                           tree_threads=args.tree_threads)
         hl.step()
         hl.ctx.set_sample_dump(smp)
@@ -154,7 +172,7 @@ def learned_comparison(sph, args):
     t_train = time.perf_counter() - t0
     log(f"learned: {args.learned_steps} Adam steps in {t_train:.2f} s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
 
-    hl = sph.Headless(prm, sph.impact_config(args.n, args.seed), eos_res=256, tree_mode=sph.TREE_EVERY,
+    hl = sph.Headless(prm, sph.impact_config(args.n, args.seed, headon=args.headon), eos_res=256, tree_mode=sph.TREE_EVERY,
                       tree_threads=args.tree_threads)
     for _ in range(3):
         hl.step()
@@ -221,11 +239,9 @@ def main():
               "the multi-GPU decomposition of this path is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
     sph = importlib.import_module(PKG)
-    prm = sph.default_params(fixes=args.fixes)
-    if args.density_lanes is not None:
-        prm.density_lanes = args.density_lanes
+    prm = apply_overrides(sph.default_params(fixes=args.fixes), args)
     tables = sph.make_eos_tables(prm, 256)
-    cfg = sph.impact_config(args.n, args.seed)
+    cfg = sph.impact_config(args.n, args.seed, headon=args.headon)
     tree_mode = {"async": sph.TREE_ASYNC, "every": sph.TREE_EVERY, "fixed": sph.TREE_FIXED}[args.tree]
     t_setup = time.perf_counter()
     hl = sph.Headless(prm, cfg, eos_res=256, tree_mode=tree_mode, tree_threads=args.tree_threads)
@@ -266,16 +282,20 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": (f"two-body 45-degree impact, {args.n} particles (89% target / 11% impactor), plain "
+            "workload": (f"two-body {'head-on' if args.headon else '45-degree'} impact, {args.n} particles "
+                         "(89% target / 11% impactor), plain "
                          "Newton-Raphson smoothing length, FP32, "
                          + ("reference step as written" if args.fixes == 0 else f"corrections 0x{args.fixes:x}")
                          + f", CPU octree rebuilt on a thread ({args.tree})"),
             "fixes": args.fixes,
+            "param_overrides": args.param,
             "n_particles": args.n,
             "eos_table": "256x256 Tillotson per material",
             "tree_mode": args.tree,
             "tree_threads": args.tree_threads,
             "density_lanes": int(prm.density_lanes),
+            "accel_lanes": int(prm.accel_lanes),
+            "lane_split": int(prm.lane_split),
             "trees_swapped_in_timed_region": trees1 - trees0,
             "last_tree_build_ms": round(build_ms, 2),
         },

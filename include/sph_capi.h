@@ -91,7 +91,12 @@ typedef struct sph_params {
      * (the reference's summation order), 8/16/32/64 = the particle's cell
      * box split over that many lanes (same terms, tree-ordered sums). */
     int32_t density_lanes;
-    int32_t reserved[6];
+    /* The same for the acceleration pass (0 = one thread per particle). */
+    int32_t accel_lanes;
+    /* With lanes > 0: 0 = each lane takes whole cells of the box, 1 = the
+     * lanes walk the box together and split each cell's candidates. */
+    int32_t lane_split;
+    int32_t reserved[4];
 } sph_params;
 
 /* One EOS table: res x res float2 (pressure, sound speed) in code units.

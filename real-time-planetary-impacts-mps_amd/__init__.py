@@ -59,7 +59,7 @@ class SphParams(ctypes.Structure):
         ("max_children_in_leaf", ctypes.c_int32), ("max_tree_depth", ctypes.c_int32),
         ("cell_width", ctypes.c_float), ("cell_power", ctypes.c_int32),
         ("fixes", ctypes.c_uint32), ("precision", ctypes.c_int32), ("density_lanes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("accel_lanes", ctypes.c_int32), ("lane_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
     ]
 
 

@@ -40,6 +40,9 @@ bool valid_params(const sph_params* p, std::string& why) {
     if (p->precision && !(p->fixes & SPH_FIX_NEAR_FIELD)) { why = "the precision form needs SPH_FIX_NEAR_FIELD"; return false; }
     if (p->density_lanes != 0 && p->density_lanes != 8 && p->density_lanes != 16 && p->density_lanes != 32 &&
         p->density_lanes != 64) { why = "density_lanes must be 0, 8, 16, 32 or 64"; return false; }
+    if (p->accel_lanes != 0 && p->accel_lanes != 8 && p->accel_lanes != 16 && p->accel_lanes != 32 &&
+        p->accel_lanes != 64) { why = "accel_lanes must be 0, 8, 16, 32 or 64"; return false; }
+    if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }
     return true;
 }
 

@@ -322,15 +322,17 @@ __device__ __forceinline__ int gsum_i(int v) {
     return v;
 }
 
-/* density.metal:12-239 with G lanes per particle.  The lanes of a group
- * split the particle's cell box (lane l takes cells l, l + G, ... in the
- * box's x -> y -> z order) and sum their cells' candidates; the group sums
- * the partial results with a butterfly, so every lane holds the particle's
- * sums and the Newton-Raphson control flow stays uniform over the group.
+/* density.metal:12-239 with G lanes per particle.  SPLIT 0: the lanes of a
+ * group split the particle's cell box (lane l takes cells l, l + G, ... in
+ * the box's x -> y -> z order).  SPLIT 1: the group walks the box's cells
+ * together and lane l takes candidates l, l + G, ... of each cell, so the
+ * lanes read consecutive records.  Either way the group sums the partial
+ * results with a butterfly, so every lane holds the particle's sums and the
+ * Newton-Raphson control flow stays uniform over the group.
  * The terms are the reference's; only the order of the additions differs
  * (per-cell partial sums, then a tree), which changes results at the
  * rounding level. */
-template <int G>
+template <int G, int SPLIT>
 __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     constexpr int PPB = 256 / G;
     const int blk = xcd_block((a.n + PPB - 1) / PPB);
@@ -385,7 +387,7 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
         const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
         float part = 0.f;
         int cnt = 0;
-        for (int c = lane; c < ncell; c += G) {
+        for (int c = SPLIT ? 0 : lane; c < ncell; c += SPLIT ? 1 : G) {
             const int iz = c % nz, t = c / nz;
             const int iy = t % ny, ix = t / ny;
             const uint32_t key = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
@@ -393,8 +395,8 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
                                  (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
             const uint2 se = cse[key];
             if (se.x == 0xFFFFFFFFu) continue;
-            w_cand += (int)(se.y - se.x + 1);
-            for (uint32_t k = se.x; k <= se.y; k++) {
+            if (!SPLIT || lane == 0) w_cand += (int)(se.y - se.x + 1);
+            for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) {
                 const float4 q = spm[k];
                 const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
         cell_box(position, cs, hb, mask + 1, mn, mx);
         const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
         const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
-        for (int c = lane; c < ncell; c += G) {
+        for (int c = SPLIT ? 0 : lane; c < ncell; c += SPLIT ? 1 : G) {
             const int iz = c % nz, t = c / nz;
             const int iy = t % ny, ix = t / ny;
             const uint32_t key = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
@@ -461,8 +463,8 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
                                  (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
             const uint2 se = cse[key];
             if (se.x == 0xFFFFFFFFu) continue;
-            f_cand += (int)(se.y - se.x + 1);
-            for (uint32_t k = se.x; k <= se.y; k++) {
+            if (!SPLIT || lane == 0) f_cand += (int)(se.y - se.x + 1);
+            for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) {
                 const float4 q = spm[k];
                 const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
@@ -650,6 +652,112 @@ __global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
     if ((threadIdx.x & 63) == 0 && wmin != 0x7FFFFFFF) atomicMin(&a.time[1], wmin);
 }
 
+/* accelerations.metal:12-141 with G lanes per particle, the box split as
+ * in k_density_group (SPLIT 0: whole cells per lane; SPLIT 1: a cell's
+ * candidates over the lanes); the group combines the partial sums with a
+ * butterfly (sums) and a butterfly maximum (v_sig).  Same terms as the reference; only the
+ * order of the additions differs. */
+template <int G, int SPLIT>
+__global__ __launch_bounds__(256) void k_acceleration_group(AccelArgs a) {
+    constexpr int PPB = 256 / G;
+    const int blk = xcd_block((a.n + PPB - 1) / PPB);
+    if (blk < 0) return;
+    const int lane = threadIdx.x % G;
+    const int index = blk * PPB + (int)threadIdx.x / G;
+    int integerDt = 0x7FFFFFFF;
+    const uint32_t ind = index < a.n ? a.sorted[index].y : 0u;
+    if (index < a.n && a.active[ind]) {
+        const float4 xi4 = a.s_posh[index];
+        const float4 vi4 = a.s_velb[index];
+        const float4 hyd = a.s_hyd[index];
+        const float h_i = xi4.w, fact_i = hyd.y, rho_i = hyd.z, c_i = hyd.w;
+        const float B_i = vi4.w;
+        const float hi1 = 1.f / h_i;
+        const float hi2x4 = h_i * h_i * 4.f;
+        float dvx = 0.f, dvy = 0.f, dvz = 0.f, du_dt = 0.f, dhdt = 0.f;
+        float v_sigi = 1e-12f;
+        const uint2* __restrict__ cse = a.cells.cell_se;
+        const int mask = a.cells.mask;
+        int mn[3], mx[3];
+        cell_box(make_float3(xi4.x, xi4.y, xi4.z), a.cells.cell_size, fminf(h_i, a.max_h_len), mask + 1, mn, mx);
+        const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
+        const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
+        for (int c = SPLIT ? 0 : lane; c < ncell; c += SPLIT ? 1 : G) {
+            const int iz = c % nz, t = c / nz;
+            const int iy = t % ny, ix = t / ny;
+            const uint32_t key = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
+                                 (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1) |
+                                 (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
+            const uint2 se = cse[key];
+            if (se.x == 0xFFFFFFFFu) continue;
+            for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) {
+                const float4 xj4 = a.s_posh[k];
+                const float dx = xi4.x - xj4.x, dy = xi4.y - xj4.y, dz = xi4.z - xj4.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float h_j = xj4.w;
+                const float hj2x4 = h_j * h_j * 4.f;
+                if ((r2 > hi2x4 && r2 > hj2x4) || r2 < 1e-12f) continue;
+                const float r = sqrtf(r2);
+                const float r1 = 1.f / r;
+                const float4 hj = a.s_hyd[k];
+                const float4 vj4 = a.s_velb[k];
+                const float m_j = hj.x;
+                const float fact_j = hj.y;
+                const float rho_j = hj.z;
+                const float c_j = hj.w;
+                const float hj1 = 1.f / h_j;
+                const float vx = vi4.x - vj4.x, vy = vi4.y - vj4.y, vz = vi4.z - vj4.z;
+                const float rho_ij = 0.5f * (rho_i + rho_j);
+                const float v_dot_r = vx * dx + vy * dy + vz * dz;
+                const float mu_ij = v_dot_r < 0.f ? v_dot_r * r1 : 0.f;
+                const float v_sigij = c_i + c_j - 3.f * mu_ij;
+                v_sigi = fmaxf(v_sigij, v_sigi);
+                const float B_ij = 0.5f * (B_i + vj4.w);
+                const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij / rho_ij;
+                const float rx = dx * r1, ry = dy * r1, rz = dz * r1;
+                const float gi = kGradWFactor(r, hi1), gj = kGradWFactor(r, hj1);
+                const float gix = gi * rx, giy = gi * ry, giz = gi * rz;
+                const float gjx = gj * rx, gjy = gj * ry, gjz = gj * rz;
+                const float gmx = 0.5f * (gix + gjx), gmy = 0.5f * (giy + gjy), gmz = 0.5f * (giz + gjz);
+                dvx -= m_j * (fact_i * gix + fact_j * gjx + nu_ij * gmx);
+                dvy -= m_j * (fact_i * giy + fact_j * gjy + nu_ij * gmy);
+                dvz -= m_j * (fact_i * giz + fact_j * gjz + nu_ij * gmz);
+                const float vgi = vx * gix + vy * giy + vz * giz;
+                const float vgm = vx * gmx + vy * gmy + vz * gmz;
+                du_dt += m_j * (fact_i * vgi + 0.5f * nu_ij * vgm);
+                dhdt += (m_j / rho_j) * vgi;
+            }
+        }
+        dvx = gsum<G>(dvx);
+        dvy = gsum<G>(dvy);
+        dvz = gsum<G>(dvz);
+        du_dt = gsum<G>(du_dt);
+        dhdt = gsum<G>(dhdt);
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) v_sigi = fmaxf(v_sigi, __shfl_xor(v_sigi, off, 64));
+        if (lane == 0) {
+            float4 ac = a.acc[ind];
+            ac.x += dvx;
+            ac.y += dvy;
+            ac.z += dvz;
+            a.acc[ind] = ac;
+            a.du[ind] = du_dt;
+            a.dhdt[ind] = dhdt;
+            const float goaldt = 2.0f * a.cfl * a.gamma * h_i / v_sigi;
+            int r = 0;
+            float dtCandidate = a.dt0;
+            while (dtCandidate > goaldt && r < a.r_max) {
+                dtCandidate *= 0.5f;
+                r++;
+            }
+            integerDt = (int)ceilf(a.dtmin * dtCandidate);
+            a.next_active[ind] = a.time[0] + integerDt;
+        }
+    }
+    const int wmin = wave_min(integerDt);
+    if ((threadIdx.x & 63) == 0 && wmin != 0x7FFFFFFF) atomicMin(&a.time[1], wmin);
+}
+
 struct StepArgs {
     int n;
     float4 *pos, *vel;
@@ -742,11 +850,16 @@ int launch_density(sph_ctx* ctx) {
     a.dt_reset = (int)(p.dt0 * (1.f / dtmin1));
     a.guess = ctx->learned ? ctx->guess : nullptr;
     const int threads = 256;
-    switch (p.density_lanes) {
-        case 16: k_density_group<16><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
-        case 32: k_density_group<32><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
-        case 64: k_density_group<64><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
-        case 8: k_density_group<8><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
+    const int dsel = p.density_lanes * 2 + (p.lane_split ? 1 : 0);
+    switch (dsel) {
+        case 16: k_density_group<8, 0><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
+        case 17: k_density_group<8, 1><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
+        case 32: k_density_group<16, 0><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
+        case 33: k_density_group<16, 1><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
+        case 64: k_density_group<32, 0><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
+        case 65: k_density_group<32, 1><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
+        case 128: k_density_group<64, 0><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
+        case 129: k_density_group<64, 1><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
         default: k_density<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a); break;
     }
     SPH_HIP_TRY(hipGetLastError());
@@ -781,7 +894,18 @@ int launch_acceleration(sph_ctx* ctx) {
         ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m, ctx->h, ctx->f, ctx->P, ctx->rho, ctx->c, ctx->B,
         ctx->s_posh, ctx->s_velb, ctx->s_hyd);
     SPH_HIP_TRY(hipGetLastError());
-    k_acceleration<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a);
+    const int asel = p.accel_lanes * 2 + (p.lane_split ? 1 : 0);
+    switch (asel) {
+        case 16: k_acceleration_group<8, 0><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
+        case 17: k_acceleration_group<8, 1><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
+        case 32: k_acceleration_group<16, 0><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
+        case 33: k_acceleration_group<16, 1><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
+        case 64: k_acceleration_group<32, 0><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
+        case 65: k_acceleration_group<32, 1><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
+        case 128: k_acceleration_group<64, 0><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
+        case 129: k_acceleration_group<64, 1><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
+        default: k_acceleration<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a); break;
+    }
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }

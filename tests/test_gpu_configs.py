@@ -1,0 +1,79 @@
+"""Whole-run checks of the stated configurations on the GPU:
+
+  plain method, 1e5 particles, up to 4 timestep rungs (individual steps):
+      ten steps against the oracle;
+  precision form, 1e6 particles, up to 8 rungs, 200 steps: gravity RMS
+      against an FP64 direct sum on 8192 sampled particles (seed 8192), and
+      energy / angular-momentum drift over the run.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_plain_1e5_four_rungs_matches_oracle(sph, orc, tables256, impact):
+    n = 100000
+    prm = sph.default_params(activity_rule=sph.SPH_ACTIVE_README, max_rungs=4)
+    ic = impact(n, 2)
+    ctx = sph.Context(prm, ic, tables256)
+    st = orc.State(prm, ic, tables256)
+    for s in range(10):
+        tree = orc.build_octree(st["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+        ctx.upload_tree(tree)
+        ctx.sort()
+        ctx.gravity()
+        ctx.density()
+        ctx.acceleration()
+        ctx.step()
+        ctx.sync()
+        st.full_step(tree)
+        act = ctx.download("active")
+        d = np.abs(ctx.download("pos")[:, :3] - st["pos"][:, :3]).max()
+        print(f"[rungs] step {s} active {act.mean():.3f} t {ctx.download('global_time')[0]}/"
+              f"{st['global_time'][0]} max |dx| {d:.2e}", flush=True)
+        assert int(ctx.download("global_time")[0]) == int(st["global_time"][0])
+    assert len(np.unique(ctx.download("next_active"))) > 1      # particles sit on different rungs
+    assert d < 1e-4
+
+
+def test_precision_1e6_rms_and_drift(sph, orc):
+    """Precision form at 1e6 (eta 2e-5, 16384 list entries), 200 steps with
+    up to 8 rungs through the host Headless loop (octree rebuilt before every
+    step), then the FP64 direct sum at 8192 particles."""
+    n = 1_000_000
+    prm = sph.default_params(fixes=sph.SPH_FIX_ALL, precision=1, gravity_eta=2e-5, max_unchecked_pointers=16384,
+                             activity_rule=sph.SPH_ACTIVE_README, max_rungs=8)
+    hl = sph.Headless(prm, sph.impact_config(n, 3), eos_res=256, tree_mode=sph.TREE_EVERY, tree_threads=8)
+    ctx = hl.ctx
+    hl.step()
+    c0 = ctx.conservation()
+    e0 = c0["kinetic"] + c0["internal"] + c0["potential"]
+    for s in range(199):
+        hl.step()
+        if s % 50 == 0:
+            print(f"[precision 1e6] step {s + 2} t {ctx.time():.2f} s", flush=True)
+    c1 = ctx.conservation()
+    e1 = c1["kinetic"] + c1["internal"] + c1["potential"]
+    de = abs(e1 - e0) / abs(e0)
+    dl = np.linalg.norm(c1["angular_momentum"] - c0["angular_momentum"]) / np.linalg.norm(c0["angular_momentum"])
+    # a fresh gravity evaluation with every particle active, on the final state
+    prm_all = sph.default_params(fixes=sph.SPH_FIX_ALL, precision=1, gravity_eta=2e-5,
+                                 max_unchecked_pointers=16384)
+    state = {k: ctx.download(k) for k in ("pos", "vel", "rho", "u", "mass", "mat", "pressure", "h")}
+    gn = ctx.download("gravnorm")
+    hl.close()
+    g = sph.Context(prm_all, state, sph.make_eos_tables(prm_all, 64))
+    g.upload("gravnorm", gn)
+    g.upload_tree(orc.build_octree(state["pos"], 4, 7))
+    g.gravity()
+    g.sync()
+    targets = np.random.default_rng(8192).choice(n, 8192, replace=False).astype(np.int32)
+    ref, _ = orc.direct_sum(state["pos"], state["mass"], targets, prm.gravity_g, -prm.gravity_softening)
+    a = g.download("acc")[targets, :3].astype(np.float64)
+    rel = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    rms = float(np.sqrt(np.mean(rel ** 2)))
+    print(f"[precision 1e6] gravity RMS {rms:.3e} median {np.median(rel):.3e} max {rel.max():.3e}; "
+          f"|dE/E| {de:.3e} |dL|/|L| {dl:.3e}", flush=True)
+    assert rms < 1e-3
+    assert de < 5e-2 and dl < 5e-3

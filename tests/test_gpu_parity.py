@@ -106,20 +106,20 @@ def test_density_parity(sph, orc, tables256, impact, n, seed):
     assert int(ctx.download("dt")[0]) == int(st["dt"][0]) == 65536
 
 
-@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
-def test_density_lanes_parity(sph, orc, tables256, impact, lanes):
+@pytest.mark.parametrize("lanes,split", [(8, 0), (16, 0), (32, 0), (64, 0), (8, 1), (16, 1), (32, 1), (64, 1)])
+def test_density_lanes_parity(sph, orc, tables256, impact, lanes, split):
     """The group-cooperative density pass (cell box split over `lanes` lanes
     per particle, tree-ordered sums) against the oracle, including the
     neighbour-cache paths (h at the cap, small cache)."""
     n = 100000
     ic = _stir(impact(n, 2), 2)
     ic["h"][::97] = 1.0
-    prm, ctx, st = _pair(sph, orc, ic, tables256, density_lanes=lanes, n_neighbours_estim=64)
+    prm, ctx, st = _pair(sph, orc, ic, tables256, density_lanes=lanes, lane_split=split, n_neighbours_estim=64)
     ctx.sort()
     st.sort()
     ctx.density()
     st.density()
-    flips, s = _density_compare(ctx, st, n, f"lanes={lanes}")
+    flips, s = _density_compare(ctx, st, n, f"lanes={lanes} split={split}")
     assert flips <= max(2, n // 1000)
     assert s["h"][0] < 1e-5 and s["rho"][0] < 1e-5 and s["gradterm"][0] < 1e-4
     assert s["pressure"][0] < 1e-4 and s["soundspeed"][0] < 1e-4
@@ -143,10 +143,13 @@ def test_density_cache_paths(sph, orc, tables256, impact):
 
 
 # ----------------------------------------------------------- acceleration
-@pytest.mark.parametrize("n,seed", [(20000, 1), (100000, 2)])
-def test_acceleration_parity(sph, orc, tables256, impact, n, seed):
+@pytest.mark.parametrize("n,seed,lanes,split", [(20000, 1, 0, 0), (100000, 2, 0, 0), (100000, 2, 8, 0),
+                                                (100000, 2, 64, 0), (100000, 2, 8, 1), (100000, 2, 16, 1),
+                                                (100000, 2, 32, 1), (100000, 2, 64, 1)])
+def test_acceleration_parity(sph, orc, tables256, impact, n, seed, lanes, split):
+    """lanes > 0: the group-cooperative pass (cell box split over the lanes)."""
     ic = _stir(impact(n, seed), seed)
-    prm, ctx, st = _pair(sph, orc, ic, tables256)
+    prm, ctx, st = _pair(sph, orc, ic, tables256, accel_lanes=lanes, lane_split=split)
     st.sort()
     st.density()
     ctx.sort()
@@ -164,7 +167,7 @@ def test_acceleration_parity(sph, orc, tables256, impact, n, seed):
     e_du = np.abs(ctx.download("du") - st["du"]) / du_s
     e_dh = np.abs(ctx.download("dhdt") - st["dhdt"]) / dh_s
     na_same = np.mean(ctx.download("next_active") == st["next_active"])
-    print(f"[accel] n={n} acc max={err.max():.3e} med={np.median(err):.3e} du max={e_du.max():.3e} "
+    print(f"[accel] n={n} lanes={lanes} split={split} acc max={err.max():.3e} med={np.median(err):.3e} du max={e_du.max():.3e} "
           f"dhdt max={e_dh.max():.3e} next_active same={na_same:.6f} dt={ctx.download('dt')[0]}/{st['dt'][0]}",
           flush=True)
     assert err.max() < 1e-5 and e_du.max() < 1e-4 and e_dh.max() < 1e-4

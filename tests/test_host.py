@@ -58,7 +58,7 @@ def test_default_params_are_the_reference_values(sph):
 
 
 def test_params_struct_layout_matches_c(sph):
-    # 26 four-byte fields, precision, density_lanes, 6 reserved ints
+    # 26 four-byte fields, precision, density_lanes, accel_lanes, lane_split, 4 reserved ints
     assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 8)
 
 
