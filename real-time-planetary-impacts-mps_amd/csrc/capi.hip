@@ -87,6 +87,7 @@ void destroy_ctx(sph_ctx* ctx) {
     dfree(ctx->cell_se);
     dfree(ctx->tile_hist);
     dfree(ctx->grav_flags);
+    dfree(ctx->digit_total);
     dfree(ctx->feats);
     dfree(ctx->guess);
     dfree(ctx->eos[0]);
@@ -269,6 +270,7 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->cell_se, (size_t)ctx->ncells));
     ctx->ntiles = (int)((n + 2047) / 2048);
     TRY(dalloc(&ctx->tile_hist, (size_t)ctx->ntiles * 256));
+    TRY(dalloc(&ctx->digit_total, 256));
     TRY(dalloc(&ctx->grav_flags, 4));
     TRY(hipMemset(ctx->grav_flags, 0, 16));
 

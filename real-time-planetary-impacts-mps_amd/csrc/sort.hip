@@ -14,10 +14,12 @@
  *
  * Per pass, three kernels:
  *   k_hash_hist / k_hist  per-tile digit histogram (tile = 2048 pairs), LDS atomics
- *   k_scan                one workgroup: exclusive scan of the digit-major
- *                         [256][ntiles] table = global destination of each
- *                         (digit, tile) run
- *   k_scatter             per tile, 8 rounds of 256 consecutive pairs; a
+ *   k_scan_rows           one workgroup per digit: exclusive scan of the
+ *                         digit's row of the digit-major [256][ntiles]
+ *                         table and the digit's total
+ *   k_scatter             per tile: destination base of each digit = prefix
+ *                         of the digit totals + the tile's row offset; then
+ *                         8 rounds of 256 consecutive pairs; a
  *                         round ranks its pairs stably with 8 wave ballots per
  *                         digit bit (lanes with equal digit = one peer mask) and
  *                         a per-wave LDS count, then scatters
@@ -80,22 +82,25 @@ __global__ __launch_bounds__(kThreads) void k_hist(const uint2* __restrict__ pai
     tile_hist[threadIdx.x * ntiles + t] = hist[threadIdx.x];
 }
 
-/* Exclusive scan of `count` entries in place, one workgroup of 1024 walking
- * the array in coalesced chunks of 4096 (4 consecutive entries per lane):
- * lane-local scan, wave scan of the lane sums with shuffles, a scan of the
- * 16 wave sums, and a carry from chunk to chunk. */
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ a, int count) {
-    __shared__ uint32_t wsum[16];
+/* Exclusive scan of one digit's row of the digit-major tile table (one
+ * workgroup per digit), in place, and the digit's total.  The workgroup
+ * walks the row in coalesced chunks of 1024 (4 consecutive entries per
+ * lane): lane-local scan, wave scan of the lane sums with shuffles, a scan
+ * of the 4 wave sums, and a carry from chunk to chunk. */
+__global__ __launch_bounds__(kThreads) void k_scan_rows(uint32_t* __restrict__ table, int ntiles,
+                                                        uint32_t* __restrict__ digit_total) {
+    __shared__ uint32_t wsum[kWaves];
     __shared__ uint32_t carry_s;
+    uint32_t* __restrict__ a = table + (int64_t)blockIdx.x * ntiles;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     if (tid == 0) carry_s = 0;
     __syncthreads();
-    for (int base = 0; base < count; base += 4096) {
+    for (int base = 0; base < ntiles; base += 4 * kThreads) {
         const int i0 = base + 4 * tid;
         uint32_t v[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = i0 + k < count ? a[i0 + k] : 0u;
+        for (int k = 0; k < 4; ++k) v[k] = i0 + k < ntiles ? a[i0 + k] : 0u;
         const uint32_t t = v[0] + v[1] + v[2] + v[3];
         uint32_t incl = t;
 #pragma unroll
@@ -105,40 +110,50 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ a, int cou
         }
         if (lane == 63) wsum[wave] = incl;
         __syncthreads();
-        if (wave == 0) {
-            uint32_t w = lane < 16 ? wsum[lane] : 0u;
-            uint32_t wi = w;
-#pragma unroll
-            for (int off = 1; off < 16; off <<= 1) {
-                const uint32_t u = __shfl_up(wi, off, 64);
-                if (lane >= off) wi += u;
-            }
-            if (lane < 16) wsum[lane] = wi - w;
-        }
-        __syncthreads();
-        uint32_t run = carry_s + wsum[wave] + (incl - t);
+        uint32_t before = 0;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        uint32_t run = carry_s + before + (incl - t);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (i0 + k < count) a[i0 + k] = run;
+            if (i0 + k < ntiles) a[i0 + k] = run;
             run += v[k];
         }
         __syncthreads();
-        if (tid == 1023) carry_s = run;
+        if (tid == kThreads - 1) carry_s = run;
         __syncthreads();
     }
+    if (tid == 0) digit_total[blockIdx.x] = carry_s;
 }
 
+/* Stable scatter of one tile: the destination of digit d's pairs is the
+ * exclusive prefix of the digit totals (formed here from the 256 totals)
+ * plus the tile's offset within digit d's row. */
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint2* __restrict__ src, uint2* __restrict__ dst, int n,
-                                                      int shift, const uint32_t* __restrict__ tile_off, int ntiles) {
+                                                      int shift, const uint32_t* __restrict__ tile_off, int ntiles,
+                                                      const uint32_t* __restrict__ digit_total) {
     __shared__ uint32_t goff[256];
     __shared__ uint32_t running[256];
     __shared__ uint32_t wave_cnt[kWaves][256];
     __shared__ uint32_t wave_off[kWaves][256];
+    __shared__ uint32_t wtot[kWaves];
     const int t = blockIdx.x;
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    goff[tid] = tile_off[tid * ntiles + t];
+    {
+        const uint32_t dt = digit_total[tid];
+        uint32_t incl = dt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
+        }
+        if (lane == 63) wtot[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int w = 0; w < wave; ++w) before += wtot[w];
+        goff[tid] = before + (incl - dt) + tile_off[tid * ntiles + t];
+    }
     running[tid] = 0;
     for (int w = 0; w < kWaves; ++w) wave_cnt[w][tid] = 0;
     __syncthreads();
@@ -238,9 +253,10 @@ int launch_sort(sph_ctx* ctx) {
             k_hist<<<ntiles, kThreads, 0, s>>>(bufs[cur], n, shift, ctx->tile_hist, ntiles);
             SPH_HIP_TRY(hipGetLastError());
         }
-        k_scan<<<1, 1024, 0, s>>>(ctx->tile_hist, 256 * ntiles);
+        k_scan_rows<<<256, kThreads, 0, s>>>(ctx->tile_hist, ntiles, ctx->digit_total);
         SPH_HIP_TRY(hipGetLastError());
-        k_scatter<<<ntiles, kThreads, 0, s>>>(bufs[cur], bufs[cur ^ 1], n, shift, ctx->tile_hist, ntiles);
+        k_scatter<<<ntiles, kThreads, 0, s>>>(bufs[cur], bufs[cur ^ 1], n, shift, ctx->tile_hist, ntiles,
+                                              ctx->digit_total);
         SPH_HIP_TRY(hipGetLastError());
         cur ^= 1;
     }

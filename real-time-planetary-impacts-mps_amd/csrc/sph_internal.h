@@ -95,6 +95,7 @@ struct sph_ctx {
     bool have_prev = false;
     uint2* cell_se = nullptr;
     uint32_t* tile_hist = nullptr; /* 256 x ntiles, digit-major */
+    uint32_t* digit_total = nullptr; /* 256: pairs per digit of the current pass */
     int ntiles = 0;
 
     /* Morton-ordered copies */

@@ -55,6 +55,7 @@ def test_default_params_are_the_reference_values(sph):
     assert p.max_tree_depth == 7 and p.cell_width == pytest.approx(0.18) and p.cell_power == 8
     assert p.gravity_g == pytest.approx(6.67e-5) and p.fixes == 0
     assert p.activity_rule == sph.SPH_ACTIVE_AS_WRITTEN and p.max_rungs == 0
+    assert p.precision == 0 and p.density_lanes == 8 and p.accel_lanes == 0 and p.lane_split == 0
 
 
 def test_params_struct_layout_matches_c(sph):
