@@ -88,7 +88,7 @@ typedef struct sph_params {
      * for the FP64 conservation sums of sph_conservation. */
     int32_t precision;
     /* Lanes per particle in the density pass: 0 = one thread per particle
-     * (the reference's summation order), 8/16/32/64 = the particle's cell
+     * (the reference's summation order), 2/4/8/16/32/64 = the particle's cell
      * box split over that many lanes (same terms, tree-ordered sums). */
     int32_t density_lanes;
     /* The same for the acceleration pass (0 = one thread per particle). */

@@ -38,10 +38,11 @@ bool valid_params(const sph_params* p, std::string& why) {
     if (p->n_neighbours_estim < 0) { why = "n_neighbours_estim must be >= 0"; return false; }
     if (p->max_nr_iterations < 0) { why = "max_nr_iterations must be >= 0"; return false; }
     if (p->precision && !(p->fixes & SPH_FIX_NEAR_FIELD)) { why = "the precision form needs SPH_FIX_NEAR_FIELD"; return false; }
-    if (p->density_lanes != 0 && p->density_lanes != 8 && p->density_lanes != 16 && p->density_lanes != 32 &&
-        p->density_lanes != 64) { why = "density_lanes must be 0, 8, 16, 32 or 64"; return false; }
-    if (p->accel_lanes != 0 && p->accel_lanes != 8 && p->accel_lanes != 16 && p->accel_lanes != 32 &&
-        p->accel_lanes != 64) { why = "accel_lanes must be 0, 8, 16, 32 or 64"; return false; }
+    auto lanes_ok = [](int v) { return v == 0 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32 || v == 64; };
+    if (!lanes_ok(p->density_lanes)) { why = "density_lanes must be 0, 2, 4, 8, 16, 32 or 64"; return false; }
+    if (!lanes_ok(p->accel_lanes)) { why = "accel_lanes must be 0, 2, 4, 8, 16, 32 or 64"; return false; }
+    if ((p->density_lanes == 2 || p->density_lanes == 4 || p->accel_lanes == 2 || p->accel_lanes == 4) &&
+        p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }
     return true;
 }

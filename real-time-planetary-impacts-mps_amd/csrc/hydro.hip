@@ -852,6 +852,8 @@ int launch_density(sph_ctx* ctx) {
     const int threads = 256;
     const int dsel = p.density_lanes * 2 + (p.lane_split ? 1 : 0);
     switch (dsel) {
+        case 4: k_density_group<2, 0><<<xcd_grid((ctx->n + 127) / 128), threads, 0, ctx->stream>>>(a); break;
+        case 8: k_density_group<4, 0><<<xcd_grid((ctx->n + 63) / 64), threads, 0, ctx->stream>>>(a); break;
         case 16: k_density_group<8, 0><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
         case 17: k_density_group<8, 1><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
         case 32: k_density_group<16, 0><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
@@ -896,6 +898,8 @@ int launch_acceleration(sph_ctx* ctx) {
     SPH_HIP_TRY(hipGetLastError());
     const int asel = p.accel_lanes * 2 + (p.lane_split ? 1 : 0);
     switch (asel) {
+        case 4: k_acceleration_group<2, 0><<<xcd_grid((ctx->n + 127) / 128), threads, 0, ctx->stream>>>(a); break;
+        case 8: k_acceleration_group<4, 0><<<xcd_grid((ctx->n + 63) / 64), threads, 0, ctx->stream>>>(a); break;
         case 16: k_acceleration_group<8, 0><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
         case 17: k_acceleration_group<8, 1><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
         case 32: k_acceleration_group<16, 0><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;

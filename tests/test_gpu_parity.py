@@ -106,7 +106,8 @@ def test_density_parity(sph, orc, tables256, impact, n, seed):
     assert int(ctx.download("dt")[0]) == int(st["dt"][0]) == 65536
 
 
-@pytest.mark.parametrize("lanes,split", [(8, 0), (16, 0), (32, 0), (64, 0), (8, 1), (16, 1), (32, 1), (64, 1)])
+@pytest.mark.parametrize("lanes,split", [(2, 0), (4, 0), (8, 0), (16, 0), (32, 0), (64, 0), (8, 1), (16, 1), (32, 1),
+                                         (64, 1)])
 def test_density_lanes_parity(sph, orc, tables256, impact, lanes, split):
     """The group-cooperative density pass (cell box split over `lanes` lanes
     per particle, tree-ordered sums) against the oracle, including the
@@ -143,7 +144,8 @@ def test_density_cache_paths(sph, orc, tables256, impact):
 
 
 # ----------------------------------------------------------- acceleration
-@pytest.mark.parametrize("n,seed,lanes,split", [(20000, 1, 0, 0), (100000, 2, 0, 0), (100000, 2, 8, 0),
+@pytest.mark.parametrize("n,seed,lanes,split", [(20000, 1, 0, 0), (100000, 2, 0, 0), (100000, 2, 2, 0),
+                                                (100000, 2, 4, 0), (100000, 2, 8, 0),
                                                 (100000, 2, 64, 0), (100000, 2, 8, 1), (100000, 2, 16, 1),
                                                 (100000, 2, 32, 1), (100000, 2, 64, 1)])
 def test_acceleration_parity(sph, orc, tables256, impact, n, seed, lanes, split):
