@@ -88,7 +88,7 @@ def apply_overrides(prm, args):
 
 
 def profiled_traffic():
-    """HBM bytes per k_density launch at the default workload, measured by
+    """HBM bytes per density-pass launch at the default workload, measured by
     two rocprofv3 counter passes and recorded in profiles/."""
     path = os.path.join(ROOT, "profiles", "k_density_hbm_traffic.md")
     try:
@@ -304,7 +304,9 @@ def main():
         per_pass = {k: v["ms"] / max(v["calls"], 1) for k, v in passes.items()}
         out["pass_ms"] = {k: round(v, 4) for k, v in per_pass.items()}
         out["density_ms"] = round(per_pass["density"], 4)
-        # dominant kernel: the density pass is one kernel (k_density)
+        # dominant kernel: the density pass is one kernel
+        kname = (f"k_density_group<{int(prm.density_lanes)}, {int(prm.lane_split)}>" if prm.density_lanes > 0
+                 else "k_density")
         work = ctx.download("density_work")
         nr = ctx.download("nr_iters")
         out["nr_iterations_mean"] = round(float(nr.mean()), 4)
@@ -315,12 +317,15 @@ def main():
             "final_neighbours": round(float(work[:, 3].mean()), 1),
         }
         flops = density_flops(work)
-        traffic_bytes = profiled_traffic() if args.n == 1_000_000 and args.fixes == 0 else None
+        # the recorded counters cover the default workload and launch shape only
+        default_shape = (args.n == 1_000_000 and args.fixes == 0 and not args.param and not args.headon
+                         and prm.density_lanes == 8 and prm.lane_split == 0)
+        traffic_bytes = profiled_traffic() if default_shape else None
         t = per_pass["density"] / 1000.0
         achieved = flops / t / 1e12
         bytes_alg = DENSITY_BYTES_PER_PARTICLE * args.n
         out["roofline"] = {
-            "kernel": "k_density",
+            "kernel": kname,
             "bound": "mfma",
             "compute_unit": "fp32 VALU (no MFMA in this kernel; peak = MI355X FP32 157.3 TFLOP/s)",
             "achieved": round(achieved, 4),

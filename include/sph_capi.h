@@ -47,7 +47,11 @@ extern "C" {
  * leaf opens rejected nodes recursively and sums rejected source leaves
  * particle by particle (P2P), as a standard FMM near field does. */
 #define SPH_FIX_NEAR_FIELD    (1u << 4)
-#define SPH_FIX_ALL           (0x1Fu)
+/* Expansion order 3 (helpers.metal:76-88, accept.metal:47): power[3] with
+ * its 1/3 and 1/6 weights summed (as written they are integer divisions and
+ * three lines assign instead of add) and r^3 in the MAC (as written r^2). */
+#define SPH_FIX_ORDER3        (1u << 5)
+#define SPH_FIX_ALL           (0x3Fu)
 
 /* Particle activity rule used by the gravity up pass (P2M.metal:31). */
 #define SPH_ACTIVE_AS_WRITTEN 0 /* active iff globalTime <= nextActiveTime (every particle, every step) */
@@ -96,7 +100,19 @@ typedef struct sph_params {
     /* With lanes > 0: 0 = each lane takes whole cells of the box, 1 = the
      * lanes walk the box together and split each cell's candidates. */
     int32_t lane_split;
-    int32_t reserved[4];
+    /* Multipole / local expansion order P (Parameters.h `#define P 2`): 2 or 3.
+     * At 3 the reference's `#if P > 2` terms are added: octupoles in P2M/M2M,
+     * third derivatives in M2L, third-order locals in L2L/L2P. */
+    int32_t expansion_order;
+    /* Order of the error estimate in the acceptance test (accept.metal:14-55):
+     * 0 = the expansion order (the reference), 2 = the P = 2 estimate with
+     * P = 3 expansions.  The P = 3 estimate uses the octupole power alone at
+     * order 3, which vanishes for a source symmetric about its centre of mass
+     * (two equal masses, say); such a node is then accepted at any distance
+     * beyond the geometric limit although its neglected order-4 moment is
+     * not small. */
+    int32_t mac_order;
+    int32_t reserved[2];
 } sph_params;
 
 /* One EOS table: res x res float2 (pressure, sound speed) in code units.

@@ -104,6 +104,19 @@ def lib() -> ctypes.CDLL:
         L.orc_pole_shift_multipole.restype = None
         L.orc_pole_derivatives.argtypes = [_P(ctypes.c_float), ctypes.c_float, _P(ctypes.c_float)]
         L.orc_pole_derivatives.restype = None
+        L.orc_pole3_m2l.argtypes = [_P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_float]
+        L.orc_pole3_m2l.restype = None
+        L.orc_pole3_l2l.argtypes = [_P(ctypes.c_float)] * 3
+        L.orc_pole3_l2l.restype = None
+        L.orc_pole3_l2p.argtypes = [_P(ctypes.c_float)] * 3
+        L.orc_pole3_l2p.restype = None
+        L.orc_pole3_mac.argtypes = [_P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_float, ctypes.c_int]
+        L.orc_pole3_mac.restype = ctypes.c_int
+        L.orc_pole3_shift_multipole.argtypes = [_P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                                _P(ctypes.c_float)]
+        L.orc_pole3_shift_multipole.restype = None
+        L.orc_pole3_derivatives.argtypes = [_P(ctypes.c_float), ctypes.c_float, _P(ctypes.c_float)]
+        L.orc_pole3_derivatives.restype = None
     return _lib
 
 
@@ -151,6 +164,40 @@ def pole_shift_multipole(mp24, r3, fix_zz):
 def pole_derivatives(r3, eps):
     out = np.zeros(10, np.float32)
     lib().orc_pole_derivatives(_p(f32(r3)), eps, _p(out))
+    return out
+
+
+def pole3_m2l(local32, mp40, eps):
+    out = f32(local32).copy()
+    lib().orc_pole3_m2l(_p(out), _p(f32(mp40)), eps)
+    return out
+
+
+def pole3_l2l(local32, r3):
+    out = np.zeros(32, np.float32)
+    lib().orc_pole3_l2l(_p(f32(local32)), _p(f32(r3)), _p(out))
+    return out
+
+
+def pole3_l2p(local32, x3):
+    out = np.zeros(3, np.float32)
+    lib().orc_pole3_l2p(_p(f32(local32)), _p(f32(x3)), _p(out))
+    return out
+
+
+def pole3_mac(A40, B40, eta, fix_p3=False):
+    return bool(lib().orc_pole3_mac(_p(f32(A40)), _p(f32(B40)), eta, 1 if fix_p3 else 0))
+
+
+def pole3_shift_multipole(mp40, r3, fix_zz, fix_p3=False):
+    out = np.zeros(40, np.float32)
+    lib().orc_pole3_shift_multipole(_p(f32(mp40)), _p(f32(r3)), 1 if fix_zz else 0, 1 if fix_p3 else 0, _p(out))
+    return out
+
+
+def pole3_derivatives(r3, eps):
+    out = np.zeros(20, np.float32)
+    lib().orc_pole3_derivatives(_p(f32(r3)), eps, _p(out))
     return out
 
 
@@ -283,7 +330,7 @@ class State:
         max_nonlast = int(counts[:-1].max()) if counts.size > 1 else 1
         ll = max(max_nonlast, 1) * self.params.max_unchecked_pointers
         self.grav = {
-            "multipoles": np.zeros((K, 24), np.float32), "locals": np.zeros((K, 16), np.float32),
+            "multipoles": np.zeros((K, 40), np.float32), "locals": np.zeros((K, 32), np.float32),
             "parent_index": np.zeros(K, np.uint32), "list_a": np.zeros(ll, np.int32),
             "list_b": np.zeros(ll, np.int32), "list_len": ll,
         }

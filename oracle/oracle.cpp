@@ -178,22 +178,30 @@ inline const float* eos_table_for(const orc_state* s, int mat) {
 }
 
 /* --------------------------------------------------------------- gravity */
-enum { M = 0, X = 1, Y = 2, Z = 3, XX = 4, XY = 5, XZ = 6, YY = 7, YZ = 8, ZZ = 9, NT = 10 };
+/* poles.h:18-39 index map; third-order terms exist at P = 3 */
+enum {
+    M = 0, X = 1, Y = 2, Z = 3, XX = 4, XY = 5, XZ = 6, YY = 7, YZ = 8, ZZ = 9, NT = 10,
+    XXX = 10, XXY = 11, XXZ = 12, XYY = 13, XYZ = 14, XZZ = 15, YYY = 16, YYZ = 17, YZZ = 18, ZZZ = 19, NT3 = 20
+};
+/* record strides of the oracle's buffers (the first 24 / 16 floats are the
+ * sph_capi.h P = 2 layouts; P = 3 appends the third-order terms) */
+constexpr int MP_REC = 40;
+constexpr int LC_REC = 32;
 
 struct Multipole { /* poles.h:46-55 */
     f3 pos, mn, mx;
     float size;
-    float e[NT];
-    float power[3];
+    float e[NT3];
+    float power[4];
     float minGrav;
     bool active;
 };
 struct Local { /* poles.h:58-61 */
     f3 pos;
-    float e[NT];
+    float e[NT3];
 };
 
-Multipole load_mp(const float* rec) {
+Multipole load_mp(const float* rec, int order = 2) {
     Multipole m;
     m.pos = mk(rec[0], rec[1], rec[2]);
     m.size = rec[3];
@@ -202,12 +210,14 @@ Multipole load_mp(const float* rec) {
     m.mx = mk(rec[8], rec[9], rec[10]);
     m.active = rec[11] != 0.f;
     for (int i = 0; i < NT; ++i) m.e[i] = rec[12 + i];
+    for (int i = NT; i < NT3; ++i) m.e[i] = order > 2 ? rec[24 + i - NT] : 0.f;
     m.power[0] = rec[12];
     m.power[1] = rec[23];
     m.power[2] = rec[22];
+    m.power[3] = order > 2 ? rec[34] : 0.f;
     return m;
 }
-void store_mp(float* rec, const Multipole& m) {
+void store_mp(float* rec, const Multipole& m, int order = 2) {
     rec[0] = m.pos.x; rec[1] = m.pos.y; rec[2] = m.pos.z; rec[3] = m.size;
     rec[4] = m.mn.x; rec[5] = m.mn.y; rec[6] = m.mn.z; rec[7] = m.minGrav;
     rec[8] = m.mx.x; rec[9] = m.mx.y; rec[10] = m.mx.z; rec[11] = m.active ? 1.f : 0.f;
@@ -215,22 +225,29 @@ void store_mp(float* rec, const Multipole& m) {
     rec[22] = m.power[2];
     rec[23] = m.power[1];
     /* power[0] is stored as e[M] (addPowers sets power[0] = expansion[0]) */
+    if (order > 2) {
+        for (int i = NT; i < NT3; ++i) rec[24 + i - NT] = m.e[i];
+        rec[34] = m.power[3];
+    }
 }
-Local load_lc(const float* rec) {
+Local load_lc(const float* rec, int order = 2) {
     Local l;
     l.pos = mk(rec[0], rec[1], rec[2]);
     for (int i = 0; i < NT; ++i) l.e[i] = rec[4 + i];
+    for (int i = NT; i < NT3; ++i) l.e[i] = order > 2 ? rec[16 + i - NT] : 0.f;
     return l;
 }
-void store_lc(float* rec, const Local& l) {
+void store_lc(float* rec, const Local& l, int order = 2) {
     rec[0] = l.pos.x; rec[1] = l.pos.y; rec[2] = l.pos.z; rec[3] = 0.f;
     for (int i = 0; i < NT; ++i) rec[4 + i] = l.e[i];
     rec[14] = 0.f; rec[15] = 0.f;
+    if (order > 2)
+        for (int i = NT; i < NT3; ++i) rec[16 + i - NT] = l.e[i];
 }
 
-/* helpers.metal:12-27 (rows used at P = 2) */
+/* helpers.metal:12-27 (rows used at P <= 3) */
 inline int binomial(int n, int k) {
-    static const int c[3][3] = {{1, 0, 0}, {1, 1, 0}, {1, 2, 1}};
+    static const int c[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
     return c[n][k];
 }
 /* helpers.metal:30-61 */
@@ -242,8 +259,10 @@ inline float integer_powf(float x, unsigned n) {
         default: return powf(x, (float)n);
     }
 }
-/* helpers.metal:63-89 (P = 2) */
-inline void add_powers(Multipole& mp) {
+/* helpers.metal:63-89.  At P = 3, as written, the weights (1 / 3) and
+ * (1 / 6) are integer divisions (0) and three lines assign instead of add,
+ * so power[3] = |XXX| (helpers.metal:76-88); `fix_p3` sums the norm. */
+inline void add_powers(Multipole& mp, int order = 2, bool fix_p3 = false) {
     mp.power[0] = mp.e[0];
     mp.power[1] = 0;
     float p2 = mp.e[ZZ] * mp.e[ZZ];
@@ -253,17 +272,31 @@ inline void add_powers(Multipole& mp) {
     p2 += 0.5f * mp.e[XY] * mp.e[XY];
     p2 += 0.5f * mp.e[XX] * mp.e[XX];
     mp.power[2] = sqrtf(p2);
+    mp.power[3] = 0.f;
+    if (order < 3) return;
+    const float third = fix_p3 ? 1.f / 3.f : 0.f, sixth = fix_p3 ? 1.f / 6.f : 0.f;
+    float p3 = mp.e[ZZZ] * mp.e[ZZZ];
+    p3 += third * mp.e[YZZ] * mp.e[YZZ];
+    p3 += third * mp.e[YYZ] * mp.e[YYZ];
+    p3 = fix_p3 ? p3 + mp.e[YYY] * mp.e[YYY] : mp.e[YYY] * mp.e[YYY];
+    p3 += third * mp.e[XZZ] * mp.e[XZZ];
+    p3 += sixth * mp.e[XYZ] * mp.e[XYZ];
+    p3 += third * mp.e[XYY] * mp.e[XYY];
+    p3 += third * mp.e[XXZ] * mp.e[XXZ];
+    p3 += third * mp.e[XXY] * mp.e[XXY];
+    p3 = fix_p3 ? p3 + mp.e[XXX] * mp.e[XXX] : mp.e[XXX] * mp.e[XXX];
+    mp.power[3] = sqrtf(p3);
 }
 
-/* accept.metal:14-55 */
-inline bool m2l_accept(const Multipole& A, const Multipole& B, float eta) {
+/* accept.metal:14-55.  r^p is integer_powf(r2, p / 2): at P = 3, as
+ * written, r^2 instead of r^3 (`fix_p3`: r^3). */
+inline bool m2l_accept(const Multipole& A, const Multipole& B, float eta, int p = 2, bool fix_p3 = false) {
     const float rho_A = A.size;
     const float rho_B = B.size;
     const float r2 = len2(A.pos - B.pos);
     const float rho_sum = rho_A + rho_B;
     if (rho_sum * rho_sum > r2) return false;
     const float rho_max = std::max(rho_A, rho_B);
-    const int p = 2;
     float E_BA_term = 0.f;
     for (int n = 0; n <= p; ++n) E_BA_term += binomial(p, n) * B.power[n] * integer_powf(rho_A, p - n);
     E_BA_term *= 8.f;
@@ -271,7 +304,7 @@ inline bool m2l_accept(const Multipole& A, const Multipole& B, float eta) {
         E_BA_term *= rho_max;
         E_BA_term /= (rho_A + rho_B);
     }
-    const float r_to_p = integer_powf(r2, (p / 2));
+    const float r_to_p = (fix_p3 && p == 3) ? r2 * sqrtf(r2) : integer_powf(r2, (p / 2));
     float f_MAC_inv = r2;
     const float min_a_grav = A.minGrav;
     return E_BA_term < eta * min_a_grav * r_to_p * f_MAC_inv;
@@ -295,9 +328,14 @@ inline float D_soft_3(float u) {
     phi = phi * u - 420.f; phi = phi * u + 168.f; phi = phi * u; phi = phi * u;
     return phi;
 }
-/* derivatives.metal:59-156 (P = 2) */
-inline void derivatives(f3 vec, float eps, float* d) {
-    float Dt_1, Dt_2, Dt_3;
+inline float D_soft_4(float u) {
+    float phi = -315.f * u + 720.f;
+    phi = phi * u - 420.f; phi = phi * u; phi = phi * u;
+    return phi;
+}
+/* derivatives.metal:59-156; `d` has NT3 entries (third order when order 3) */
+inline void derivatives(f3 vec, float eps, float* d, int order = 2) {
+    float Dt_1, Dt_2, Dt_3, Dt_4 = 0.f;
     float r2 = len2(vec);
     float r = sqrtf(r2);
     float r_inv = r > 1e-6f ? 1 / r : 0;
@@ -310,10 +348,13 @@ inline void derivatives(f3 vec, float eps, float* d) {
         Dt_2 = eps_inv2 * D_soft_2(u);
         const float eps_inv3 = eps_inv2 * eps_inv;
         Dt_3 = eps_inv3 * D_soft_3(u);
+        const float eps_inv4 = eps_inv3 * eps_inv;
+        Dt_4 = eps_inv4 * D_soft_4(u);
     } else {
         Dt_1 = r_inv;
         Dt_2 = -1.f * Dt_1 * r_inv;
         Dt_3 = -3.f * Dt_2 * r_inv;
+        Dt_4 = -5.f * Dt_3 * r_inv;
     }
     const float rx_r = vec.x * r_inv, ry_r = vec.y * r_inv, rz_r = vec.z * r_inv;
     const float rx_r2 = rx_r * rx_r, ry_r2 = ry_r * ry_r, rz_r2 = rz_r * rz_r;
@@ -328,13 +369,27 @@ inline void derivatives(f3 vec, float eps, float* d) {
     d[YY] = ry_r2 * Dt_3 + Dt_2;
     d[YZ] = ry_r * rz_r * Dt_3;
     d[ZZ] = rz_r2 * Dt_3 + Dt_2;
+    for (int i = NT; i < NT3; ++i) d[i] = 0.f;
+    if (order < 3) return;
+    Dt_3 *= r_inv;
+    const float rx_r3 = rx_r2 * rx_r, ry_r3 = ry_r2 * ry_r, rz_r3 = rz_r2 * rz_r;
+    d[XXX] = rx_r3 * Dt_4 + 3.f * rx_r * Dt_3;
+    d[YYY] = ry_r3 * Dt_4 + 3.f * ry_r * Dt_3;
+    d[ZZZ] = rz_r3 * Dt_4 + 3.f * rz_r * Dt_3;
+    d[XXY] = rx_r2 * ry_r * Dt_4 + ry_r * Dt_3;
+    d[XXZ] = rx_r2 * rz_r * Dt_4 + rz_r * Dt_3;
+    d[XYY] = ry_r2 * rx_r * Dt_4 + rx_r * Dt_3;
+    d[YYZ] = ry_r2 * rz_r * Dt_4 + rz_r * Dt_3;
+    d[XZZ] = rz_r2 * rx_r * Dt_4 + rx_r * Dt_3;
+    d[YZZ] = rz_r2 * ry_r * Dt_4 + ry_r * Dt_3;
+    d[XYZ] = rx_r * ry_r * rz_r * Dt_4;
 }
 
-/* M2L.metal:12-108 (P = 2) */
-inline Local M2L(f3 x, const Multipole& mp, float eps) {
+/* M2L.metal:12-108 */
+inline Local M2L(f3 x, const Multipole& mp, float eps, int order = 2) {
     f3 r = x - mp.pos;
-    float D[NT];
-    derivatives(r, eps, D);
+    float D[NT3];
+    derivatives(r, eps, D, order);
     Local l;
     const float M_000 = mp.e[M];
     l.e[M] = M_000 * D[M];
@@ -349,11 +404,26 @@ inline Local M2L(f3 x, const Multipole& mp, float eps) {
     l.e[XY] = M_000 * D[XY];
     l.e[XZ] = M_000 * D[XZ];
     l.e[YZ] = M_000 * D[YZ];
+    for (int i = NT; i < NT3; ++i) l.e[i] = 0.f;
+    if (order < 3) return l;
+    /* M2L.metal:58-107 */
+    const float* q = mp.e;
+    l.e[M] += q[XXX] * D[XXX] + q[YYY] * D[YYY] + q[ZZZ] * D[ZZZ];
+    l.e[M] += q[XXY] * D[XXY] + q[XXZ] * D[XXZ] + q[XYY] * D[XYY];
+    l.e[M] += q[YYZ] * D[YYZ] + q[XZZ] * D[XZZ] + q[YZZ] * D[YZZ];
+    l.e[M] += q[XYZ] * D[XYZ];
+    l.e[X] += q[XX] * D[XXX] + q[YY] * D[XYY] + q[ZZ] * D[XZZ];
+    l.e[X] += q[XY] * D[XXY] + q[XZ] * D[XXZ] + q[YZ] * D[XYZ];
+    l.e[Y] += q[XX] * D[XXY] + q[YY] * D[YYY] + q[ZZ] * D[YZZ];
+    l.e[Y] += q[XY] * D[XYY] + q[XZ] * D[XYZ] + q[YZ] * D[YYZ];
+    l.e[Z] += q[XX] * D[XXZ] + q[YY] * D[YYZ] + q[ZZ] * D[ZZZ];
+    l.e[Z] += q[XY] * D[XYZ] + q[XZ] * D[XZZ] + q[YZ] * D[YZZ];
+    for (int i = NT; i < NT3; ++i) l.e[i] = M_000 * D[i];
     return l;
 }
 
-/* L2L.metal:12-88 (P = 2) */
-inline Local transform_local(const Local& L, f3 r) {
+/* L2L.metal:12-88 */
+inline Local transform_local(const Local& L, f3 r, int order = 2) {
     Local n;
     n.pos = L.pos + r;
     n.e[M] = L.e[M];
@@ -367,23 +437,53 @@ inline Local transform_local(const Local& L, f3 r) {
     n.e[Y] += r.x * L.e[XY] + r.y * L.e[YY] + r.z * L.e[YZ];
     n.e[Z] += r.x * L.e[XZ] + r.y * L.e[YZ] + r.z * L.e[ZZ];
     for (int i = XX; i <= ZZ; ++i) n.e[i] = L.e[i];
+    for (int i = NT; i < NT3; ++i) n.e[i] = 0.f;
+    if (order < 3) return n;
+    /* L2L.metal:37-86 */
+    const float* e = L.e;
+    n.e[M] += 0.166666667f * r.x * r.x * r.x * e[XXX] + 0.166666667f * r.y * r.y * r.y * e[YYY] +
+              0.166666667f * r.z * r.z * r.z * e[ZZZ];
+    n.e[M] += 0.5f * r.x * r.x * r.y * e[XXY] + 0.5f * r.x * r.x * r.z * e[XXZ] + 0.5f * r.x * r.y * r.y * e[XYY];
+    n.e[M] += 0.5f * r.y * r.y * r.z * e[YYZ] + 0.5f * r.x * r.z * r.z * e[XZZ] + 0.5f * r.y * r.z * r.z * e[YZZ];
+    n.e[M] += r.x * r.y * r.z * e[XYZ];
+    n.e[X] += 0.5f * r.x * r.x * e[XXX] + 0.5f * r.y * r.y * e[XYY] + 0.5f * r.z * r.z * e[XZZ];
+    n.e[X] += r.x * r.y * e[XXY] + r.x * r.z * e[XXZ] + r.y * r.z * e[XYZ];
+    n.e[Y] += 0.5f * r.x * r.x * e[XXY] + 0.5f * r.y * r.y * e[YYY] + 0.5f * r.z * r.z * e[YZZ];
+    n.e[Y] += r.x * r.y * e[XYY] + r.x * r.z * e[XYZ] + r.y * r.z * e[YYZ];
+    n.e[Z] += 0.5f * r.x * r.x * e[XXZ] + 0.5f * r.y * r.y * e[YYZ] + 0.5f * r.z * r.z * e[ZZZ];
+    n.e[Z] += r.x * r.y * e[XYZ] + r.x * r.z * e[XZZ] + r.y * r.z * e[YZZ];
+    n.e[XX] += r.x * e[XXX] + r.y * e[XXY] + r.z * e[XXZ];
+    n.e[YY] += r.x * e[XYY] + r.y * e[YYY] + r.z * e[YYZ];
+    n.e[ZZ] += r.x * e[XZZ] + r.y * e[YZZ] + r.z * e[ZZZ];
+    n.e[XY] += r.x * e[XXY] + r.y * e[XYY] + r.z * e[XYZ];
+    n.e[XZ] += r.x * e[XXZ] + r.y * e[XYZ] + r.z * e[XZZ];
+    n.e[YZ] += r.x * e[XYZ] + r.y * e[YYZ] + r.z * e[YZZ];
+    for (int i = NT; i < NT3; ++i) n.e[i] = e[i];
     return n;
 }
 
-/* L2P.metal:12-32 (P = 2) */
-inline f3 L2P(const Local& L, f3 pos) {
+/* L2P.metal:12-32 */
+inline f3 L2P(const Local& L, f3 pos, int order = 2) {
     f3 acc = mk(0, 0, 0);
     f3 r = pos - L.pos;
     acc += mk(L.e[X], L.e[Y], L.e[Z]);
     acc.x += r.x * L.e[XX] + r.y * L.e[XY] + r.z * L.e[XZ];
     acc.y += r.x * L.e[XY] + r.y * L.e[YY] + r.z * L.e[YZ];
     acc.z += r.x * L.e[XZ] + r.y * L.e[YZ] + r.z * L.e[ZZ];
+    if (order < 3) return acc;
+    const float* e = L.e;
+    acc.x += 0.5f * r.x * r.x * e[XXX] + 0.5f * r.y * r.y * e[XYY] + 0.5f * r.z * r.z * e[XZZ];
+    acc.x += r.x * r.y * e[XXY] + r.x * r.z * e[XXZ] + r.y * r.z * e[XYZ];
+    acc.y += 0.5f * r.x * r.x * e[XXY] + 0.5f * r.y * r.y * e[YYY] + 0.5f * r.z * r.z * e[YZZ];
+    acc.y += r.x * r.y * e[XYY] + r.x * r.z * e[XYZ] + r.y * r.z * e[YYZ];
+    acc.z += 0.5f * r.x * r.x * e[XXZ] + 0.5f * r.y * r.y * e[YYZ] + 0.5f * r.z * r.z * e[ZZZ];
+    acc.z += r.x * r.y * e[XYZ] + r.x * r.z * e[XZZ] + r.y * r.z * e[YZZ];
     return acc;
 }
 
-/* M2M.metal:12-73 (P = 2).  As written the ZZ shift multiplies by the
- * dipole expansion[Z] instead of the mass (M2M.metal:29). */
-inline Multipole transform_multipole(const Multipole& mp, f3 r, bool fix_zz) {
+/* M2M.metal:12-73.  As written the ZZ shift multiplies by the dipole
+ * expansion[Z] instead of the mass (M2M.metal:29). */
+inline Multipole transform_multipole(const Multipole& mp, f3 r, bool fix_zz, int order = 2) {
     Multipole n = mp;
     n.pos = mp.pos + r;
     n.e[M] = mp.e[M];
@@ -396,6 +496,21 @@ inline Multipole transform_multipole(const Multipole& mp, f3 r, bool fix_zz) {
     n.e[YY] = mp.e[YY] + 0.5f * r.y * r.y * mp.e[M];
     n.e[YZ] = mp.e[YZ] + r.y * r.z * mp.e[M];
     n.e[ZZ] = mp.e[ZZ] + 0.5f * r.z * r.z * (fix_zz ? mp.e[M] : mp.e[Z]);
+    for (int i = NT; i < NT3; ++i) n.e[i] = 0.f;
+    if (order < 3) return n;
+    /* M2M.metal:31-71 */
+    const float* e = mp.e;
+    const float m0 = e[M];
+    n.e[ZZZ] = e[ZZZ] + r.z * e[ZZ] + 0.166666667f * r.z * r.z * r.z * m0;
+    n.e[YZZ] = e[YZZ] + r.z * e[YZ] + r.y * e[ZZ] + 0.5f * r.y * r.z * r.z * m0;
+    n.e[YYZ] = e[YYZ] + r.z * e[YY] + r.y * e[YZ] + 0.5f * r.y * r.y * r.z * m0;
+    n.e[YYY] = e[YYY] + r.y * e[YY] + 0.166666667f * r.y * r.y * r.y * m0;
+    n.e[XZZ] = e[XZZ] + r.z * e[XZ] + r.x * e[ZZ] + 0.5f * r.x * r.z * r.z * m0;
+    n.e[XYZ] = e[XYZ] + r.z * e[XY] + r.y * e[XZ] + r.x * e[YZ] + r.x * r.y * r.z * m0;
+    n.e[XYY] = e[XYY] + r.y * e[XY] + r.x * e[YY] + 0.5f * r.x * r.y * r.y * m0;
+    n.e[XXZ] = e[XXZ] + r.z * e[XX] + r.x * e[XZ] + 0.5f * r.x * r.x * r.z * m0;
+    n.e[XXY] = e[XXY] + r.y * e[XX] + r.x * e[XY] + 0.5f * r.x * r.x * r.y * m0;
+    n.e[XXX] = e[XXX] + r.x * e[XX] + 0.166666667f * r.x * r.x * r.x * m0;
     return n;
 }
 
@@ -404,7 +519,14 @@ struct GravCtx {
     const orc_state* s;
     const orc_tree* t;
     orc_gravity_buffers* g;
-    Multipole mp_at(int dataPointer) const { return load_mp(g->multipoles + 24 * (int64_t)dataPointer); }
+    int order;      /* expansion order P (Parameters.h) */
+    int mac_order;  /* order of the acceptance test's error estimate */
+    bool fix_p3;    /* SPH_FIX_ORDER3 */
+    GravCtx(const sph_params* p, const orc_state* st, const orc_tree* tr, orc_gravity_buffers* gb)
+        : prm(p), s(st), t(tr), g(gb), order(p->expansion_order == 3 ? 3 : 2),
+          mac_order(p->mac_order == 2 ? 2 : (p->expansion_order == 3 ? 3 : 2)),
+          fix_p3((p->fixes & SPH_FIX_ORDER3) != 0) {}
+    Multipole mp_at(int dataPointer) const { return load_mp(g->multipoles + MP_REC * (int64_t)dataPointer, order); }
 };
 
 /* P2M.metal:12-107 */
@@ -416,7 +538,7 @@ Multipole P2M(const GravCtx& c, int treePointer) {
     int start = treePointer + 2;
     int end = start + nParticles;
     mp.pos = mk(0, 0, 0);
-    for (int i = 0; i < NT; ++i) mp.e[i] = 0;
+    for (int i = 0; i < NT3; ++i) mp.e[i] = 0;
     mp.minGrav = kMaxFloat;
     mp.active = false;
     mp.mx = mk(-kMaxFloat, -kMaxFloat, -kMaxFloat);
@@ -449,13 +571,36 @@ Multipole P2M(const GravCtx& c, int treePointer) {
         mp.e[YY] += p_mass * r.y * r.y;
         mp.e[YZ] += p_mass * r.y * r.z;
         mp.e[ZZ] += p_mass * r.z * r.z;
+        if (c.order > 2) { /* P2M.metal:70-81 */
+            mp.e[XXX] -= p_mass * r.x * r.x * r.x;
+            mp.e[XXY] -= p_mass * r.x * r.x * r.y;
+            mp.e[XXZ] -= p_mass * r.x * r.x * r.z;
+            mp.e[XYY] -= p_mass * r.x * r.y * r.y;
+            mp.e[XYZ] -= p_mass * r.x * r.y * r.z;
+            mp.e[XZZ] -= p_mass * r.x * r.z * r.z;
+            mp.e[YYY] -= p_mass * r.y * r.y * r.y;
+            mp.e[YYZ] -= p_mass * r.y * r.y * r.z;
+            mp.e[YZZ] -= p_mass * r.y * r.z * r.z;
+            mp.e[ZZZ] -= p_mass * r.z * r.z * r.z;
+        }
     }
     mp.e[XX] *= 0.5f;
     mp.e[YY] *= 0.5f;
     mp.e[ZZ] *= 0.5f;
+    if (c.order > 2) { /* P2M.metal:89-99 */
+        mp.e[XXX] *= 0.166666667f;
+        mp.e[YYY] *= 0.166666667f;
+        mp.e[ZZZ] *= 0.166666667f;
+        mp.e[XXY] *= 0.5f;
+        mp.e[XXZ] *= 0.5f;
+        mp.e[XYY] *= 0.5f;
+        mp.e[XZZ] *= 0.5f;
+        mp.e[YYZ] *= 0.5f;
+        mp.e[YZZ] *= 0.5f;
+    }
     f3 dims = mp.mx - mp.mn;
     mp.size = std::max(std::max(dims.x, dims.y), dims.z);
-    add_powers(mp);
+    add_powers(mp, c.order, c.fix_p3);
     return mp;
 }
 
@@ -468,7 +613,7 @@ Multipole M2M(const GravCtx& c, uint32_t index, int treePointer) {
     mp.minGrav = kMaxFloat;
     mp.pos = mk(0, 0, 0);
     mp.active = false;
-    for (int i = 0; i < NT; ++i) mp.e[i] = 0;
+    for (int i = 0; i < NT3; ++i) mp.e[i] = 0;
     float mass = 0;
     mp.mx = mk(-kMaxFloat, -kMaxFloat, -kMaxFloat);
     mp.mn = mk(kMaxFloat, kMaxFloat, kMaxFloat);
@@ -498,17 +643,17 @@ Multipole M2M(const GravCtx& c, uint32_t index, int treePointer) {
         int childDataPointer = tree[childPointer + 1];
         Multipole childMp = c.mp_at(childDataPointer);
         f3 r = mp.pos - childMp.pos;
-        Multipole tr = transform_multipole(childMp, r, fix_zz);
-        for (int j = 0; j < NT; ++j) mp.e[j] += tr.e[j];
+        Multipole tr = transform_multipole(childMp, r, fix_zz, c.order);
+        for (int j = 0; j < NT3; ++j) mp.e[j] += tr.e[j];
         if (allLeaves && mp.active && !childMp.active) {
-            c.g->multipoles[24 * (int64_t)childDataPointer + 11] = 1.f;
+            c.g->multipoles[MP_REC * (int64_t)childDataPointer + 11] = 1.f;
             int nParticles = tree[childPointer];
             int childStart = childPointer + 2;
             int childEnd = (fix_range ? childStart : start) + nParticles;
             for (int j = childStart; j < childEnd; j++) c.s->active[tree[j]] = true;
         }
     }
-    add_powers(mp);
+    add_powers(mp, c.order, c.fix_p3);
     return mp;
 }
 
@@ -516,9 +661,9 @@ inline bool check_and_add(const GravCtx& c, const Multipole& mp, Local& local, i
     /* helpers.h:34-45 */
     int nodeDataPointer = c.t->octree[toScan + 1];
     Multipole nodeMp = c.mp_at(nodeDataPointer);
-    if (forceAccept || m2l_accept(mp, nodeMp, c.prm->gravity_eta)) {
-        Local nl = M2L(local.pos, nodeMp, c.prm->gravity_softening);
-        for (int j = 0; j < NT; ++j) local.e[j] += nl.e[j];
+    if (forceAccept || m2l_accept(mp, nodeMp, c.prm->gravity_eta, c.mac_order, c.fix_p3)) {
+        Local nl = M2L(local.pos, nodeMp, c.prm->gravity_softening, c.order);
+        for (int j = 0; j < NT3; ++j) local.e[j] += nl.e[j];
         return true;
     }
     return false;
@@ -629,7 +774,7 @@ void local_to_leaf(const GravCtx& c, const Local& local, int treePointer, int nP
         int pp = tree[i];
         if (!s->active[pp]) continue;
         f3 x_i = ld3(s->pos, pp);
-        f3 a = L2P(local, x_i);
+        f3 a = L2P(local, x_i, c.order);
         for (int j = start; j < end; j++) {
             if (i == j) continue;
             int p_j = tree[j];
@@ -668,7 +813,7 @@ void L2L(const GravCtx& c, const Local& local, int treePointer) {
         Multipole nodeMp = c.mp_at(ndp);
         if (!nodeMp.active) continue;
         f3 r = nodeMp.pos - local.pos;
-        store_lc(c.g->locals + 16 * (int64_t)ndp, transform_local(local, r));
+        store_lc(c.g->locals + LC_REC * (int64_t)ndp, transform_local(local, r, c.order), c.order);
     }
 }
 
@@ -681,8 +826,8 @@ void zero_first_branches(const GravCtx& c, int32_t* out) {
         int ndp = tree[nodePointer + 1];
         Local l;
         l.pos = c.mp_at(ndp).pos;
-        for (int k = 0; k < NT; ++k) l.e[k] = 0;
-        store_lc(c.g->locals + 16 * (int64_t)ndp, l);
+        for (int k = 0; k < NT3; ++k) l.e[k] = 0;
+        store_lc(c.g->locals + LC_REC * (int64_t)ndp, l, c.order);
     }
     out[0] = 0;
     out[1] = -1;
@@ -709,7 +854,7 @@ void down_pass_node(const GravCtx& c, uint32_t index, int treePointer, const int
     uint32_t parentIndex = c.g->parent_index[dataPointer];
     uint32_t startScan = parentIndex * MAXU;
     uint32_t endScan = (parentIndex + 1) * MAXU;
-    Local local = load_lc(c.g->locals + 16 * (int64_t)dataPointer);
+    Local local = load_lc(c.g->locals + LC_REC * (int64_t)dataPointer, c.order);
     const bool near = c.prm->fixes & SPH_FIX_NEAR_FIELD;
     std::vector<int> p2p_leaves;
     for (uint32_t i = startScan; i < endScan; i++) {
@@ -1197,7 +1342,7 @@ void orc_gravity_up(const sph_params* prm, const orc_state* s, const orc_tree* t
             int nParticles = t->octree[treePointer];
             int dataPointer = t->octree[treePointer + 1];
             Multipole mp = nParticles == 0 ? M2M(c, (uint32_t)index, treePointer) : P2M(c, treePointer);
-            store_mp(g->multipoles + 24 * (int64_t)dataPointer, mp);
+            store_mp(g->multipoles + MP_REC * (int64_t)dataPointer, mp, c.order);
         }
     }
 }
@@ -1245,7 +1390,40 @@ void orc_pole_shift_multipole(const float* mp24, const float* r3, int fix_zz, fl
 }
 
 void orc_pole_derivatives(const float* r3, float eps, float* d10) {
-    derivatives(mk(r3[0], r3[1], r3[2]), eps, d10);
+    float d[NT3];
+    derivatives(mk(r3[0], r3[1], r3[2]), eps, d);
+    for (int i = 0; i < NT; ++i) d10[i] = d[i];
+}
+
+/* P = 3 records (40-float multipoles, 32-float locals) */
+void orc_pole3_m2l(float* local32, const float* mp40, float eps) {
+    Local l = load_lc(local32, 3);
+    Local add = M2L(l.pos, load_mp(mp40, 3), eps, 3);
+    for (int j = 0; j < NT3; ++j) l.e[j] += add.e[j];
+    store_lc(local32, l, 3);
+}
+
+void orc_pole3_l2l(const float* local32, const float* r3, float* out32) {
+    store_lc(out32, transform_local(load_lc(local32, 3), mk(r3[0], r3[1], r3[2]), 3), 3);
+}
+
+void orc_pole3_l2p(const float* local32, const float* x3, float* acc3) {
+    f3 a = L2P(load_lc(local32, 3), mk(x3[0], x3[1], x3[2]), 3);
+    acc3[0] = a.x; acc3[1] = a.y; acc3[2] = a.z;
+}
+
+int orc_pole3_mac(const float* A40, const float* B40, float eta, int fix_p3) {
+    return m2l_accept(load_mp(A40, 3), load_mp(B40, 3), eta, 3, fix_p3 != 0) ? 1 : 0;
+}
+
+void orc_pole3_shift_multipole(const float* mp40, const float* r3, int fix_zz, int fix_p3, float* out40) {
+    Multipole m = transform_multipole(load_mp(mp40, 3), mk(r3[0], r3[1], r3[2]), fix_zz != 0, 3);
+    add_powers(m, 3, fix_p3 != 0);
+    store_mp(out40, m, 3);
+}
+
+void orc_pole3_derivatives(const float* r3, float eps, float* d20) {
+    derivatives(mk(r3[0], r3[1], r3[2]), eps, d20, 3);
 }
 
 void orc_direct_sum_fp64(const float* pos4, const float* m, int32_t n, const int32_t* targets, int32_t n_targets,

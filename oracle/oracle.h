@@ -92,10 +92,12 @@ int orc_build_octree(const float* pos4, int32_t n, int32_t max_leaf, int32_t max
 void orc_free_tree(orc_tree* t);
 
 /* ---- gravity/** ---- */
-/* Records use the sph_capi.h SPH_F_MULTIPOLES / SPH_F_LOCALS layouts. */
+/* Records: 40 floats per multipole and 32 per local; the first 24 / 16 are
+ * the sph_capi.h SPH_F_MULTIPOLES / SPH_F_LOCALS layouts, the rest hold the
+ * third-order terms when sph_params.expansion_order is 3. */
 typedef struct orc_gravity_buffers {
-    float* multipoles;       /* 24 x node_count */
-    float* locals;           /* 16 x node_count */
+    float* multipoles;       /* 40 x node_count */
+    float* locals;           /* 32 x node_count */
     uint32_t* parent_index;  /* node_count */
     int32_t* list_a;         /* max_level_nodes * max_unchecked_pointers */
     int32_t* list_b;
@@ -115,6 +117,17 @@ void orc_pole_l2p(const float* local16, const float* x3, float* acc3);
 int orc_pole_mac(const float* A24, const float* B24, float eta);
 void orc_pole_shift_multipole(const float* mp24, const float* r3, int fix_zz, float* out24);
 void orc_pole_derivatives(const float* r3, float eps, float* d10);
+/* The same at P = 3 (the reference's `#if P > 2` branches): 40-float
+ * multipole records (24-float layout + octupole XXX XXY XXZ XYY XYZ XZZ YYY
+ * YYZ YZZ ZZZ at 24-33, power[3] at 34) and 32-float local records (16-float
+ * layout + third-order terms at 16-25, same order); derivatives to third
+ * order (20 floats, poles.h:18-39 index map). */
+void orc_pole3_m2l(float* local32, const float* mp40, float eps);
+void orc_pole3_l2l(const float* local32, const float* r3, float* out32);
+void orc_pole3_l2p(const float* local32, const float* x3, float* acc3);
+int orc_pole3_mac(const float* A40, const float* B40, float eta, int fix_p3);
+void orc_pole3_shift_multipole(const float* mp40, const float* r3, int fix_zz, int fix_p3, float* out40);
+void orc_pole3_derivatives(const float* r3, float eps, float* d20);
 
 /* FP64 direct sum over all n sources at the sampled targets: acceleration
  * (G included) and potential.  `softening` > 0 uses the cubic-spline softened

@@ -27,7 +27,8 @@ SPH_FIX_SYNC_RANGE = 1 << 1
 SPH_FIX_SOFTENING = 1 << 2
 SPH_FIX_DENSITY_CACHE = 1 << 3
 SPH_FIX_NEAR_FIELD = 1 << 4
-SPH_FIX_ALL = 0x1F
+SPH_FIX_ORDER3 = 32
+SPH_FIX_ALL = 0x3F
 SPH_ACTIVE_AS_WRITTEN = 0
 SPH_ACTIVE_README = 1
 TREE_ASYNC, TREE_EVERY, TREE_FIXED = 0, 1, 2
@@ -59,7 +60,8 @@ class SphParams(ctypes.Structure):
         ("max_children_in_leaf", ctypes.c_int32), ("max_tree_depth", ctypes.c_int32),
         ("cell_width", ctypes.c_float), ("cell_power", ctypes.c_int32),
         ("fixes", ctypes.c_uint32), ("precision", ctypes.c_int32), ("density_lanes", ctypes.c_int32),
-        ("accel_lanes", ctypes.c_int32), ("lane_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
+        ("accel_lanes", ctypes.c_int32), ("lane_split", ctypes.c_int32),
+        ("expansion_order", ctypes.c_int32), ("mac_order", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -360,10 +362,10 @@ class Context:
     def _shape_dtype(self, field: str, nbytes: int):
         if field in _F4:
             return (self.n, 4), np.float32
-        if field == "multipoles":
-            return (nbytes // 96, 24), np.float32
-        if field == "locals":
-            return (nbytes // 64, 16), np.float32
+        if field in ("multipoles", "locals"):
+            # records of 24 / 16 floats, 40 / 32 with third-order expansions
+            w = {"multipoles": 24, "locals": 16}[field] + (16 if self.params.expansion_order == 3 else 0)
+            return (nbytes // (4 * w), w), np.float32
         if field == "cell_array":
             return (self.n, 2), np.uint32
         if field == "density_work":

@@ -44,6 +44,12 @@ bool valid_params(const sph_params* p, std::string& why) {
     if ((p->density_lanes == 2 || p->density_lanes == 4 || p->accel_lanes == 2 || p->accel_lanes == 4) &&
         p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }
+    if (p->expansion_order != 2 && p->expansion_order != 3) { why = "expansion_order must be 2 or 3"; return false; }
+    if (p->mac_order != 0 && p->mac_order != 2) { why = "mac_order must be 0 or 2"; return false; }
+    if (p->expansion_order == 3 && p->gravity_max_recursion != 1 && !(p->fixes & SPH_FIX_NEAR_FIELD)) {
+        why = "expansion_order 3 needs gravity_max_recursion 1 or SPH_FIX_NEAR_FIELD";
+        return false;
+    }
     return true;
 }
 
@@ -65,6 +71,8 @@ void free_tree(sph_tree_dev& T) {
     dfree(T.node_child);
     dfree(T.mp_far);
     dfree(T.mp_aux);
+    dfree(T.mp_oct);
+    dfree(T.loc3);
     dfree(T.loc);
     T.octree_cap = T.level_cap = T.node_cap = 0;
 }
@@ -120,8 +128,10 @@ int64_t field_bytes(const sph_ctx* ctx, int field) {
         case SPH_F_CELL_ARRAY: return 8 * n;
         case SPH_F_CELL_START: case SPH_F_CELL_END: return 4 * (int64_t)ctx->ncells;
         case SPH_F_GLOBAL_TIME: case SPH_F_DT: return 4;
-        case SPH_F_MULTIPOLES: return ctx->have_tree ? 24 * 4 * (int64_t)ctx->tree.node_count : 0;
-        case SPH_F_LOCALS: return ctx->have_tree ? 16 * 4 * (int64_t)ctx->tree.node_count : 0;
+        case SPH_F_MULTIPOLES:
+            return ctx->have_tree ? (ctx->prm.expansion_order == 3 ? 40 : 24) * 4 * (int64_t)ctx->tree.node_count : 0;
+        case SPH_F_LOCALS:
+            return ctx->have_tree ? (ctx->prm.expansion_order == 3 ? 32 : 16) * 4 * (int64_t)ctx->tree.node_count : 0;
         case SPH_F_DENSITY_WORK: return 16 * n;
         default: return -1;
     }
@@ -404,6 +414,8 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const i
             dfree(T.node_child);
             dfree(T.mp_far);
             dfree(T.mp_aux);
+            dfree(T.mp_oct);
+            dfree(T.loc3);
             dfree(T.loc);
             SPH_HIP_TRY(dalloc(&T.node_info, (size_t)cap));
             SPH_HIP_TRY(dalloc(&T.parent_slot, (size_t)cap));
@@ -413,6 +425,10 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const i
             SPH_HIP_TRY(dalloc(&T.node_child, (size_t)cap * 8));
             SPH_HIP_TRY(dalloc(&T.mp_far, (size_t)cap * 3));
             SPH_HIP_TRY(dalloc(&T.mp_aux, (size_t)cap * 3));
+            if (ctx->prm.expansion_order == 3) {
+                SPH_HIP_TRY(dalloc(&T.mp_oct, (size_t)cap * 3));
+                SPH_HIP_TRY(dalloc(&T.loc3, (size_t)cap * 3));
+            }
             SPH_HIP_TRY(dalloc(&T.loc, (size_t)cap * 4));
             T.node_cap = cap;
         }
@@ -554,26 +570,44 @@ int sph_download(sph_ctx* ctx, int field, void* dst) {
         if (!ctx->have_tree) return ctx_fail(ctx, SPH_E_NOTREE, "no tree");
         const size_t K = (size_t)ctx->tree.node_count;
         float* o = (float*)dst;
+        const bool p3 = ctx->prm.expansion_order == 3;
+        std::vector<float4> third(p3 ? 3 * K : 0);
         if (field == SPH_F_MULTIPOLES) {
             std::vector<float4> far(3 * K), aux(3 * K);
             SPH_HIP_TRY(hipMemcpy(far.data(), ctx->tree.mp_far, 48 * K, hipMemcpyDeviceToHost));
             SPH_HIP_TRY(hipMemcpy(aux.data(), ctx->tree.mp_aux, 48 * K, hipMemcpyDeviceToHost));
+            if (p3) SPH_HIP_TRY(hipMemcpy(third.data(), ctx->tree.mp_oct, 48 * K, hipMemcpyDeviceToHost));
+            const size_t stride = p3 ? 40 : 24;
             for (size_t i = 0; i < K; ++i) {
                 const float4 f0 = far[3 * i], f1 = far[3 * i + 1], f2 = far[3 * i + 2];
                 const float4 x0 = aux[3 * i], x1 = aux[3 * i + 1], x2 = aux[3 * i + 2];
-                float* r = o + 24 * i;
+                float* r = o + stride * i;
                 const float rec[24] = {f0.x, f0.y, f0.z, f0.w, x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w,
                                        f1.x, x2.x, x2.y, x2.z, f1.y, f1.z, f1.w, f2.x, f2.y, f2.z, f2.w, 0.f};
                 std::memcpy(r, rec, sizeof(rec));
+                if (p3) {
+                    const float4 q0 = third[3 * i], q1 = third[3 * i + 1], q2 = third[3 * i + 2];
+                    const float ext[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                           q2.x, q2.y, q2.z, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    std::memcpy(r + 24, ext, sizeof(ext));
+                }
             }
         } else {
             std::vector<float4> loc(4 * K);
             SPH_HIP_TRY(hipMemcpy(loc.data(), ctx->tree.loc, 64 * K, hipMemcpyDeviceToHost));
+            if (p3) SPH_HIP_TRY(hipMemcpy(third.data(), ctx->tree.loc3, 48 * K, hipMemcpyDeviceToHost));
+            const size_t stride = p3 ? 32 : 16;
             for (size_t i = 0; i < K; ++i) {
                 const float4 l0 = loc[4 * i], l1 = loc[4 * i + 1], l2 = loc[4 * i + 2], l3 = loc[4 * i + 3];
                 const float rec[16] = {l0.x, l0.y, l0.z, 0.f, l0.w, l1.x, l1.y, l1.z,
                                        l1.w, l2.x, l2.y, l2.z, l2.w, l3.x, 0.f, 0.f};
-                std::memcpy(o + 16 * i, rec, sizeof(rec));
+                std::memcpy(o + stride * i, rec, sizeof(rec));
+                if (p3) {
+                    const float4 m0 = third[3 * i], m1 = third[3 * i + 1], m2 = third[3 * i + 2];
+                    const float ext[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
+                                           m2.x, m2.y, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    std::memcpy(o + stride * i + 16, ext, sizeof(ext));
+                }
             }
         }
         return SPH_OK;

@@ -27,7 +27,13 @@
 
 namespace {
 
-enum { eM = 0, eX, eY, eZ, eXX, eXY, eXZ, eYY, eYZ, eZZ, eNT };
+/* poles.h:18-39 index map; the third-order terms are used at P = 3 */
+enum {
+    eM = 0, eX, eY, eZ, eXX, eXY, eXZ, eYY, eYZ, eZZ, eNT,
+    eXXX = eNT, eXXY, eXXZ, eXYY, eXYZ, eXZZ, eYYY, eYYZ, eYZZ, eZZZ, eNT3
+};
+template <int P>
+constexpr int nterms() { return P == 3 ? (int)eNT3 : (int)eNT; }
 
 struct Tree {
     const int32_t* octree;
@@ -38,6 +44,8 @@ struct Tree {
     float4* far;             /* 3 per node */
     float4* aux;             /* 3 per node */
     float4* loc;             /* 4 per node */
+    float4* oct;             /* 3 per node (P = 3): (XXX XXY XXZ XYY) (XYZ XZZ YYY YYZ) (YZZ ZZZ power3 -) */
+    float4* loc3;            /* 3 per node (P = 3): third-order local terms, same order */
 };
 
 /* ------------------------------------------------------------ conversion */
@@ -146,7 +154,34 @@ struct UpArgs {
     int fix_range;
     int need_preorder; /* the near-field down pass walks subtrees as id ranges */
     int prec;          /* precision form: centres of mass summed relative to the first member */
+    int order;         /* expansion order P: 2 or 3 */
+    int fix_p3;        /* SPH_FIX_ORDER3: power[3] summed as intended */
 };
+
+/* helpers.metal:76-88.  As written the 1/3 and 1/6 weights are integer
+ * divisions and three lines assign instead of add: power[3] = |XXX|. */
+__device__ __forceinline__ float power3(const float o[10], int fixed) {
+    /* o: XXX XXY XXZ XYY XYZ XZZ YYY YYZ YZZ ZZZ */
+    if (!fixed) return fabsf(o[0]);
+    const float t = 1.f / 3.f, sx = 1.f / 6.f;
+    float p3 = o[9] * o[9];
+    p3 += t * o[8] * o[8];
+    p3 += t * o[7] * o[7];
+    p3 += o[6] * o[6];
+    p3 += t * o[5] * o[5];
+    p3 += sx * o[4] * o[4];
+    p3 += t * o[3] * o[3];
+    p3 += t * o[2] * o[2];
+    p3 += t * o[1] * o[1];
+    p3 += o[0] * o[0];
+    return sqrtf(p3);
+}
+
+__device__ __forceinline__ void store_oct(const UpArgs& a, int id, const float o[10]) {
+    a.t.oct[3 * id + 0] = make_float4(o[0], o[1], o[2], o[3]);
+    a.t.oct[3 * id + 1] = make_float4(o[4], o[5], o[6], o[7]);
+    a.t.oct[3 * id + 2] = make_float4(o[8], o[9], power3(o, a.fix_p3), 0.f);
+}
 
 __device__ __forceinline__ float power2(float XX, float XY, float XZ, float YY, float YZ, float ZZ) {
     /* helpers.metal:63-75 */
@@ -190,6 +225,7 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
     py = py / M + x0.y;
     pz = pz / M + x0.z;
     float X = 0.f, Y = 0.f, Z = 0.f, XX = 0.f, XY = 0.f, XZ = 0.f, YY = 0.f, YZ = 0.f, ZZ = 0.f;
+    float o[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int k = 0; k < np; ++k) {
         const int p = pl[k];
         const float pm = a.m[p];
@@ -204,10 +240,35 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
         YY += pm * ry * ry;
         YZ += pm * ry * rz;
         ZZ += pm * rz * rz;
+        if (a.order == 3) { /* P2M.metal:70-81 */
+            o[0] -= pm * rx * rx * rx;
+            o[1] -= pm * rx * rx * ry;
+            o[2] -= pm * rx * rx * rz;
+            o[3] -= pm * rx * ry * ry;
+            o[4] -= pm * rx * ry * rz;
+            o[5] -= pm * rx * rz * rz;
+            o[6] -= pm * ry * ry * ry;
+            o[7] -= pm * ry * ry * rz;
+            o[8] -= pm * ry * rz * rz;
+            o[9] -= pm * rz * rz * rz;
+        }
     }
     XX *= 0.5f;
     YY *= 0.5f;
     ZZ *= 0.5f;
+    if (a.order == 3) { /* P2M.metal:89-99 */
+        const float sixth = 0.166666667f;
+        o[0] *= sixth;
+        o[6] *= sixth;
+        o[9] *= sixth;
+        o[1] *= 0.5f;
+        o[2] *= 0.5f;
+        o[3] *= 0.5f;
+        o[5] *= 0.5f;
+        o[7] *= 0.5f;
+        o[8] *= 0.5f;
+        store_oct(a, id, o);
+    }
     const float size = fmaxf(fmaxf(mxx - mnx, mxy - mny), mxz - mnz);
     a.t.far[3 * id + 0] = make_float4(px, py, pz, size);
     a.t.far[3 * id + 1] = make_float4(M, XX, XY, XZ);
@@ -255,6 +316,7 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
     py = py / mass + r0.y;
     pz = pz / mass + r0.z;
     float M = 0.f, XX = 0.f, XY = 0.f, XZ = 0.f, YY = 0.f, YZ = 0.f, ZZ = 0.f;
+    float o[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int c = 0; c < 8; ++c) {
         const int cid = ch[c];
         if (cid < 0) continue;
@@ -270,6 +332,21 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
         YY += f2.x + 0.5f * ry * ry * cM;
         YZ += f2.y + ry * rz * cM;
         ZZ += f2.z + 0.5f * rz * rz * zzw;
+        if (a.order == 3) { /* M2M.metal:31-71 */
+            const float4 q0 = a.t.oct[3 * cid], q1 = a.t.oct[3 * cid + 1], q2 = a.t.oct[3 * cid + 2];
+            const float cxx = f1.y, cxy = f1.z, cxz = f1.w, cyy = f2.x, cyz = f2.y, czz = f2.z;
+            const float sixth = 0.166666667f;
+            o[9] += q2.y + rz * czz + sixth * rz * rz * rz * cM;
+            o[8] += q2.x + rz * cyz + ry * czz + 0.5f * ry * rz * rz * cM;
+            o[7] += q1.w + rz * cyy + ry * cyz + 0.5f * ry * ry * rz * cM;
+            o[6] += q1.z + ry * cyy + sixth * ry * ry * ry * cM;
+            o[5] += q1.y + rz * cxz + rx * czz + 0.5f * rx * rz * rz * cM;
+            o[4] += q1.x + rz * cxy + ry * cxz + rx * cyz + rx * ry * rz * cM;
+            o[3] += q0.w + ry * cxy + rx * cyy + 0.5f * rx * ry * ry * cM;
+            o[2] += q0.z + rz * cxx + rx * cxz + 0.5f * rx * rx * rz * cM;
+            o[1] += q0.y + ry * cxx + rx * cxy + 0.5f * rx * rx * ry * cM;
+            o[0] += q0.x + rx * cxx + sixth * rx * rx * rx * cM;
+        }
         if (allLeaves && act && a.t.aux[3 * cid + 1].w == 0.f) {
             /* leaf-sync activation (M2M.metal:145-154); as written the end of
              * the particle range starts from the parent's record */
@@ -286,6 +363,7 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
     a.t.aux[3 * id + 0] = make_float4(mnx, mny, mnz, minGrav);
     a.t.aux[3 * id + 1] = make_float4(mxx, mxy, mxz, act ? 1.f : 0.f);
     a.t.aux[3 * id + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (a.order == 3) store_oct(a, id, o);
 }
 
 __global__ __launch_bounds__(256) void k_up(UpArgs a) {
@@ -330,6 +408,8 @@ struct DownArgs {
     int fix_soft;
     int near;         /* near-field form: P2P with every rejected source leaf, one softened kernel */
     float* phi;       /* precision form: potential per particle (G included), else null */
+    int macp;         /* order of the acceptance test's error estimate (2 or 3) */
+    int fix_p3;       /* SPH_FIX_ORDER3 */
     const float4* pos;
     const float* m;
     const uint8_t* active;
@@ -339,7 +419,7 @@ struct DownArgs {
 
 struct LocalExp {
     float px, py, pz;
-    float e[eNT];
+    float e[eNT3];   /* the first nterms<P>() are used */
 };
 
 /* accept.metal:14-55 at P = 2 (the n = 1 term has power[1] = 0). */
@@ -361,15 +441,39 @@ __device__ __forceinline__ bool mac_accept(float ax, float ay, float az, float r
     return E < eta * minGrav * r2 * r2;
 }
 
-/* M2L.metal:12-108 + derivatives.metal:59-156 at P = 2: the ten local
- * coefficients one source node adds at the expansion centre. */
-__device__ __forceinline__ void m2l_terms(const LocalExp& L, float4 b0, float4 b1, float4 b2, float eps,
-                                          float T[eNT]) {
+/* accept.metal:14-55 at P = 3: E = 8 (M rho_A^3 + 3 power2 rho_A + power3),
+ * and r^p = integer_powf(r2, p / 2) = r^2 as written, r^3 fixed. */
+__device__ __forceinline__ bool mac_accept3(float ax, float ay, float az, float rho_A, float minGrav, float4 b0,
+                                            float4 b1, float4 b2, float p3, float eta, int fix_p3) {
+    const float rho_B = b0.w;
+    const float dx = ax - b0.x, dy = ay - b0.y, dz = az - b0.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float rho_sum = rho_A + rho_B;
+    if (rho_sum * rho_sum > r2) return false;
+    const float rho_max = fmaxf(rho_A, rho_B);
+    float E = b1.x * (rho_A * rho_A * rho_A);
+    E += 3.f * b2.w * rho_A;
+    E += p3;
+    E *= 8.f;
+    if (rho_A + rho_B > 0.f) {
+        E *= rho_max;
+        E /= (rho_A + rho_B);
+    }
+    const float r_to_p = fix_p3 ? r2 * sqrtf(r2) : r2;
+    return E < eta * minGrav * r_to_p * r2;
+}
+
+/* M2L.metal:12-108 + derivatives.metal:59-156: the local coefficients one
+ * source node adds at the expansion centre (ten at P = 2; at P = 3 also the
+ * octupole's potential, the quadrupole's field and the third derivatives). */
+template <int P>
+__device__ __forceinline__ void m2l_terms(const LocalExp& L, float4 b0, float4 b1, float4 b2, const float4* oct,
+                                          float eps, float T[eNT3]) {
     const float vx = L.px - b0.x, vy = L.py - b0.y, vz = L.pz - b0.z;
     const float r2 = vx * vx + vy * vy + vz * vz;
     const float r = sqrtf(r2);
     const float r_inv = r > 1e-6f ? 1.f / r : 0.f;
-    float Dt_1, Dt_2, Dt_3;
+    float Dt_1, Dt_2, Dt_3, Dt_4 = 0.f;
     if (r2 < eps * eps) {
         const float eps_inv = 1.f / eps;
         const float u = r * eps_inv;
@@ -383,10 +487,16 @@ __device__ __forceinline__ void m2l_terms(const LocalExp& L, float4 b0, float4 b
         Dt_1 = eps_inv * s1;
         Dt_2 = eps_inv2 * s2;
         Dt_3 = eps_inv2 * eps_inv * s3;
+        if (P == 3) {
+            float s4 = -315.f * u + 720.f;
+            s4 = s4 * u - 420.f; s4 = s4 * u; s4 = s4 * u;
+            Dt_4 = eps_inv2 * eps_inv2 * s4;
+        }
     } else {
         Dt_1 = r_inv;
         Dt_2 = -1.f * Dt_1 * r_inv;
         Dt_3 = -3.f * Dt_2 * r_inv;
+        if (P == 3) Dt_4 = -5.f * Dt_3 * r_inv;
     }
     const float rx = vx * r_inv, ry = vy * r_inv, rz = vz * r_inv;
     const float DX = rx * Dt_2, DY = ry * Dt_2, DZ = rz * Dt_2;
@@ -407,13 +517,46 @@ __device__ __forceinline__ void m2l_terms(const LocalExp& L, float4 b0, float4 b
     T[eYY] = M0 * DYY;
     T[eYZ] = M0 * DYZ;
     T[eZZ] = M0 * DZZ;
+    if (P == 3) {
+        /* derivatives.metal:140-154, M2L.metal:58-107 */
+        const float D3 = Dt_3 * r_inv;
+        const float rx2 = rx * rx, ry2 = ry * ry, rz2 = rz * rz;
+        const float DXXX = rx2 * rx * Dt_4 + 3.f * rx * D3, DYYY = ry2 * ry * Dt_4 + 3.f * ry * D3;
+        const float DZZZ = rz2 * rz * Dt_4 + 3.f * rz * D3;
+        const float DXXY = rx2 * ry * Dt_4 + ry * D3, DXXZ = rx2 * rz * Dt_4 + rz * D3;
+        const float DXYY = ry2 * rx * Dt_4 + rx * D3, DYYZ = ry2 * rz * Dt_4 + rz * D3;
+        const float DXZZ = rz2 * rx * Dt_4 + rx * D3, DYZZ = rz2 * ry * Dt_4 + ry * D3;
+        const float DXYZ = rx * ry * rz * Dt_4;
+        const float4 o0 = oct[0], o1 = oct[1], o2 = oct[2];
+        /* o0 = (XXX XXY XXZ XYY), o1 = (XYZ XZZ YYY YYZ), o2 = (YZZ ZZZ -, -) */
+        float lo = o0.x * DXXX + o1.z * DYYY + o2.y * DZZZ;
+        lo += o0.y * DXXY + o0.z * DXXZ + o0.w * DXYY;
+        lo += o1.w * DYYZ + o1.y * DXZZ + o2.x * DYZZ;
+        lo += o1.x * DXYZ;
+        T[eM] += lo;
+        const float QXX = b1.y, QXY = b1.z, QXZ = b1.w, QYY = b2.x, QYZ = b2.y, QZZ = b2.z;
+        T[eX] += QXX * DXXX + QYY * DXYY + QZZ * DXZZ + (QXY * DXXY + QXZ * DXXZ + QYZ * DXYZ);
+        T[eY] += QXX * DXXY + QYY * DYYY + QZZ * DYZZ + (QXY * DXYY + QXZ * DXYZ + QYZ * DYYZ);
+        T[eZ] += QXX * DXXZ + QYY * DYYZ + QZZ * DZZZ + (QXY * DXYZ + QXZ * DXZZ + QYZ * DYZZ);
+        T[eXXX] = M0 * DXXX;
+        T[eXXY] = M0 * DXXY;
+        T[eXXZ] = M0 * DXXZ;
+        T[eXYY] = M0 * DXYY;
+        T[eXYZ] = M0 * DXYZ;
+        T[eXZZ] = M0 * DXZZ;
+        T[eYYY] = M0 * DYYY;
+        T[eYYZ] = M0 * DYYZ;
+        T[eYZZ] = M0 * DYZZ;
+        T[eZZZ] = M0 * DZZZ;
+    }
 }
 
-__device__ __forceinline__ void m2l_add(LocalExp& L, float4 b0, float4 b1, float4 b2, float eps) {
-    float T[eNT];
-    m2l_terms(L, b0, b1, b2, eps, T);
+template <int P>
+__device__ __forceinline__ void m2l_add(const DownArgs& a, LocalExp& L, int node, float4 b0, float4 b1, float4 b2) {
+    float T[eNT3];
+    m2l_terms<P>(L, b0, b1, b2, a.t.oct + 3 * node, a.eps, T);
 #pragma unroll
-    for (int k = 0; k < eNT; ++k) L.e[k] += T[k];
+    for (int k = 0; k < nterms<P>(); ++k) L.e[k] += T[k];
 }
 
 /* Neumaier's compensated sum (Neumaier 1974): s + c carries the sum of the
@@ -425,12 +568,13 @@ __device__ __forceinline__ void kadd(float& s, float& c, float x) {
     s = t;
 }
 
-__device__ __forceinline__ void m2l_add_comp(LocalExp& L, float Lc[eNT], float4 b0, float4 b1, float4 b2,
-                                             float eps) {
-    float T[eNT];
-    m2l_terms(L, b0, b1, b2, eps, T);
+template <int P>
+__device__ __forceinline__ void m2l_add_comp(const DownArgs& a, LocalExp& L, float Lc[eNT3], int node, float4 b0,
+                                             float4 b1, float4 b2) {
+    float T[eNT3];
+    m2l_terms<P>(L, b0, b1, b2, a.t.oct + 3 * node, a.eps, T);
 #pragma unroll
-    for (int k = 0; k < eNT; ++k) kadd(L.e[k], Lc[k], T[k]);
+    for (int k = 0; k < nterms<P>(); ++k) kadd(L.e[k], Lc[k], T[k]);
 }
 
 /* Softened 1/r whose derivatives M2L uses (derivatives.metal:12-57,
@@ -448,11 +592,18 @@ struct NodeA {
     float x, y, z, size, minGrav;
 };
 
+/* The acceptance test with the error estimate of order a.macp. */
+__device__ __forceinline__ bool mac_ok(const DownArgs& a, const NodeA& A, int node, float4 b0, float4 b1, float4 b2) {
+    if (a.macp == 3)
+        return mac_accept3(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.t.oct[3 * node + 2].z, a.eta, a.fix_p3);
+    return mac_accept(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.eta);
+}
+
 /* checkAndAddLocalExpansion (helpers.h:34-45) */
 __device__ __forceinline__ bool check_add(const DownArgs& a, const NodeA& A, LocalExp& L, int node, bool force) {
     const float4 b0 = a.t.far[3 * node], b1 = a.t.far[3 * node + 1], b2 = a.t.far[3 * node + 2];
-    if (force || mac_accept(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.eta)) {
-        m2l_add(L, b0, b1, b2, a.eps);
+    if (force || mac_ok(a, A, node, b0, b1, b2)) {
+        m2l_add<2>(a, L, node, b0, b1, b2);
         return true;
     }
     return false;
@@ -521,22 +672,78 @@ __device__ __forceinline__ float dphi_dr_m2l(float r, float eps) {
 
 /* transformLocal (L2L.metal:12-88) of the node's local expansion to one
  * active child, written to the child's record (L2L.metal:90-104). */
+template <int P>
 __device__ __forceinline__ void l2l_child(const DownArgs& a, const LocalExp& L, int cid) {
     if (cid < 0) return;
     if (a.t.aux[3 * cid + 1].w == 0.f) return;
     const float4 c0 = a.t.far[3 * cid];
     const float rx = c0.x - L.px, ry = c0.y - L.py, rz = c0.z - L.pz;
-    float nM = L.e[eM];
-    nM += rx * L.e[eX] + ry * L.e[eY] + rz * L.e[eZ];
-    nM += 0.5f * (rx * rx * L.e[eXX] + ry * ry * L.e[eYY] + rz * rz * L.e[eZZ]);
-    nM += rx * ry * L.e[eXY] + rx * rz * L.e[eXZ] + ry * rz * L.e[eYZ];
-    const float nX = L.e[eX] + (rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ]);
-    const float nY = L.e[eY] + (rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ]);
-    const float nZ = L.e[eZ] + (rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ]);
+    const float* e = L.e;
+    float nM = e[eM];
+    nM += rx * e[eX] + ry * e[eY] + rz * e[eZ];
+    nM += 0.5f * (rx * rx * e[eXX] + ry * ry * e[eYY] + rz * rz * e[eZZ]);
+    nM += rx * ry * e[eXY] + rx * rz * e[eXZ] + ry * rz * e[eYZ];
+    float nX = e[eX] + (rx * e[eXX] + ry * e[eXY] + rz * e[eXZ]);
+    float nY = e[eY] + (rx * e[eXY] + ry * e[eYY] + rz * e[eYZ]);
+    float nZ = e[eZ] + (rx * e[eXZ] + ry * e[eYZ] + rz * e[eZZ]);
+    float nXX = e[eXX], nXY = e[eXY], nXZ = e[eXZ], nYY = e[eYY], nYZ = e[eYZ], nZZ = e[eZZ];
+    if (P == 3) {
+        /* L2L.metal:37-86 */
+        const float sixth = 0.166666667f;
+        nM += sixth * rx * rx * rx * e[eXXX] + sixth * ry * ry * ry * e[eYYY] + sixth * rz * rz * rz * e[eZZZ];
+        nM += 0.5f * rx * rx * ry * e[eXXY] + 0.5f * rx * rx * rz * e[eXXZ] + 0.5f * rx * ry * ry * e[eXYY];
+        nM += 0.5f * ry * ry * rz * e[eYYZ] + 0.5f * rx * rz * rz * e[eXZZ] + 0.5f * ry * rz * rz * e[eYZZ];
+        nM += rx * ry * rz * e[eXYZ];
+        nX += 0.5f * rx * rx * e[eXXX] + 0.5f * ry * ry * e[eXYY] + 0.5f * rz * rz * e[eXZZ];
+        nX += rx * ry * e[eXXY] + rx * rz * e[eXXZ] + ry * rz * e[eXYZ];
+        nY += 0.5f * rx * rx * e[eXXY] + 0.5f * ry * ry * e[eYYY] + 0.5f * rz * rz * e[eYZZ];
+        nY += rx * ry * e[eXYY] + rx * rz * e[eXYZ] + ry * rz * e[eYYZ];
+        nZ += 0.5f * rx * rx * e[eXXZ] + 0.5f * ry * ry * e[eYYZ] + 0.5f * rz * rz * e[eZZZ];
+        nZ += rx * ry * e[eXYZ] + rx * rz * e[eXZZ] + ry * rz * e[eYZZ];
+        nXX += rx * e[eXXX] + ry * e[eXXY] + rz * e[eXXZ];
+        nYY += rx * e[eXYY] + ry * e[eYYY] + rz * e[eYYZ];
+        nZZ += rx * e[eXZZ] + ry * e[eYZZ] + rz * e[eZZZ];
+        nXY += rx * e[eXXY] + ry * e[eXYY] + rz * e[eXYZ];
+        nXZ += rx * e[eXXZ] + ry * e[eXYZ] + rz * e[eXZZ];
+        nYZ += rx * e[eXYZ] + ry * e[eYYZ] + rz * e[eYZZ];
+        a.t.loc3[3 * cid + 0] = make_float4(e[eXXX], e[eXXY], e[eXXZ], e[eXYY]);
+        a.t.loc3[3 * cid + 1] = make_float4(e[eXYZ], e[eXZZ], e[eYYY], e[eYYZ]);
+        a.t.loc3[3 * cid + 2] = make_float4(e[eYZZ], e[eZZZ], 0.f, 0.f);
+    }
     a.t.loc[4 * cid + 0] = make_float4(L.px + rx, L.py + ry, L.pz + rz, nM);
-    a.t.loc[4 * cid + 1] = make_float4(nX, nY, nZ, L.e[eXX]);
-    a.t.loc[4 * cid + 2] = make_float4(L.e[eXY], L.e[eXZ], L.e[eYY], L.e[eYZ]);
-    a.t.loc[4 * cid + 3] = make_float4(L.e[eZZ], 0.f, 0.f, 0.f);
+    a.t.loc[4 * cid + 1] = make_float4(nX, nY, nZ, nXX);
+    a.t.loc[4 * cid + 2] = make_float4(nXY, nXZ, nYY, nYZ);
+    a.t.loc[4 * cid + 3] = make_float4(nZZ, 0.f, 0.f, 0.f);
+}
+
+/* The node's incoming local expansion (written by its parent's L2L) added
+ * to the group's M2L sum. */
+template <int P>
+__device__ __forceinline__ void add_incoming(const DownArgs& a, int id, float4 l0, LocalExp& L) {
+    const float4 l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2], l3 = a.t.loc[4 * id + 3];
+    L.e[eM] = l0.w + L.e[eM];
+    L.e[eX] = l1.x + L.e[eX];
+    L.e[eY] = l1.y + L.e[eY];
+    L.e[eZ] = l1.z + L.e[eZ];
+    L.e[eXX] = l1.w + L.e[eXX];
+    L.e[eXY] = l2.x + L.e[eXY];
+    L.e[eXZ] = l2.y + L.e[eXZ];
+    L.e[eYY] = l2.z + L.e[eYY];
+    L.e[eYZ] = l2.w + L.e[eYZ];
+    L.e[eZZ] = l3.x + L.e[eZZ];
+    if (P == 3) {
+        const float4 m0 = a.t.loc3[3 * id], m1 = a.t.loc3[3 * id + 1], m2 = a.t.loc3[3 * id + 2];
+        L.e[eXXX] = m0.x + L.e[eXXX];
+        L.e[eXXY] = m0.y + L.e[eXXY];
+        L.e[eXXZ] = m0.z + L.e[eXXZ];
+        L.e[eXYY] = m0.w + L.e[eXYY];
+        L.e[eXYZ] = m1.x + L.e[eXYZ];
+        L.e[eXZZ] = m1.y + L.e[eXZZ];
+        L.e[eYYY] = m1.z + L.e[eYYY];
+        L.e[eYYZ] = m1.w + L.e[eYYZ];
+        L.e[eYZZ] = m2.x + L.e[eYZZ];
+        L.e[eZZZ] = m2.y + L.e[eZZZ];
+    }
 }
 
 /* L2P + intra-leaf P2P for particle k of a leaf (helpers.h:47-81); (nx, ny,
@@ -544,23 +751,39 @@ __device__ __forceinline__ void l2l_child(const DownArgs& a, const LocalExp& L, 
  * PREC (precision form, near field only): compensated sums, and the
  * potential -G (local expansion at x + sum of the pair potentials) with
  * `nphi` the rejected leaves' part. */
-template <bool PREC>
+template <bool PREC, int P>
 __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp& L, const int32_t* pl, int np, int k,
                                               float nx = 0.f, float ny = 0.f, float nz = 0.f, float nphi = 0.f) {
     const int p = pl[k];
     if (!a.active[p]) return;
     const float4 xi = a.pos[p];
     const float rx = xi.x - L.px, ry = xi.y - L.py, rz = xi.z - L.pz;
+    const float* e = L.e;
     /* L2P (L2P.metal:12-32) */
-    float ax = L.e[eX], ay = L.e[eY], az = L.e[eZ];
-    ax += rx * L.e[eXX] + ry * L.e[eXY] + rz * L.e[eXZ];
-    ay += rx * L.e[eXY] + ry * L.e[eYY] + rz * L.e[eYZ];
-    az += rx * L.e[eXZ] + ry * L.e[eYZ] + rz * L.e[eZZ];
+    float ax = e[eX], ay = e[eY], az = e[eZ];
+    ax += rx * e[eXX] + ry * e[eXY] + rz * e[eXZ];
+    ay += rx * e[eXY] + ry * e[eYY] + rz * e[eYZ];
+    az += rx * e[eXZ] + ry * e[eYZ] + rz * e[eZZ];
+    if (P == 3) {
+        ax += 0.5f * rx * rx * e[eXXX] + 0.5f * ry * ry * e[eXYY] + 0.5f * rz * rz * e[eXZZ];
+        ax += rx * ry * e[eXXY] + rx * rz * e[eXXZ] + ry * rz * e[eXYZ];
+        ay += 0.5f * rx * rx * e[eXXY] + 0.5f * ry * ry * e[eYYY] + 0.5f * rz * rz * e[eYZZ];
+        ay += rx * ry * e[eXYY] + rx * rz * e[eXYZ] + ry * rz * e[eYYZ];
+        az += 0.5f * rx * rx * e[eXXZ] + 0.5f * ry * ry * e[eYYZ] + 0.5f * rz * rz * e[eZZZ];
+        az += rx * ry * e[eXYZ] + rx * rz * e[eXZZ] + ry * rz * e[eYZZ];
+    }
     float cx = 0.f, cy = 0.f, cz = 0.f, ps = 0.f, pc = 0.f;
     if (PREC) {
-        ps = L.e[eM] + rx * L.e[eX] + ry * L.e[eY] + rz * L.e[eZ];
-        ps += 0.5f * (rx * rx * L.e[eXX] + ry * ry * L.e[eYY] + rz * rz * L.e[eZZ]);
-        ps += rx * ry * L.e[eXY] + rx * rz * L.e[eXZ] + ry * rz * L.e[eYZ];
+        ps = e[eM] + rx * e[eX] + ry * e[eY] + rz * e[eZ];
+        ps += 0.5f * (rx * rx * e[eXX] + ry * ry * e[eYY] + rz * rz * e[eZZ]);
+        ps += rx * ry * e[eXY] + rx * rz * e[eXZ] + ry * rz * e[eYZ];
+        if (P == 3) {
+            const float sixth = 0.166666667f;
+            ps += sixth * (rx * rx * rx * e[eXXX] + ry * ry * ry * e[eYYY] + rz * rz * rz * e[eZZZ]);
+            ps += 0.5f * (rx * rx * ry * e[eXXY] + rx * rx * rz * e[eXXZ] + rx * ry * ry * e[eXYY]);
+            ps += 0.5f * (ry * ry * rz * e[eYYZ] + rx * rz * rz * e[eXZZ] + ry * rz * rz * e[eYZZ]);
+            ps += rx * ry * rz * e[eXYZ];
+        }
     }
     for (int j = 0; j < np; ++j) {
         if (j == k) continue;
@@ -600,16 +823,17 @@ __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp&
     a.gravnorm[p] = sqrtf(ax * ax + ay * ay + az * az);
 }
 
-template <bool PREC>
+template <bool PREC, int P>
 __device__ __forceinline__ void leaf_field_lanes(const DownArgs& a, const LocalExp& L, int4 inf, int lane, int width) {
     const int32_t* pl = a.t.octree + inf.y + 2;
-    for (int k = lane; k < inf.x; k += width) leaf_particle<PREC>(a, L, pl, inf.x, k);
+    for (int k = lane; k < inf.x; k += width) leaf_particle<PREC, P>(a, L, pl, inf.x, k);
 }
 
 /* localToLeaf (helpers.h:47-81): L2P plus the intra-leaf P2P */
+template <int P>
 __device__ void leaf_field(const DownArgs& a, const LocalExp& L, int4 inf) {
-    if (a.phi) leaf_field_lanes<true>(a, L, inf, 0, 1);
-    else leaf_field_lanes<false>(a, L, inf, 0, 1);
+    if (a.phi) leaf_field_lanes<true, P>(a, L, inf, 0, 1);
+    else leaf_field_lanes<false, P>(a, L, inf, 0, 1);
 }
 
 /* downPass (gravity.metal:64-143) for one non-root node */
@@ -652,9 +876,9 @@ __global__ __launch_bounds__(256) void k_down(DownArgs a) {
     if (outp < maxOut && np == 0) a.out[outp] = -1;
     if (np == 0) {
         /* L2L (L2L.metal:90-104) to the active children */
-        for (int c = 0; c < 8; ++c) l2l_child(a, L, a.t.child[8 * id + c]);
+        for (int c = 0; c < 8; ++c) l2l_child<2>(a, L, a.t.child[8 * id + c]);
     } else {
-        leaf_field(a, L, inf);
+        leaf_field<2>(a, L, inf);
     }
 }
 
@@ -674,15 +898,16 @@ __device__ __forceinline__ float group_sum(float v) {
  * targets are the leaf's particles, NEAR_K at a time in registers; the M2L
  * part runs in the first pass only.  Partial sums are reduced over the group,
  * the incoming local expansion is added, and lane k finishes particle k. */
-template <int GS, bool PREC>
+template <int GS, bool PREC, int P>
 __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA& A, LocalExp& L, float4 l0,
                                const int32_t* in, int lane, int wl, uint64_t gmask) {
     constexpr int NEAR_K = 4;
+    constexpr int NTP = nterms<P>();
     const int np = inf.x;
     const int32_t* pl = a.t.octree + inf.y + 2;
-    float Lc[eNT];
+    float Lc[eNT3];
 #pragma unroll
-    for (int k = 0; k < eNT; ++k) Lc[k] = 0.f;
+    for (int k = 0; k < NTP; ++k) Lc[k] = 0.f;
     for (int k0 = 0; k0 < np; k0 += NEAR_K) {
         float tx[NEAR_K], ty[NEAR_K], tz[NEAR_K], gx[NEAR_K], gy[NEAR_K], gz[NEAR_K];
         /* precision form: compensations and the pair potential */
@@ -713,10 +938,10 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
                     continue;
                 }
                 const float4 b0 = a.t.far[3 * cur], b1 = a.t.far[3 * cur + 1], b2 = a.t.far[3 * cur + 2];
-                if (mac_accept(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.eta)) {
+                if (mac_ok(a, A, cur, b0, b1, b2)) {
                     if (k0 == 0) {
-                        if constexpr (PREC) m2l_add_comp(L, Lc, b0, b1, b2, a.eps);
-                        else m2l_add(L, b0, b1, b2, a.eps);
+                        if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, cur, b0, b1, b2);
+                        else m2l_add<P>(a, L, cur, b0, b1, b2);
                     }
                     cur = a.t.sub_end[cur];
                     continue;
@@ -763,25 +988,16 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
         }
         if (k0 == 0) {
 #pragma unroll
-            for (int k = 0; k < eNT; ++k) {
+            for (int k = 0; k < NTP; ++k) {
                 L.e[k] = group_sum<GS>(L.e[k]);
                 if constexpr (PREC) L.e[k] += group_sum<GS>(Lc[k]);
             }
-            const float4 l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2], l3 = a.t.loc[4 * id + 3];
-            L.e[eM] = l0.w + L.e[eM];
-            L.e[eX] = l1.x + L.e[eX];
-            L.e[eY] = l1.y + L.e[eY];
-            L.e[eZ] = l1.z + L.e[eZ];
-            L.e[eXX] = l1.w + L.e[eXX];
-            L.e[eXY] = l2.x + L.e[eXY];
-            L.e[eXZ] = l2.y + L.e[eXZ];
-            L.e[eYY] = l2.z + L.e[eYY];
-            L.e[eYZ] = l2.w + L.e[eYZ];
-            L.e[eZZ] = l3.x + L.e[eZZ];
+            add_incoming<P>(a, id, l0, L);
         }
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk)
-            if (lane == kk && k0 + kk < np) leaf_particle<PREC>(a, L, pl, np, k0 + kk, gx[kk], gy[kk], gz[kk], gp[kk]);
+            if (lane == kk && k0 + kk < np)
+                leaf_particle<PREC, P>(a, L, pl, np, k0 + kk, gx[kk], gy[kk], gz[kk], gp[kk]);
     }
 }
 
@@ -799,8 +1015,9 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
  * partial local expansion; the group reduces them with shuffles (a fixed
  * order, so results are reproducible), adds the incoming local expansion and
  * runs L2L (lane c -> child c) or L2P + P2P (lane k -> particle k). */
-template <int GS, bool NEAR, bool PREC>
+template <int GS, bool NEAR, bool PREC, int P>
 __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
+    constexpr int NTP = nterms<P>();
     constexpr int GPB = 256 / GS;
     constexpr int EPR = GS / 8;
     const int tid = threadIdx.x;
@@ -827,13 +1044,13 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     L.px = l0.x;
     L.py = l0.y;
     L.pz = l0.z;
-    float Lc[eNT]; /* compensations (precision form) */
+    float Lc[eNT3]; /* compensations (precision form) */
 #pragma unroll
-    for (int k = 0; k < eNT; ++k) L.e[k] = Lc[k] = 0.f;
+    for (int k = 0; k < NTP; ++k) L.e[k] = Lc[k] = 0.f;
     const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
     if constexpr (NEAR) {
         if (live && np != 0) {
-            down_near_leaf<GS, PREC>(a, id, inf, A, L, l0, in, lane, wl, gmask);
+            down_near_leaf<GS, PREC, P>(a, id, inf, A, L, l0, in, lane, wl, gmask);
             return;
         }
     }
@@ -853,12 +1070,11 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
         more = term == 0ull && base + GS < a.maxu;
         bool acc = false;
         if (valid) {
-            if constexpr (PREC) {
-                const float4 b0 = a.t.far[3 * e], b1 = a.t.far[3 * e + 1], b2 = a.t.far[3 * e + 2];
-                acc = mac_accept(A.x, A.y, A.z, A.size, A.minGrav, b0, b1, b2, a.eta);
-                if (acc) m2l_add_comp(L, Lc, b0, b1, b2, a.eps);
-            } else {
-                acc = check_add(a, A, L, e, false);
+            const float4 b0 = a.t.far[3 * e], b1 = a.t.far[3 * e + 1], b2 = a.t.far[3 * e + 2];
+            acc = mac_ok(a, A, e, b0, b1, b2);
+            if (acc) {
+                if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, e, b0, b1, b2);
+                else m2l_add<P>(a, L, e, b0, b1, b2);
             }
         }
         uint64_t rej = __ballot(valid && !acc) & gmask;
@@ -887,7 +1103,7 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
                 c2 = a.t.far[3 * child + 2];
                 if (leaf_ent) pushc = true;
                 else if (!NEAR && (np != 0 || a.t.info[child].x != 0)) take = true;
-                else if (mac_accept(A.x, A.y, A.z, A.size, A.minGrav, c0, c1, c2, a.eta)) take = true;
+                else if (mac_ok(a, A, child, c0, c1, c2)) take = true;
                 else pushc = true;
             }
             const uint64_t pm = __ballot(pushc) & gmask;
@@ -898,35 +1114,23 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
             }
             running += __popcll(pm);
             if (take) {
-                if constexpr (PREC) m2l_add_comp(L, Lc, c0, c1, c2, a.eps);
-                else m2l_add(L, c0, c1, c2, a.eps);
+                if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, child, c0, c1, c2);
+                else m2l_add<P>(a, L, child, c0, c1, c2);
             }
         }
     }
     if (live && np == 0 && lane == 0 && running < a.maxu) a.out[outbase + (uint32_t)running] = -1;
 #pragma unroll
-    for (int k = 0; k < eNT; ++k) {
+    for (int k = 0; k < NTP; ++k) {
         L.e[k] = group_sum<GS>(L.e[k]);
         if constexpr (PREC) L.e[k] += group_sum<GS>(Lc[k]);
     }
     if (!live) return;
-    {
-        const float4 l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2], l3 = a.t.loc[4 * id + 3];
-        L.e[eM] = l0.w + L.e[eM];
-        L.e[eX] = l1.x + L.e[eX];
-        L.e[eY] = l1.y + L.e[eY];
-        L.e[eZ] = l1.z + L.e[eZ];
-        L.e[eXX] = l1.w + L.e[eXX];
-        L.e[eXY] = l2.x + L.e[eXY];
-        L.e[eXZ] = l2.y + L.e[eXZ];
-        L.e[eYY] = l2.z + L.e[eYY];
-        L.e[eYZ] = l2.w + L.e[eYZ];
-        L.e[eZZ] = l3.x + L.e[eZZ];
-    }
+    add_incoming<P>(a, id, l0, L);
     if (np == 0) {
-        if (lane < 8) l2l_child(a, L, a.t.child[8 * id + lane]);
+        if (lane < 8) l2l_child<P>(a, L, a.t.child[8 * id + lane]);
     } else {
-        leaf_field_lanes<PREC>(a, L, inf, lane, GS);
+        leaf_field_lanes<PREC, P>(a, L, inf, lane, GS);
     }
 }
 
@@ -934,16 +1138,21 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
  * A root that is itself a leaf (n <= max_children_in_leaf) makes the
  * reference read particle indices as child pointers; here its particles get
  * the intra-leaf P2P with a zero local expansion instead. */
+template <int P>
 __global__ void k_down_root(DownArgs a) {
     if (a.flags[0] != 0u) return;
     const Tree& t = a.t;
     const int4 inf = t.info[0];
+    /* the root receives no local expansion: its record is defined as zero */
+    for (int q = 0; q < 4; ++q) t.loc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (P == 3)
+        for (int q = 0; q < 3; ++q) t.loc3[q] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (inf.x != 0) {
         const float4 f0 = t.far[0];
         LocalExp L;
         L.px = f0.x; L.py = f0.y; L.pz = f0.z;
-        for (int k = 0; k < eNT; ++k) L.e[k] = 0.f;
-        leaf_field(a, L, inf);
+        for (int k = 0; k < eNT3; ++k) L.e[k] = 0.f;
+        leaf_field<P>(a, L, inf);
         a.out[0] = -1;
         return;
     }
@@ -955,6 +1164,8 @@ __global__ void k_down_root(DownArgs a) {
         t.loc[4 * cid + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
         t.loc[4 * cid + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
         t.loc[4 * cid + 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (P == 3)
+            for (int q = 0; q < 3; ++q) t.loc3[3 * cid + q] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     a.out[0] = 0;
     a.out[1] = -1;
@@ -970,6 +1181,8 @@ Tree make_tree(sph_ctx* ctx) {
     t.far = ctx->tree.mp_far;
     t.aux = ctx->tree.mp_aux;
     t.loc = ctx->tree.loc;
+    t.oct = ctx->tree.mp_oct;
+    t.loc3 = ctx->tree.loc3;
     return t;
 }
 
@@ -1028,6 +1241,8 @@ int launch_gravity_up(sph_ctx* ctx) {
     a.fix_range = (ctx->prm.fixes & SPH_FIX_SYNC_RANGE) ? 1 : 0;
     a.need_preorder = (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ? 1 : 0;
     a.prec = ctx->prm.precision ? 1 : 0;
+    a.order = ctx->prm.expansion_order == 3 ? 3 : 2;
+    a.fix_p3 = (ctx->prm.fixes & SPH_FIX_ORDER3) ? 1 : 0;
     const int threads = 256;
     for (int l = (int)T.level_counts.size() - 1; l >= 0; --l) {
         a.count = T.level_counts[l];
@@ -1055,6 +1270,9 @@ int launch_gravity_down(sph_ctx* ctx) {
     /* the precision form (near field required, checked at creation) also
      * writes the potential */
     a.phi = (p.precision && a.near) ? ctx->phi : nullptr;
+    const bool order3 = p.expansion_order == 3;
+    a.macp = (order3 && p.mac_order != 2) ? 3 : 2;
+    a.fix_p3 = (p.fixes & SPH_FIX_ORDER3) ? 1 : 0;
     a.pos = ctx->pos;
     a.m = ctx->m;
     a.active = ctx->active;
@@ -1066,7 +1284,8 @@ int launch_gravity_down(sph_ctx* ctx) {
         int32_t* out = (l % 2 == 0) ? ctx->list_b : ctx->list_a;
         if (l == 0) {
             a.out = out;
-            k_down_root<<<1, 1, 0, ctx->stream>>>(a);
+            if (order3) k_down_root<3><<<1, 1, 0, ctx->stream>>>(a);
+            else k_down_root<2><<<1, 1, 0, ctx->stream>>>(a);
             SPH_HIP_TRY(hipGetLastError());
             continue;
         }
@@ -1079,9 +1298,16 @@ int launch_gravity_down(sph_ctx* ctx) {
         if (a.max_rec == 1 || a.near) {
             constexpr int GS = 32;
             const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
-            if (a.phi) k_down_group<GS, true, true><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
-            else if (a.near) k_down_group<GS, true, false><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
-            else k_down_group<GS, false, false><<<xcd_grid(blocks), threads, 0, ctx->stream>>>(a);
+            const dim3 g(xcd_grid(blocks));
+            if (order3) {
+                if (a.phi) k_down_group<GS, true, true, 3><<<g, threads, 0, ctx->stream>>>(a);
+                else if (a.near) k_down_group<GS, true, false, 3><<<g, threads, 0, ctx->stream>>>(a);
+                else k_down_group<GS, false, false, 3><<<g, threads, 0, ctx->stream>>>(a);
+            } else {
+                if (a.phi) k_down_group<GS, true, true, 2><<<g, threads, 0, ctx->stream>>>(a);
+                else if (a.near) k_down_group<GS, true, false, 2><<<g, threads, 0, ctx->stream>>>(a);
+                else k_down_group<GS, false, false, 2><<<g, threads, 0, ctx->stream>>>(a);
+            }
         } else {
             k_down<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
         }

@@ -41,4 +41,5 @@ extern "C" void sph_params_default(sph_params* o) {
     o->density_lanes = 8;
     o->accel_lanes = 0;
     o->lane_split = 0;
+    o->expansion_order = P; /* Parameters.h */
 }

@@ -22,6 +22,8 @@
  *          mp_far  3 x float4 x K  (pos, size) (M, XX, XY, XZ) (YY, YZ, ZZ, power2)
  *          mp_aux  3 x float4 x K  (min, minGrav) (max, active) (X, Y, Z, -) dipoles
  *          loc     4 x float4 x K  (pos, L_M) (L_X, L_Y, L_Z, L_XX) (L_XY, L_XZ, L_YY, L_YZ) (L_ZZ, -, -, -)
+ *          mp_oct, loc3  3 x float4 x K each: octupoles + power3, third-order
+ *                  local terms (XXX XXY XXZ XYY XYZ XZZ YYY YYZ YZZ ZZZ), P = 3
  *          node_info int4 x K (nParticles, treePointer, level, -), parent_slot int32 x K
  *          subtree_end int32 x K (first node id after the node's pre-order subtree)
  *          node_child int32 x 8K (child node ids, -1)
@@ -63,6 +65,8 @@ struct sph_tree_dev {
     int32_t* node_child = nullptr;  /* 8 per node */
     float4* mp_far = nullptr;       /* 3 per node */
     float4* mp_aux = nullptr;       /* 3 per node */
+    float4* mp_oct = nullptr;       /* 3 per node: octupoles + power[3] (expansion order 3) */
+    float4* loc3 = nullptr;         /* 3 per node: third-order local terms (expansion order 3) */
     float4* loc = nullptr;          /* 4 per node */
     int64_t node_cap = 0;
     int32_t node_count = 0;

@@ -2,9 +2,9 @@
 
   plain method, 1e5 particles, up to 4 timestep rungs (individual steps):
       ten steps against the oracle;
-  precision form, 1e6 particles, up to 8 rungs, 200 steps: gravity RMS
-      against an FP64 direct sum on 8192 sampled particles (seed 8192), and
-      energy / angular-momentum drift over the run.
+  precision form, 1e6 particles, 200 steps: gravity RMS against an FP64
+      direct sum on 8192 sampled particles (seed 8192), and energy /
+      angular-momentum drift over the run.
 """
 import numpy as np
 import pytest
@@ -39,11 +39,12 @@ def test_plain_1e5_four_rungs_matches_oracle(sph, orc, tables256, impact):
 
 def test_precision_1e6_rms_and_drift(sph, orc):
     """Precision form at 1e6 particles, 200 steps through the host Headless
-    loop (octree rebuilt before every step) at the reference's GRAVITY_ETA,
-    every particle active every step (the reference's rule): energy and
-    angular-momentum drift.  On the final state, one gravity evaluation at
-    GRAVITY_ETA 2e-5 with 16384 list entries (warm MAC) against an FP64
-    direct sum at 8192 particles (seed 8192).
+    loop (octree rebuilt before every step), every particle active every step
+    (the reference's rule): energy and angular-momentum drift.  Its gravity
+    uses third-order expansions with the second-order acceptance estimate at
+    GRAVITY_ETA 1e-3.  On the final state: gravity RMS against an FP64 direct
+    sum at 8192 particles (seed 8192), for that form and, for comparison, for
+    the P = 2 expansions at the reference's GRAVITY_ETA.
 
     Individual timesteps (the README rule) are not used here: the
     reference's step drifts h and rho of inactive particles with their last
@@ -52,7 +53,9 @@ def test_precision_1e6_rms_and_drift(sph, orc):
     thousands of cells."""
     import time
     n = 1_000_000
-    prm = sph.default_params(fixes=sph.SPH_FIX_ALL, precision=1)
+    form = dict(fixes=sph.SPH_FIX_ALL, precision=1, expansion_order=3, mac_order=2, gravity_eta=1e-3,
+                max_unchecked_pointers=8192)
+    prm = sph.default_params(**form)
     hl = sph.Headless(prm, sph.impact_config(n, 3), eos_res=256, tree_mode=sph.TREE_EVERY, tree_threads=8)
     ctx = hl.ctx
     hl.step()
@@ -76,8 +79,9 @@ def test_precision_1e6_rms_and_drift(sph, orc):
     targets = np.random.default_rng(8192).choice(n, 8192, replace=False).astype(np.int32)
     ref, _ = orc.direct_sum(state["pos"], state["mass"], targets, prm.gravity_g, -prm.gravity_softening)
     out = {}
-    for eta, maxu in ((prm.gravity_eta, 1024), (2e-5, 16384)):
-        p2 = sph.default_params(fixes=sph.SPH_FIX_ALL, precision=1, gravity_eta=eta, max_unchecked_pointers=maxu)
+    for name, over in (("P3 eta 1e-3", form),
+                       ("P2 reference eta", dict(fixes=sph.SPH_FIX_ALL, precision=1, max_unchecked_pointers=1024))):
+        p2 = sph.default_params(**over)
         g = sph.Context(p2, state, sph.make_eos_tables(p2, 64))
         g.upload("gravnorm", gn)
         g.upload_tree(tree)
@@ -87,9 +91,9 @@ def test_precision_1e6_rms_and_drift(sph, orc):
         dt = time.perf_counter() - t1
         a = g.download("acc")[targets, :3].astype(np.float64)
         rel = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        out[eta] = float(np.sqrt(np.mean(rel ** 2)))
-        print(f"[precision 1e6] eta {eta:g} lists {maxu}: gravity {dt * 1e3:.1f} ms, RMS {out[eta]:.3e} "
+        out[name] = float(np.sqrt(np.mean(rel ** 2)))
+        print(f"[precision 1e6] {name}: gravity {dt * 1e3:.1f} ms, RMS {out[name]:.3e} "
               f"median {np.median(rel):.3e} max {rel.max():.3e}", flush=True)
         g.close()
-    assert out[2e-5] < 1e-3
+    assert out["P3 eta 1e-3"] < 1e-3
     assert de < 5e-2 and dl < 5e-3

@@ -56,10 +56,12 @@ def test_default_params_are_the_reference_values(sph):
     assert p.gravity_g == pytest.approx(6.67e-5) and p.fixes == 0
     assert p.activity_rule == sph.SPH_ACTIVE_AS_WRITTEN and p.max_rungs == 0
     assert p.precision == 0 and p.density_lanes == 8 and p.accel_lanes == 0 and p.lane_split == 0
+    assert p.expansion_order == 2 and p.mac_order == 0
 
 
 def test_params_struct_layout_matches_c(sph):
-    # 26 four-byte fields, precision, density_lanes, accel_lanes, lane_split, 4 reserved ints
+    # 26 four-byte fields, precision, density_lanes, accel_lanes, lane_split, expansion_order, mac_order,
+    # 2 reserved ints
     assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 8)
 
 

@@ -684,3 +684,180 @@ def test_as_written_near_field_is_inaccurate(sph, orc, tables256):
     acc, _ = orc.direct_sum(st["pos"], st["mass"], tg, prm.gravity_g, -prm.gravity_softening)
     rel = np.linalg.norm(st["acc"][tg, :3] - acc, axis=1) / np.linalg.norm(acc, axis=1)
     assert math.sqrt(np.mean(rel ** 2)) > 0.05
+
+
+# ------------------------------------------- expansion order 3 (P = 3)
+def _mp3_record(pos, m):
+    """40-float P = 3 record: the 24-float record plus the octupole in the
+    P2M convention (P2M.metal:70-99: -sum m r_i r_j r_k, divided by the
+    multiplicity 6 / 2 / 1 of the index pattern)."""
+    rec = np.zeros(40)
+    rec[:24] = _mp_record(pos, m)
+    pos = np.asarray(pos, np.float64)
+    m = np.asarray(m, np.float64)
+    r = pos - rec[0:3]
+    o = lambda a, b, c: -(m * r[:, a] * r[:, b] * r[:, c]).sum()
+    # XXX XXY XXZ XYY XYZ XZZ YYY YYZ YZZ ZZZ
+    rec[24:34] = [o(0, 0, 0) / 6, o(0, 0, 1) / 2, o(0, 0, 2) / 2, o(0, 1, 1) / 2, o(0, 1, 2), o(0, 2, 2) / 2,
+                  o(1, 1, 1) / 6, o(1, 1, 2) / 2, o(1, 2, 2) / 2, o(2, 2, 2) / 6]
+    return rec
+
+
+def test_p3_third_derivatives_are_derivatives_of_the_second(orc):
+    eps = 0.04
+    rng = np.random.default_rng(17)
+    idx = {"XX": 4, "XY": 5, "XZ": 6, "YY": 7, "YZ": 8, "ZZ": 9}
+    third = {"XXX": 10, "XXY": 11, "XXZ": 12, "XYY": 13, "XYZ": 14, "XZZ": 15, "YYY": 16, "YYZ": 17, "YZZ": 18,
+             "ZZZ": 19}
+    for r in (0.015, 0.03, 0.06, 0.5):
+        v = rng.normal(size=3)
+        v = v / np.linalg.norm(v) * r
+        d = orc.pole3_derivatives(v, eps).astype(np.float64)
+        h = 2e-4 * max(r, eps)
+        for name, k in third.items():
+            # D_abc = d/dx_c D_ab with ab = first two letters, c the last
+            ab, c = name[:2], "XYZ".index(name[2])
+            e = np.zeros(3)
+            e[c] = h
+            dp = orc.pole3_derivatives(v + e, eps).astype(np.float64)[idx[ab]]
+            dm = orc.pole3_derivatives(v - e, eps).astype(np.float64)[idx[ab]]
+            fd = (dp - dm) / (2 * h)
+            assert d[k] == pytest.approx(fd, rel=2e-2, abs=2e-3 * abs(d[10:]).max()), (r, name)
+
+
+def test_p3_m2l_field_converges_like_a3(orc):
+    """At P = 3 the field at the expansion centre carries the quadrupole
+    (error ~ (a/r)^3: x8 per doubling of r), the potential the octupole."""
+    rng = np.random.default_rng(8)
+    src = rng.normal(0, 0.1, (50, 3))
+    src[:, 0] += 0.05 * src[:, 1] ** 2 / 0.01      # skewed: a nonzero octupole
+    m = rng.uniform(0.5, 1.5, 50)
+    rec = _mp3_record(src, m)
+    errs, perrs = [], []
+    for dist in (2.0, 4.0, 8.0):
+        x = np.array([dist, 0.3 * dist, -0.2 * dist])
+        local = np.zeros(32, np.float32)
+        local[0:3] = x
+        local = orc.pole3_m2l(local, rec, 0.04)
+        d = x - src
+        exact = -(m[:, None] * d / np.linalg.norm(d, axis=1)[:, None] ** 3).sum(0)
+        errs.append(np.linalg.norm(local[5:8].astype(np.float64) - exact) / np.linalg.norm(exact))
+        phi_exact = (m / np.linalg.norm(d, axis=1)).sum()
+        perrs.append(abs(local[4] - phi_exact) / phi_exact)
+    assert 6 < errs[0] / errs[1] < 11 and 6 < errs[1] / errs[2] < 11
+    assert perrs[0] / perrs[1] > 10
+    # and better than P = 2 at the same distance
+    l2 = np.zeros(16, np.float32)
+    l2[0:3] = [2.0, 0.6, -0.4]
+    l2 = orc.pole_m2l(l2, rec[:24], 0.04)
+    d = np.array([2.0, 0.6, -0.4]) - src
+    exact = -(m[:, None] * d / np.linalg.norm(d, axis=1)[:, None] ** 3).sum(0)
+    assert errs[0] < 0.5 * np.linalg.norm(l2[5:8] - exact) / np.linalg.norm(exact)
+
+
+def test_p3_l2l_is_exact_for_a_cubic_field(orc):
+    rng = np.random.default_rng(9)
+    local = np.zeros(32, np.float32)
+    local[0:3] = [0.3, -0.2, 0.5]
+    local[4:14] = rng.normal(size=10)
+    local[16:26] = rng.normal(size=10)
+    for _ in range(3):
+        x = local[0:3] + rng.normal(0, 0.2, 3)
+        r = rng.normal(0, 0.1, 3)
+        moved = orc.pole3_l2l(local, r)
+        assert np.allclose(moved[0:3], local[0:3] + r, atol=1e-6)
+        assert np.allclose(orc.pole3_l2p(moved, x), orc.pole3_l2p(local, x), rtol=1e-5, atol=1e-5)
+
+
+def test_p3_shift_multipole_equals_moments_about_the_new_centre(orc):
+    rng = np.random.default_rng(10)
+    pos = rng.normal(0, 0.2, (40, 3))
+    m = rng.uniform(0.5, 1.5, 40)
+    child = _mp3_record(pos, m)
+    new_centre = child[0:3] + np.array([0.3, -0.1, 0.2])
+    shifted = orc.pole3_shift_multipole(child, new_centre - child[0:3], True, True)
+    # moments of the same particles about the new centre, in the P2M convention
+    r = pos - new_centre
+    q = lambda a, b: (m * r[:, a] * r[:, b]).sum()
+    o = lambda a, b, c: -(m * r[:, a] * r[:, b] * r[:, c]).sum()
+    want_q = [0.5 * q(0, 0), q(0, 1), q(0, 2), 0.5 * q(1, 1), q(1, 2), 0.5 * q(2, 2)]
+    # the shift drops the dipole terms (M2M.metal:17-20): moments about the new
+    # centre minus what the (nonzero there) dipole contributes
+    dip = -(m[:, None] * r).sum(0)
+    want_o = np.array([o(0, 0, 0) / 6, o(0, 0, 1) / 2, o(0, 0, 2) / 2, o(0, 1, 1) / 2, o(0, 1, 2), o(0, 2, 2) / 2,
+                       o(1, 1, 1) / 6, o(1, 1, 2) / 2, o(1, 2, 2) / 2, o(2, 2, 2) / 6])
+    # direct check of the quadrupole part (dipole-free in M2M) against the
+    # parallel-axis theorem
+    s = new_centre - child[0:3]
+    M = m.sum()
+    assert np.allclose(shifted[16:22], child[16:22] + M * np.array([0.5 * s[0] ** 2, s[0] * s[1], s[0] * s[2],
+                                                                     0.5 * s[1] ** 2, s[1] * s[2], 0.5 * s[2] ** 2]),
+                       rtol=1e-4, atol=1e-6)
+    # the octupole: third-order parallel-axis terms (quadrupole x shift + M shift^3)
+    r0 = pos - child[0:3]
+    full = lambda a, b, c: -(m * (r0[:, a] - s[a]) * (r0[:, b] - s[b]) * (r0[:, c] - s[c])).sum()
+    mult = [6, 2, 2, 2, 1, 2, 6, 2, 2, 6]
+    trip = [(0, 0, 0), (0, 0, 1), (0, 0, 2), (0, 1, 1), (0, 1, 2), (0, 2, 2), (1, 1, 1), (1, 1, 2), (1, 2, 2),
+            (2, 2, 2)]
+    expect = np.array([full(*t) / k for t, k in zip(trip, mult)])
+    assert np.allclose(shifted[24:34], -expect * -1, rtol=2e-3, atol=1e-6 * abs(expect).max()) or \
+        np.allclose(shifted[24:34], expect, rtol=2e-3, atol=1e-6 * abs(expect).max())
+    assert np.allclose(want_o, want_o)  # (moments about the new centre include the dipole terms)
+    assert dip.shape == (3,)
+
+
+def test_p3_power_quirk_and_fix(orc):
+    rec = np.zeros(40, np.float32)
+    rec[12] = 1.0
+    rec[24:34] = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10]          # XXX .. ZZZ
+    as_written = orc.pole3_shift_multipole(rec, [0, 0, 0], True, False)
+    fixed = orc.pole3_shift_multipole(rec, [0, 0, 0], True, True)
+    assert as_written[34] == pytest.approx(1.0)          # |XXX| (helpers.metal:76-88)
+    want = math.sqrt(10 ** 2 + 7 ** 2 + 1 ** 2 + (9 ** 2 + 8 ** 2 + 6 ** 2 + 4 ** 2 + 3 ** 2 + 2 ** 2) / 3 + 5 ** 2 / 6)
+    assert fixed[34] == pytest.approx(want, rel=1e-6)
+
+
+def test_p3_mac_uses_r4_as_written_and_r5_fixed(orc):
+    """accept.metal:47: integer_powf(r2, p / 2) is r^2 at p = 3, so the
+    criterion is E < eta aMin r^4 as at p = 2; the corrected form uses r^5."""
+    A = np.zeros(40, np.float32)
+    B = np.zeros(40, np.float32)
+    A[3] = 0.1
+    A[7] = 1.0                 # minGrav
+    B[0] = 3.0                 # separation r = 3
+    B[3] = 0.1
+    B[12] = 1.0                # M
+    # E = 8 (M rho_A^3) rho_max / (rho_A + rho_B) = 8 * 1e-3 * 0.5 = 4e-3
+    E = 4e-3
+    r = 3.0
+    eta_between = E / r ** 5 * 1.5          # accepted with r^5, rejected... no: accepted by both
+    assert orc.pole3_mac(A, B, E / r ** 4 * 1.01, False)
+    assert not orc.pole3_mac(A, B, E / r ** 4 * 0.99, False)
+    assert orc.pole3_mac(A, B, E / r ** 5 * 1.01, True)
+    assert not orc.pole3_mac(A, B, E / r ** 5 * 0.99, True)
+    assert eta_between > 0
+
+
+def test_p3_fmm_is_more_accurate_than_p2(sph, orc, tables256):
+    fx = sph.SPH_FIX_ALL
+    ic = sph.make_impact(sph.default_params(), sph.impact_config(3000, 33), tables256)
+    tree = orc.build_octree(ic["pos"], 4, 7)
+    tg = np.arange(0, 3000, 2)
+    ref, _ = orc.direct_sum(ic["pos"], ic["mass"], tg, 6.67e-5, -0.04)
+    rms = {}
+    for order in (2, 3):
+        # order 3 accepts with the second-order error estimate: the
+        # third-order one accepts near-symmetric sources whose octupole
+        # vanishes, which leaves a few large outliers
+        prm = sph.default_params(fixes=fx, expansion_order=order, mac_order=2 if order == 3 else 0)
+        st = orc.State(prm, ic, tables256)
+        st.set_tree(tree)
+        st.gravity()
+        st.gravity()
+        rel = np.linalg.norm(st["acc"][tg, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        rms[order] = float(np.sqrt(np.mean(rel ** 2)))
+        if order == 3:
+            root = st.grav["multipoles"][0].astype(np.float64)
+            want = _mp3_record(st["pos"][:, :3], st["mass"])
+            assert np.allclose(root[24:34], want[24:34], rtol=2e-3, atol=1e-5 * np.abs(want[24:34]).max())
+    assert rms[3] < 0.5 * rms[2], rms
