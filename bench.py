@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--n", type=int, default=None, help="particles (default 1e6; 4e6 with --learned-precision)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--tree", choices=["async", "every", "fixed"], default="async")
     ap.add_argument("--tree-threads", type=int, default=8)
@@ -70,8 +70,15 @@ def parse():
     ap.add_argument("--headon", action="store_true",
                     help="head-on impact variant (impact parameter 0.3, 12 km/s, u0 5e6 J/kg)")
     ap.add_argument("--fixes", type=lambda v: int(v, 0), default=0,
-                    help="SPH_FIX_* bit mask (0 = the reference step as written, 0x1f = every correction)")
-    return ap.parse_args()
+                    help="SPH_FIX_* bit mask (0 = the reference step as written, 0x3f = every correction)")
+    ap.add_argument("--learned-precision", action="store_true",
+                    help="instead of the headline: the learned-h + precision form against the plain FP32 form on "
+                         "a head-on impact (--n, default 4e6 here), README activity rule with up to 8 rungs; "
+                         "prints one JSON line with both forms' steps/s and per-pass times")
+    args = ap.parse_args()
+    if args.n is None:
+        args.n = 4_000_000 if args.learned_precision else 1_000_000
+    return args
 
 
 def apply_overrides(prm, args):
@@ -150,28 +157,7 @@ def learned_comparison(sph, args):
     """
     prm = apply_overrides(sph.default_params(fixes=args.fixes), args)
     prm.density_eta = args.learned_eta
-    dump_steps = 8
-    sizes = ((100_000, 11), (args.n, 12))
-    smp = sph.Samples(sum(n for n, _ in sizes) * dump_steps)
-    t0 = time.perf_counter()
-    for n, seed in sizes:
-        hl = sph.Headless(prm, sph.impact_config(n, seed, headon=args.headon), eos_res=256, tree_mode=sph.TREE_EVERY,
-                          tree_threads=args.tree_threads)
-        hl.step()
-        hl.ctx.set_sample_dump(smp)
-        for _ in range(dump_steps):
-            hl.step()
-        hl.ctx.set_sample_dump(None)
-        hl.close()
-    t_dump = time.perf_counter() - t0
-    log(f"learned: {len(smp)} samples dumped in {t_dump:.1f} s")
-    net = sph.Mlp(seed=13)
-    net.fit_norm(smp)
-    t0 = time.perf_counter()
-    losses = net.train(smp, steps=args.learned_steps, batch=65536, lr=2e-3, seed=14)
-    t_train = time.perf_counter() - t0
-    log(f"learned: {args.learned_steps} Adam steps in {t_train:.2f} s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-
+    net, train = train_guess(sph, args, prm, ((100_000, 11), (args.n, 12)), args.headon)
     hl = sph.Headless(prm, sph.impact_config(args.n, args.seed, headon=args.headon), eos_res=256, tree_mode=sph.TREE_EVERY,
                       tree_threads=args.tree_threads)
     for _ in range(3):
@@ -206,13 +192,129 @@ def learned_comparison(sph, args):
         "nr_iterations_learned": round(out["learned"]["it"], 4),
         "converged_h_rel_diff_max": float(dh.max()),
         "converged_h_rel_diff_q999": float(np.quantile(dh, 0.999)),
-        "train": {"samples": len(smp), "adam_steps": args.learned_steps, "batch": 65536, "lr": 2e-3,
-                  "seconds": round(t_train, 3), "loss_first": float(losses[0]),
-                  "loss_last10_mean": float(np.mean(losses[-10:])), "dump_seconds": round(t_dump, 2)},
+        "train": train,
     }
-    smp.close()
     net.close()
     return res
+
+
+def train_guess(sph, args, prm, sizes, headon, dump_steps=8):
+    """Trains the learned-h network on (features, converged h) dumped by plain
+    runs of the given (n, seed) impacts.  Returns (net, info)."""
+    smp = sph.Samples(sum(n for n, _ in sizes) * dump_steps)
+    t0 = time.perf_counter()
+    for n, seed in sizes:
+        hl = sph.Headless(prm, sph.impact_config(n, seed, headon=headon), eos_res=256, tree_mode=sph.TREE_EVERY,
+                          tree_threads=args.tree_threads)
+        hl.step()
+        hl.ctx.set_sample_dump(smp)
+        for _ in range(dump_steps):
+            hl.step()
+        hl.ctx.set_sample_dump(None)
+        hl.close()
+    t_dump = time.perf_counter() - t0
+    log(f"learned: {len(smp)} samples dumped in {t_dump:.1f} s")
+    net = sph.Mlp(seed=13)
+    net.fit_norm(smp)
+    t0 = time.perf_counter()
+    losses = net.train(smp, steps=args.learned_steps, batch=65536, lr=2e-3, seed=14)
+    t_train = time.perf_counter() - t0
+    log(f"learned: {args.learned_steps} Adam steps in {t_train:.2f} s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    info = {"samples": len(smp), "adam_steps": args.learned_steps, "batch": 65536, "lr": 2e-3,
+            "seconds": round(t_train, 3), "loss_first": float(losses[0]),
+            "loss_last10_mean": float(np.mean(losses[-10:])), "dump_seconds": round(t_dump, 2)}
+    smp.close()
+    return net, info
+
+
+# Algorithmic bytes per particle of the sort phase (hash 20 B, three 8-bit
+# radix passes of 16 B) and the compulsory bytes of the neighbour passes.
+SORT_BYTES_PER_PARTICLE = 68.0
+ACCEL_BYTES_PER_PARTICLE = 93.0
+
+
+def learned_precision_run(sph, args):
+    """The learned-h + precision form against the plain FP32 form, same GPU.
+
+    Workload: head-on impact (impact parameter 0.3 (R_t + R_i), 12 km/s,
+    u0 5e6 J/kg) of --n particles, individual timesteps (README rule, up to 8
+    rungs, leaf/branch activation syncing kept), octree rebuilt on a CPU
+    thread.  Plain: FP32, the reference step as written except for the
+    inactive-particle drift (SPH_FIX_INACTIVE_DRIFT: as written, the README
+    rule drives h of inactive particles to 0 and rho to infinity within a few
+    steps at 1e6).  Learned-h + precision: every correction, compensated sums,
+    FP64 time and conservation sums, third-order gravity at GRAVITY_ETA 1e-3,
+    and the network (trained first on dumps of 1e5 and 1e6 head-on runs of the
+    plain form) as the Newton-Raphson guess.  Both run --warmup untimed then
+    --steps timed steps."""
+    base = dict(activity_rule=sph.SPH_ACTIVE_README, max_rungs=8)
+    forms = {
+        "plain_fp32": sph.default_params(fixes=sph.SPH_FIX_INACTIVE_DRIFT, **base),
+        "learned_precision": sph.default_params(fixes=sph.SPH_FIX_ALL, precision=1, expansion_order=3, mac_order=2,
+                                                gravity_eta=1e-3, max_unchecked_pointers=8192, **base),
+    }
+    for prm in forms.values():
+        apply_overrides(prm, args)
+    net, train = train_guess(sph, args, forms["plain_fp32"], ((100_000, 11), (1_000_000, 12)), True)
+    res = {}
+    for name, prm in forms.items():
+        hl = sph.Headless(prm, sph.impact_config(args.n, 5, headon=True), eos_res=256, tree_mode=sph.TREE_ASYNC,
+                          tree_threads=args.tree_threads)
+        ctx = hl.ctx
+        if name == "learned_precision":
+            ctx.set_learned_guess(net)
+        for i in range(args.warmup):
+            hl.step()
+        ctx.sync()
+        c0 = ctx.conservation() if prm.precision else None
+        ctx.reset_pass_times()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            hl.step()
+            if i % 20 == 0:
+                log(f"{name}: step {i}")
+        ctx.sync()
+        el = time.perf_counter() - t0
+        passes = ctx.pass_times()
+        ctx.set_timing(False)
+        per = {k: v["ms"] / max(v["calls"], 1) for k, v in passes.items()}
+        r = {"steps_per_s": args.steps / el, "ms_per_step": 1000 * el / args.steps,
+             "pass_ms": {k: round(v, 4) for k, v in per.items()},
+             "active_fraction_last": float(ctx.download("active").mean()),
+             "nr_iterations_mean_last": float(ctx.download("nr_iters").mean()),
+             "sort_gbs_algorithmic": SORT_BYTES_PER_PARTICLE * args.n / (per["sort"] * 1e-3) / 1e9,
+             "density_gbs_algorithmic": DENSITY_BYTES_PER_PARTICLE * args.n / (per["density"] * 1e-3) / 1e9,
+             "acceleration_gbs_algorithmic": ACCEL_BYTES_PER_PARTICLE * args.n / (per["acceleration"] * 1e-3) / 1e9}
+        if c0 is not None:
+            c1 = ctx.conservation()
+            e0 = c0["kinetic"] + c0["internal"] + c0["potential"]
+            e1 = c1["kinetic"] + c1["internal"] + c1["potential"]
+            r["energy_drift"] = abs(e1 - e0) / abs(e0)
+            r["angular_momentum_drift"] = float(np.linalg.norm(c1["angular_momentum"] - c0["angular_momentum"]) /
+                                                np.linalg.norm(c0["angular_momentum"]))
+            r["time_s"] = ctx.time()
+        ctx.set_learned_guess(None)
+        hl.close()
+        log(f"{name}: {r['steps_per_s']:.2f} steps/s")
+        res[name] = r
+    net.close()
+    out = {
+        "metric": "steps/s, learned-h + precision form vs plain FP32 form",
+        "value": res["learned_precision"]["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "dtype": "fp32 (fp64 time and conservation sums in the precision form)",
+        "data": "synthetic",
+        "config": {"workload": f"two-body head-on impact, {args.n} particles, README activity rule, up to 8 rungs, "
+                               "CPU octree rebuilt on a thread (async)", "seed": 5},
+        "forms": res,
+        "training": train,
+    }
+    print(json.dumps(out))
 
 
 def copy_probe(nbytes):
@@ -239,6 +341,9 @@ def main():
               "the multi-GPU decomposition of this path is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
     sph = importlib.import_module(PKG)
+    if args.learned_precision:
+        learned_precision_run(sph, args)
+        return
     prm = apply_overrides(sph.default_params(fixes=args.fixes), args)
     tables = sph.make_eos_tables(prm, 256)
     cfg = sph.impact_config(args.n, args.seed, headon=args.headon)

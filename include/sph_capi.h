@@ -51,7 +51,16 @@ extern "C" {
  * its 1/3 and 1/6 weights summed (as written they are integer divisions and
  * three lines assign instead of add) and r^3 in the MAC (as written r^2). */
 #define SPH_FIX_ORDER3        (1u << 5)
-#define SPH_FIX_ALL           (0x3Fu)
+/* step.metal:34-45 drifts every particle with h <- h exp(dh/dt dt / h) and
+ * rho <- rho exp(-3 dh/dt dt / h), dh/dt from the particle's last active step
+ * and h the current one.  Under the README activity rule an inactive particle
+ * with dh/dt < 0 then shrinks faster each step (the rate is divided by an h
+ * that keeps falling) until h reaches 0 and rho infinity, within a few steps
+ * at 1e6 particles.  With this flag the step keeps the log-rate (dh/dt)/h of
+ * the last active step (SPH_F_DLNH) and drifts inactive particles with it;
+ * active particles are integrated exactly as written. */
+#define SPH_FIX_INACTIVE_DRIFT (1u << 6)
+#define SPH_FIX_ALL           (0x7Fu)
 
 /* Particle activity rule used by the gravity up pass (P2M.metal:31). */
 #define SPH_ACTIVE_AS_WRITTEN 0 /* active iff globalTime <= nextActiveTime (every particle, every step) */
@@ -219,6 +228,8 @@ enum sph_field {
                              of the last density pass (diagnostic) */
     SPH_F_POTENTIAL,      /* float x n: gravitational potential (G included) of the last
                              gravity pass in the precision form, else 0 */
+    SPH_F_DLNH,           /* float x n: (dh/dt)/h of each particle's last active step, kept by
+                             the step pass under SPH_FIX_INACTIVE_DRIFT (0 until then) */
     SPH_F_COUNT
 };
 /* Multipole record (24 floats, poles.h:46-55 reordered):

@@ -75,7 +75,9 @@ typedef struct sph_octree {
     int32_t node_count;
 } sph_octree;
 /* `nthreads` <= 1 builds serially; otherwise subtrees are built in parallel.
- * The result is identical for every thread count. */
+ * The result is identical for every thread count.  A non-finite coordinate
+ * returns SPH_E_ARG (octree.cpp:68-86 would put every particle into one
+ * depth-limited leaf). */
 int sph_octree_build(const float* pos4, int32_t n, int32_t max_leaf, int32_t max_depth,
                      int32_t nthreads, sph_octree* out);
 void sph_octree_free(sph_octree* t);

@@ -33,6 +33,7 @@ class OrcState(ctypes.Structure):
         ("cell_array", _P(ctypes.c_uint32)), ("cell_start", _P(ctypes.c_uint32)), ("cell_end", _P(ctypes.c_uint32)),
         ("global_time", _P(ctypes.c_int32)), ("dt", _P(ctypes.c_int32)), ("nr_iters", _P(ctypes.c_int32)),
         ("eos_forsterite", _P(ctypes.c_float)), ("eos_fe", _P(ctypes.c_float)), ("eos_res", ctypes.c_int32),
+        ("dlnh", _P(ctypes.c_float)),
     ]
 
 
@@ -241,7 +242,8 @@ def build_octree(pos4: np.ndarray, max_leaf: int = 4, max_depth: int = 7) -> Dic
     L = lib()
     pos4 = np.ascontiguousarray(pos4, np.float32)
     t = OrcTree()
-    L.orc_build_octree(_p(pos4), pos4.shape[0], max_leaf, max_depth, ctypes.byref(t))
+    if L.orc_build_octree(_p(pos4), pos4.shape[0], max_leaf, max_depth, ctypes.byref(t)) != 0:
+        raise ValueError("build_octree: non-finite particle position")
     try:
         counts = np.ctypeslib.as_array(t.level_counts, shape=(t.n_levels,)).copy()
         return {
@@ -287,6 +289,7 @@ class State:
             "gravnorm": np.zeros(n, np.float32), "cell_array": np.zeros((n, 2), np.uint32),
             "cell_start": np.full(ncells, 0xFFFFFFFF, np.uint32), "cell_end": np.zeros(ncells, np.uint32),
             "global_time": np.zeros(1, np.int32), "dt": np.zeros(1, np.int32), "nr_iters": np.zeros(n, np.int32),
+            "dlnh": np.zeros(n, np.float32),
         }
         self.tab400 = np.ascontiguousarray(tables[400], np.float32)
         self.tab402 = np.ascontiguousarray(tables[402], np.float32)
@@ -308,7 +311,7 @@ class State:
             _p(a["next_active"], ctypes.c_int32), _p(a["gravnorm"]), _p(a["cell_array"], ctypes.c_uint32),
             _p(a["cell_start"], ctypes.c_uint32), _p(a["cell_end"], ctypes.c_uint32),
             _p(a["global_time"], ctypes.c_int32), _p(a["dt"], ctypes.c_int32), _p(a["nr_iters"], ctypes.c_int32),
-            _p(self.tab400), _p(self.tab402), self.tab400.shape[0])
+            _p(self.tab400), _p(self.tab402), self.tab400.shape[0], _p(a["dlnh"]))
 
     def _run(self, name):
         s = self._s()

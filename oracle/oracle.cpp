@@ -1285,8 +1285,14 @@ void orc_step(const sph_params* prm, const orc_state* s) {
         float h_i = s->h[index];
         float h1 = 1 / h_i;
         float dhdt = s->dhdt[index];
-        s->h[index] = std::min(h_i * expf(h1 * dhdt * dt), prm->max_smoothing_length);
-        s->rho[index] *= expf(-3 * h1 * dhdt * dt);
+        float rate = h1 * dhdt;
+        if (prm->fixes & SPH_FIX_INACTIVE_DRIFT) {
+            /* the corrected form: inactive particles keep their last log-rate */
+            if (s->active[index]) s->dlnh[index] = rate;
+            else rate = s->dlnh[index];
+        }
+        s->h[index] = std::min(h_i * expf(rate * dt), prm->max_smoothing_length);
+        s->rho[index] *= expf(-3 * rate * dt);
         st3(s->vel, index, velocity + dt * acceleration);
         s->u[index] += dt * s->du[index];
         st3(s->pos, index, ld3(s->pos, index) + (dt * velocity + 0.5f * acceleration * dt * dt));
@@ -1301,6 +1307,10 @@ int orc_build_octree(const float* pos4, int32_t n, int32_t max_leaf, int32_t max
     int nodeValues = 0;
     std::memset(out, 0, sizeof(*out));
     if (n <= 0) return 0;
+    /* not in octree.cpp: a non-finite coordinate would make every mean split
+     * NaN and put all particles into one depth-limited leaf */
+    for (int64_t i = 0; i < 4 * (int64_t)n; ++i)
+        if ((i & 3) != 3 && !std::isfinite(pos4[i])) return -1;
     std::vector<int> all(n);
     for (int i = 0; i < n; ++i) all[i] = i;
     build_recursive(pos4, all, 0, max_leaf, max_depth, data, levels, nodeValues);

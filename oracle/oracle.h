@@ -54,6 +54,7 @@ typedef struct orc_state {
     const float* eos_forsterite; /* 2*res*res */
     const float* eos_fe;         /* 2*res*res */
     int32_t eos_res;
+    float* dlnh;                 /* n: (dh/dt)/h of the last active step (SPH_FIX_INACTIVE_DRIFT) */
 } orc_state;
 
 /* ---- utils/morton.h, sort.metal ---- */

@@ -28,7 +28,8 @@ SPH_FIX_SOFTENING = 1 << 2
 SPH_FIX_DENSITY_CACHE = 1 << 3
 SPH_FIX_NEAR_FIELD = 1 << 4
 SPH_FIX_ORDER3 = 32
-SPH_FIX_ALL = 0x3F
+SPH_FIX_INACTIVE_DRIFT = 64
+SPH_FIX_ALL = 0x7F
 SPH_ACTIVE_AS_WRITTEN = 0
 SPH_ACTIVE_README = 1
 TREE_ASYNC, TREE_EVERY, TREE_FIXED = 0, 1, 2
@@ -37,7 +38,7 @@ FIELDS = [
     "pos", "vel", "acc", "rho", "u", "mass", "pressure", "h", "mat", "gradterm",
     "soundspeed", "du", "balsara", "dhdt", "active", "next_active", "gravnorm",
     "cell_array", "cell_start", "cell_end", "global_time", "dt", "nr_iters",
-    "multipoles", "locals", "density_work", "potential",
+    "multipoles", "locals", "density_work", "potential", "dlnh",
 ]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 _F4 = {"pos", "vel", "acc"}

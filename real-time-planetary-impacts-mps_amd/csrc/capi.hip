@@ -85,7 +85,7 @@ void destroy_ctx(sph_ctx* ctx) {
                      &ctx->s_posh, &ctx->s_velb, &ctx->s_hyd, &ctx->snap};
     for (auto p : f4) dfree(*p);
     float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c,
-                    &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->phi};
+                    &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->phi, &ctx->dlnh};
     for (auto p : f1) dfree(*p);
     int32_t** i1[] = {&ctx->mat, &ctx->next_active, &ctx->nr_iters, &ctx->time, &ctx->list_a, &ctx->list_b};
     for (auto p : i1) dfree(*p);
@@ -123,7 +123,7 @@ int64_t field_bytes(const sph_ctx* ctx, int field) {
         case SPH_F_RHO: case SPH_F_U: case SPH_F_MASS: case SPH_F_PRESSURE: case SPH_F_H:
         case SPH_F_MAT: case SPH_F_GRADTERM: case SPH_F_SOUNDSPEED: case SPH_F_DU:
         case SPH_F_BALSARA: case SPH_F_DHDT: case SPH_F_NEXT_ACTIVE: case SPH_F_GRAVNORM:
-        case SPH_F_NR_ITERS: case SPH_F_POTENTIAL: return 4 * n;
+        case SPH_F_NR_ITERS: case SPH_F_POTENTIAL: case SPH_F_DLNH: return 4 * n;
         case SPH_F_ACTIVE: return n;
         case SPH_F_CELL_ARRAY: return 8 * n;
         case SPH_F_CELL_START: case SPH_F_CELL_END: return 4 * (int64_t)ctx->ncells;
@@ -159,6 +159,7 @@ void* field_ptr(sph_ctx* ctx, int field) {
         case SPH_F_NR_ITERS: return ctx->nr_iters;
         case SPH_F_DENSITY_WORK: return ctx->density_work;
         case SPH_F_POTENTIAL: return ctx->phi;
+        case SPH_F_DLNH: return ctx->dlnh;
         case SPH_F_CELL_ARRAY: return ctx->pairs_a;
         case SPH_F_GLOBAL_TIME: return ctx->time;
         case SPH_F_DT: return ctx->time + 1;
@@ -266,9 +267,10 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->s_posh, n));
     TRY(dalloc(&ctx->s_velb, n));
     TRY(dalloc(&ctx->s_hyd, n));
-    float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c, &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->phi};
+    float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c, &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->phi, &ctx->dlnh};
     for (auto p : f1) TRY(dalloc(p, n));
     TRY(hipMemset(ctx->phi, 0, 4 * n));
+    TRY(hipMemset(ctx->dlnh, 0, 4 * n));
     TRY(dalloc(&ctx->mat, n));
     TRY(dalloc(&ctx->next_active, n));
     TRY(dalloc(&ctx->nr_iters, n));

@@ -767,8 +767,10 @@ struct StepArgs {
     const uint8_t* active;
     int32_t* next_active;
     int32_t* time;
+    float* dlnh;
     float dtmin1, max_h_len;
     int max_rungs;
+    int fix_drift;
 };
 
 /* step.metal:12-46 */
@@ -782,8 +784,14 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     const float h_i = a.h[index];
     const float h1 = 1.f / h_i;
     const float dhdt = a.dhdt[index];
-    a.h[index] = fminf(h_i * expf(h1 * dhdt * dt), a.max_h_len);
-    a.rho[index] *= expf(-3.f * h1 * dhdt * dt);
+    float rate = h1 * dhdt;
+    if (a.fix_drift) {
+        /* inactive particles keep the log-rate of their last active step */
+        if (a.active[index]) a.dlnh[index] = rate;
+        else rate = a.dlnh[index];
+    }
+    a.h[index] = fminf(h_i * expf(rate * dt), a.max_h_len);
+    a.rho[index] *= expf(-3.f * rate * dt);
     a.vel[index] = make_float4(velocity.x + dt * acceleration.x, velocity.y + dt * acceleration.y,
                                velocity.z + dt * acceleration.z, velocity.w);
     a.u[index] += dt * a.du[index];
@@ -932,6 +940,8 @@ int launch_step(sph_ctx* ctx) {
     a.dtmin1 = p.dt0 / (float)(1 << p.r_max);
     a.max_h_len = p.max_smoothing_length;
     a.max_rungs = p.max_rungs;
+    a.dlnh = ctx->dlnh;
+    a.fix_drift = (p.fixes & SPH_FIX_INACTIVE_DRIFT) ? 1 : 0;
     const int threads = 256;
     k_step<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
     SPH_HIP_TRY(hipGetLastError());

@@ -87,6 +87,7 @@ struct sph_ctx {
     float4 *pos = nullptr, *vel = nullptr, *acc = nullptr;
     float *rho = nullptr, *u = nullptr, *m = nullptr, *P = nullptr, *h = nullptr;
     float *f = nullptr, *c = nullptr, *du = nullptr, *B = nullptr, *dhdt = nullptr, *gravnorm = nullptr;
+    float* dlnh = nullptr; /* (dh/dt)/h of the last active step (SPH_FIX_INACTIVE_DRIFT) */
     float* phi = nullptr; /* potential (precision form) */
     int32_t *mat = nullptr, *next_active = nullptr, *nr_iters = nullptr;
     int4* density_work = nullptr; /* NR (candidates, inside 2h), final pass (candidates, inside) */

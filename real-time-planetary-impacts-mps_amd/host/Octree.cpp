@@ -16,6 +16,7 @@
  * on several threads.  The pieces are then stitched back in pre-order, so the
  * output is identical for every thread count.
  */
+#include <cmath>
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
@@ -160,12 +161,17 @@ int sph_octree_build(const float* pos4, int32_t n, int32_t max_leaf, int32_t max
     w.x.resize(n); w.y.resize(n); w.z.resize(n);
     w.tx.resize(n); w.ty.resize(n); w.tz.resize(n);
     w.id.resize(n); w.tid.resize(n); w.oct.resize(n);
+    bool finite = true;
     for (int i = 0; i < n; ++i) {
         w.x[i] = pos4[4 * i];
         w.y[i] = pos4[4 * i + 1];
         w.z[i] = pos4[4 * i + 2];
         w.id[i] = i;
+        finite = finite && std::isfinite(w.x[i]) && std::isfinite(w.y[i]) && std::isfinite(w.z[i]);
     }
+    /* a non-finite coordinate makes every mean NaN, every `>` false and the
+     * whole set one depth-limited leaf, whose P2P is quadratic in n */
+    if (!finite) return SPH_E_ARG;
     const int threads = nthreads < 1 ? 1 : nthreads;
     const int split_level = threads <= 1 ? 0 : (threads <= 8 ? 2 : 3);
     std::vector<Piece> pieces;

@@ -2,6 +2,8 @@
 
   plain method, 1e5 particles, up to 4 timestep rungs (individual steps):
       ten steps against the oracle;
+  the same with up to 8 rungs and the inactive-drift correction: twelve
+      steps against the oracle;
   precision form, 1e6 particles, 200 steps: gravity RMS against an FP64
       direct sum on 8192 sampled particles (seed 8192), and energy /
       angular-momentum drift over the run.
@@ -35,6 +37,35 @@ def test_plain_1e5_four_rungs_matches_oracle(sph, orc, tables256, impact):
         assert int(ctx.download("global_time")[0]) == int(st["global_time"][0])
     assert len(np.unique(ctx.download("next_active"))) > 1      # particles sit on different rungs
     assert d < 1e-4
+
+
+def test_inactive_drift_fix_1e5_eight_rungs_matches_oracle(sph, orc, tables256, impact):
+    """README rule, up to 8 rungs, SPH_FIX_INACTIVE_DRIFT: twelve steps
+    against the oracle, including the kept log-rates (dlnh); every value
+    stays finite."""
+    n = 100000
+    prm = sph.default_params(activity_rule=sph.SPH_ACTIVE_README, max_rungs=8, fixes=sph.SPH_FIX_INACTIVE_DRIFT)
+    ic = impact(n, 2)
+    ctx = sph.Context(prm, ic, tables256)
+    st = orc.State(prm, ic, tables256)
+    for s in range(12):
+        tree = orc.build_octree(st["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+        ctx.upload_tree(tree)
+        ctx.sort()
+        ctx.gravity()
+        ctx.density()
+        ctx.acceleration()
+        ctx.step()
+        ctx.sync()
+        st.full_step(tree)
+        d = np.abs(ctx.download("pos")[:, :3] - st["pos"][:, :3]).max()
+        g, o = ctx.download("dlnh").astype(np.float64), st["dlnh"].astype(np.float64)
+        dr = np.abs(g - o).max() / max(np.abs(o).max(), 1e-30)
+        print(f"[drift fix] step {s} active {ctx.download('active').mean():.4f} max |dx| {d:.2e} "
+              f"dlnh max rel {dr:.2e} h min {ctx.download('h').min():.3e}", flush=True)
+        assert int(ctx.download("global_time")[0]) == int(st["global_time"][0])
+    assert np.all(np.isfinite(ctx.download("pos"))) and np.all(np.isfinite(ctx.download("rho")))
+    assert d < 1e-4 and dr < 1e-4
 
 
 def test_precision_1e6_rms_and_drift(sph, orc):

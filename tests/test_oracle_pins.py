@@ -383,6 +383,58 @@ def test_step_closed_form(sph, orc, prm, tables256):
     assert st["global_time"][0] == 4096
 
 
+def test_inactive_drift_as_written_runs_away_and_the_fix_is_geometric(sph, orc, tables256):
+    """step.metal:34-45 divides a stale dh/dt by the current h: an inactive
+    particle with dh/dt < 0 shrinks faster every step (h_{k+1} = h_k
+    exp(dh/dt dt / h_k)).  With SPH_FIX_INACTIVE_DRIFT the log-rate of the
+    last active step is kept, so k inactive steps give h_0 exp(k rate dt) and
+    rho_0 exp(-3 k rate dt); the active step itself is the as-written one."""
+    n = 200
+    out = {}
+    for fixes in (0, sph.SPH_FIX_INACTIVE_DRIFT):
+        prm = sph.default_params(fixes=fixes)
+        ic = sph.make_impact(prm, sph.impact_config(n, 25), tables256)
+        st = orc.State(prm, ic, tables256)
+        st["acc"] = np.zeros((n, 4), np.float32)
+        st["du"] = np.zeros(n, np.float32)
+        st["dhdt"] = np.full(n, -0.05, np.float32)
+        st["h"] = np.full(n, 0.08, np.float32)
+        st["rho"] = np.full(n, 0.02, np.float32)
+        st["dt"] = 2048                       # 1 s
+        st["active"] = np.ones(n, np.uint8)
+        st.step()                              # the active step: as written in both forms
+        hs = [st["h"][0]]
+        st["active"] = np.zeros(n, np.uint8)
+        for _ in range(3):
+            st.step()
+            hs.append(st["h"][0])
+        out[fixes] = (np.array(hs, np.float64), float(st["rho"][0]))
+    dt = 2048 * 32.0 / 65536
+    rate = -0.05 / 0.08
+    h_fix, rho_fix = out[sph.SPH_FIX_INACTIVE_DRIFT]
+    assert np.allclose(h_fix, 0.08 * np.exp(rate * dt * np.arange(1, 5)), rtol=1e-5)
+    assert rho_fix == pytest.approx(0.02 * np.exp(-3 * 4 * rate * dt), rel=1e-5)
+    h_aw, _ = out[0]
+    assert h_aw[0] == pytest.approx(h_fix[0], rel=1e-7)
+    want = [0.08]
+    for _ in range(4):
+        want.append(want[-1] * np.exp(-0.05 * dt / want[-1]))
+    assert np.allclose(h_aw, want[1:], rtol=1e-4)
+    assert h_aw[-1] < 1e-3 * h_fix[-1]          # the runaway
+
+
+def test_octree_rejects_non_finite_positions(sph, orc):
+    pos = np.random.default_rng(3).normal(size=(100, 4)).astype(np.float32)
+    pos[17, 1] = np.nan
+    with pytest.raises(ValueError, match="non-finite"):
+        orc.build_octree(pos, 4, 7)
+    with pytest.raises(sph.SphError):
+        sph.octree_build(pos, 4, 7, 4)
+    pos[17, 1] = np.inf
+    with pytest.raises(sph.SphError):
+        sph.octree_build(pos, 4, 7, 1)
+
+
 # -------------------------------------------------------------- a16 octree
 def _walk(tree, n):
     oc = tree["octree"]
