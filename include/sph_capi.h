@@ -57,9 +57,14 @@ extern "C" {
  * with dh/dt < 0 then shrinks faster each step (the rate is divided by an h
  * that keeps falling) until h reaches 0 and rho infinity, within a few steps
  * at 1e6 particles.  With this flag the step keeps the log-rate (dh/dt)/h of
- * the last active step (SPH_F_DLNH) and drifts inactive particles with it;
- * active particles are integrated exactly as written. */
+ * the last active step (SPH_F_DLNH) and drifts inactive particles with it,
+ * and the timestep goal also bounds |dh/dt| dt / h (SPH_DLNH_LIMIT); active
+ * particles are drifted exactly as written. */
 #define SPH_FIX_INACTIVE_DRIFT (1u << 6)
+/* With SPH_FIX_INACTIVE_DRIFT the timestep goal (accelerations.metal:130-136)
+ * is also bounded by SPH_DLNH_LIMIT h / |dh/dt|: the kept log-rate then moves
+ * h by at most exp(SPH_DLNH_LIMIT) before the particle is recomputed. */
+#define SPH_DLNH_LIMIT 0.25f
 #define SPH_FIX_ALL           (0x7Fu)
 
 /* Particle activity rule used by the gravity up pass (P2M.metal:31). */

@@ -1257,6 +1257,8 @@ void orc_acceleration(const sph_params* prm, const orc_state* s) {
         s->du[ind] = du_dt;
         s->dhdt[ind] = dhdt;
         float goaldt = 2.0f * prm->cfl * prm->gamma * h_i / v_sigi;
+        /* the corrected form also bounds |dh/dt| dt / h by SPH_DLNH_LIMIT */
+        if (prm->fixes & SPH_FIX_INACTIVE_DRIFT) goaldt = std::min(goaldt, SPH_DLNH_LIMIT * h_i / fabsf(dhdt));
         int r = 0;
         float dtCandidate = prm->dt0;
         while (dtCandidate > goaldt && r < prm->r_max) {

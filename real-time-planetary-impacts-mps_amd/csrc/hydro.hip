@@ -552,7 +552,18 @@ struct AccelArgs {
     int32_t* time;
     float alpha, cfl, gamma, dt0, dtmin, max_h_len;
     int r_max;
+    float dlnh_limit; /* > 0: SPH_FIX_INACTIVE_DRIFT's bound on |dh/dt| dt / h */
 };
+
+/* accelerations.metal:130-136: the step goal 2 CFL Gamma h / v_sig; with
+ * SPH_FIX_INACTIVE_DRIFT also dlnh_limit h / |dh/dt|, so that no particle's
+ * h or rho drifts by more than exp(dlnh_limit) (exp(3 dlnh_limit)) before it
+ * is recomputed. */
+__device__ __forceinline__ float step_goal(const AccelArgs& a, float h_i, float v_sigi, float dhdt) {
+    float goaldt = 2.0f * a.cfl * a.gamma * h_i / v_sigi;
+    if (a.dlnh_limit > 0.f) goaldt = fminf(goaldt, a.dlnh_limit * h_i / fabsf(dhdt));
+    return goaldt;
+}
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -637,7 +648,7 @@ __global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
             a.acc[ind] = ac;
             a.du[ind] = du_dt;
             a.dhdt[ind] = dhdt;
-            const float goaldt = 2.0f * a.cfl * a.gamma * h_i / v_sigi;
+            const float goaldt = step_goal(a, h_i, v_sigi, dhdt);
             int r = 0;
             float dtCandidate = a.dt0;
             while (dtCandidate > goaldt && r < a.r_max) {
@@ -743,7 +754,7 @@ __global__ __launch_bounds__(256) void k_acceleration_group(AccelArgs a) {
             a.acc[ind] = ac;
             a.du[ind] = du_dt;
             a.dhdt[ind] = dhdt;
-            const float goaldt = 2.0f * a.cfl * a.gamma * h_i / v_sigi;
+            const float goaldt = step_goal(a, h_i, v_sigi, dhdt);
             int r = 0;
             float dtCandidate = a.dt0;
             while (dtCandidate > goaldt && r < a.r_max) {
@@ -894,6 +905,7 @@ int launch_acceleration(sph_ctx* ctx) {
     a.alpha = p.viscosity_alpha;
     a.cfl = p.cfl;
     a.gamma = p.gamma;
+    a.dlnh_limit = (p.fixes & SPH_FIX_INACTIVE_DRIFT) ? SPH_DLNH_LIMIT : 0.f;
     a.dt0 = p.dt0;
     const float dtmin1 = p.dt0 / (float)(1 << p.r_max);
     a.dtmin = 1.f / dtmin1;

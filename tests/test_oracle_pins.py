@@ -423,6 +423,31 @@ def test_inactive_drift_as_written_runs_away_and_the_fix_is_geometric(sph, orc, 
     assert h_aw[-1] < 1e-3 * h_fix[-1]          # the runaway
 
 
+def test_drift_fix_bounds_the_log_rate_per_own_step(sph, orc, tables256):
+    """With SPH_FIX_INACTIVE_DRIFT every particle's own step (next_active -
+    globalTime ticks) satisfies |dh/dt| dt / h <= SPH_DLNH_LIMIT (0.25) unless
+    it already sits on the finest rung; as written some particles of a
+    strongly stirred impact exceed it."""
+    n = 3000
+    worst = {}
+    for fixes in (0, sph.SPH_FIX_INACTIVE_DRIFT):
+        prm = sph.default_params(fixes=fixes)
+        ic = sph.make_impact(prm, sph.impact_config(n, 26), tables256)
+        rng = np.random.default_rng(26)
+        ic["vel"][:, :3] += rng.normal(0, 0.02, (n, 3)).astype(np.float32)
+        st = orc.State(prm, ic, tables256)
+        st.sort()
+        st.density()
+        st.acceleration()
+        ticks = (st["next_active"] - st["global_time"][0]).astype(np.float64)
+        dt = ticks * prm.dt0 / 2 ** prm.r_max
+        x = np.abs(st["dhdt"].astype(np.float64)) * dt / st["h"].astype(np.float64)
+        free = ticks > 1
+        worst[fixes] = float(x[free].max())
+    assert worst[sph.SPH_FIX_INACTIVE_DRIFT] <= 0.25 * (1 + 1e-5)
+    assert worst[0] > 0.25
+
+
 def test_octree_rejects_non_finite_positions(sph, orc):
     pos = np.random.default_rng(3).normal(size=(100, 4)).astype(np.float32)
     pos[17, 1] = np.nan
