@@ -301,6 +301,8 @@ class State:
 
     def __setitem__(self, k, v):
         self.a[k][...] = v
+        if k == "gravnorm":
+            self._grav_known = True
 
     def _s(self) -> OrcState:
         a = self.a
@@ -358,8 +360,26 @@ class State:
         lib().orc_gravity_down(_pp(self.params), ctypes.byref(s), ctypes.byref(ot), ctypes.byref(gb))
 
     def gravity(self):
+        """gravity.metal upPass + downPass.  The near-field form first
+        estimates |a| with one as-written pass (P = 2, leaves force-accept)
+        while no field is known, as the GPU path does: with gravNorm = 0 every
+        MAC test rejects and the near field would sum every pair."""
+        if (self.params.fixes & 16) and not getattr(self, "_grav_known", False):
+            real = self.params
+            boot = type(real).from_buffer_copy(real)
+            boot.fixes = real.fixes & ~16
+            boot.expansion_order = 2
+            boot.mac_order = 0
+            boot.precision = 0
+            self.params = boot
+            try:
+                self.gravity_up()
+                self.gravity_down()
+            finally:
+                self.params = real
         self.gravity_up()
         self.gravity_down()
+        self._grav_known = True
 
     def full_step(self, tree: Optional[Dict[str, object]] = None):
         """Headless::step order: sort, gravity, density, acceleration, step."""

@@ -496,11 +496,21 @@ int sph_gravity(sph_ctx* ctx) {
     /* the near-field form checks the node numbering in the up pass */
     if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ctx->tree_check_pending = true;
     int rc = ctx_begin_pass(ctx, SPH_PASS_GRAVITY_UP);
+    /* With gravNorm = 0 every MAC test rejects (minGrav = 0, Q6).  As written
+     * the leaves then force-accept; in the near-field form every pair would be
+     * summed directly, O(n^2).  Before its first evaluation the near-field form
+     * therefore estimates |a| with one as-written pass (P = 2, leaves
+     * force-accept) and rebuilds the multipoles' minGrav from it. */
+    if (rc == SPH_OK && (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) && !ctx->gravnorm_known) {
+        rc = launch_gravity_up(ctx);
+        if (rc == SPH_OK) rc = launch_gravity_down(ctx, true);
+    }
     if (rc == SPH_OK) rc = launch_gravity_up(ctx);
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_GRAVITY_UP);
     if (rc == SPH_OK) rc = ctx_begin_pass(ctx, SPH_PASS_GRAVITY_DOWN);
     if (rc == SPH_OK) rc = launch_gravity_down(ctx);
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_GRAVITY_DOWN);
+    if (rc == SPH_OK) ctx->gravnorm_known = true;
     return rc;
 }
 
@@ -630,6 +640,7 @@ int sph_upload(sph_ctx* ctx, int field, const void* src) {
         return ctx_fail(ctx, SPH_E_ARG, "derived field cannot be uploaded");
     void* dst = field_ptr(ctx, field);
     if (!dst) return ctx_fail(ctx, SPH_E_ARG, "field not uploadable");
+    if (field == SPH_F_GRAVNORM) ctx->gravnorm_known = true;
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     SPH_HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
     return SPH_OK;

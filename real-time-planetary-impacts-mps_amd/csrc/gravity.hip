@@ -1254,7 +1254,7 @@ int launch_gravity_up(sph_ctx* ctx) {
     return SPH_OK;
 }
 
-int launch_gravity_down(sph_ctx* ctx) {
+int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
     sph_tree_dev& T = ctx->tree;
     const sph_params& p = ctx->prm;
     DownArgs a;
@@ -1266,11 +1266,11 @@ int launch_gravity_down(sph_ctx* ctx) {
     a.eps = p.gravity_softening;
     a.G = p.gravity_g;
     a.fix_soft = (p.fixes & SPH_FIX_SOFTENING) ? 1 : 0;
-    a.near = (p.fixes & SPH_FIX_NEAR_FIELD) ? 1 : 0;
+    a.near = (p.fixes & SPH_FIX_NEAR_FIELD) && !bootstrap ? 1 : 0;
     /* the precision form (near field required, checked at creation) also
      * writes the potential */
     a.phi = (p.precision && a.near) ? ctx->phi : nullptr;
-    const bool order3 = p.expansion_order == 3;
+    const bool order3 = p.expansion_order == 3 && !bootstrap;
     a.macp = (order3 && p.mac_order != 2) ? 3 : 2;
     a.fix_p3 = (p.fixes & SPH_FIX_ORDER3) ? 1 : 0;
     a.pos = ctx->pos;

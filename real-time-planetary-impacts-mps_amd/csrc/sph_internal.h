@@ -118,6 +118,8 @@ struct sph_ctx {
     int64_t list_cap = 0; /* entries per list buffer */
     uint32_t* grav_flags = nullptr; /* [0] != 0: the last uploaded tree failed validation */
     bool tree_check_pending = false;
+    /* gravnorm holds a field (a gravity pass ran or it was uploaded) */
+    bool gravnorm_known = false;
 
     /* octree handoff */
     float4* snap = nullptr;
@@ -162,7 +164,9 @@ int launch_density_epilogue(sph_ctx* ctx);
 /* gravity.hip */
 int launch_tree_convert(sph_ctx* ctx);
 int launch_gravity_up(sph_ctx* ctx);
-int launch_gravity_down(sph_ctx* ctx);
+/* bootstrap: the as-written P = 2 down pass (leaves force-accept), used by
+ * the near-field form to estimate |a| before its first evaluation */
+int launch_gravity_down(sph_ctx* ctx, bool bootstrap = false);
 int launch_fill_cells(sph_ctx* ctx);
 
 /* ---------------------------------------------------------- device helpers */
