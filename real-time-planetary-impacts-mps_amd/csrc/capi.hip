@@ -99,6 +99,8 @@ void destroy_ctx(sph_ctx* ctx) {
     dfree(ctx->digit_total);
     dfree(ctx->feats);
     dfree(ctx->guess);
+    dfree(ctx->slot_list);
+    dfree(ctx->slot_count);
     dfree(ctx->eos[0]);
     dfree(ctx->eos[1]);
     free_tree(ctx->tree);

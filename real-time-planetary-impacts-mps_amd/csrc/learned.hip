@@ -12,7 +12,8 @@
  *              d = log(h_converged) - log(h_start) on normalised features
  *   training   weighted MSE, backward on the matrix cores (dW2, dA1, dW1 are
  *              16x16x32 MFMA products over 32 samples), per-wave partial
- *              gradients reduced in a fixed order, Adam in FP32
+ *              gradients summed per workgroup, then over the workgroups, in
+ *              a fixed order; Adam in FP32
  *   samples    16-float records (12 features, d, weight, 2 spare) appended
  *              by density passes with dumping on
  *
@@ -25,6 +26,7 @@
  * Transposes between the C and A/B layouts go through LDS; all waves of a
  * workgroup run the same number of iterations, so block barriers order them.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -45,7 +47,8 @@ constexpr int S = 32;    /* samples per wave iteration */
 constexpr int P64 = 72;  /* LDS pitch (bf16) of 64-wide rows: +16 B against bank conflicts */
 constexpr int P32 = 40;  /* LDS pitch (bf16) of 32-wide rows */
 constexpr int WAVES = 4; /* per workgroup */
-constexpr int TRAIN_BLOCKS = 64;
+constexpr int TRAIN_BLOCKS = 256;  /* one workgroup per CU (the training LDS is ~130 KB) */
+constexpr int INFER_BLOCKS = 2048; /* inference: ~35 KB LDS, several workgroups per CU */
 constexpr float D_CLAMP = 0.5f; /* |log(h_guess / h_start)| cap */
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -66,6 +69,18 @@ struct WeightsLds {
     float norm[SPH_MLP_NORM];
 };
 
+/* Inference needs only the forward operands. */
+struct WeightsInf {
+    __bf16 w1[NH * P32];
+    __bf16 w2[NH * P64];
+    float b1[NH], b2[NH], w3[NH];
+    float b3;
+    float norm[SPH_MLP_NORM];
+};
+struct WaveInf {
+    __bf16 a1[S * P64];
+};
+
 struct WaveLds {
     __bf16 a1[S * P64];    /* [s][k] */
     __bf16 a1t[NH * P32];  /* [k][s] */
@@ -76,7 +91,8 @@ struct WaveLds {
     float tgt[S], wgt[S];
 };
 
-__device__ void stage_weights(WeightsLds& w, const float* __restrict__ p, const float* __restrict__ norm, bool train) {
+template <bool TRAIN, class W>
+__device__ void stage_weights(W& w, const float* __restrict__ p, const float* __restrict__ norm) {
     for (int e = threadIdx.x; e < NH * 32; e += blockDim.x) {
         const int j = e / 32, k = e % 32;
         w.w1[j * P32 + k] = (__bf16)(k < NI ? p[OW1 + j * NI + k] : 0.f);
@@ -85,7 +101,7 @@ __device__ void stage_weights(WeightsLds& w, const float* __restrict__ p, const 
         const int j = e / NH, k = e % NH;
         const float v = p[OW2 + e];
         w.w2[j * P64 + k] = (__bf16)v;
-        if (train) w.w2t[k * P64 + j] = (__bf16)v;
+        if constexpr (TRAIN) w.w2t[k * P64 + j] = (__bf16)v;
     }
     for (int e = threadIdx.x; e < NH; e += blockDim.x) {
         w.b1[e] = p[OB1 + e];
@@ -100,8 +116,8 @@ __device__ void stage_weights(WeightsLds& w, const float* __restrict__ p, const 
  * (normalised features of sample 16 m + c, features 8g .. 8g+7).  Leaves
  * z1, z2 in C layout, a1 in LDS (row-major, and transposed if `train`), and
  * y[m][r] = network output of sample 16 m + 4 g + r in every lane of group g. */
-template <bool TRAIN>
-__device__ __forceinline__ void forward(const WeightsLds& w, WaveLds& l, const bf16x8 xa[2], int c, int g,
+template <bool TRAIN, class W, class L>
+__device__ __forceinline__ void forward(const W& w, L& l, const bf16x8 xa[2], int c, int g,
                                         f32x4 z1[2][4], f32x4 z2[2][4], float y[2][4]) {
 #pragma unroll
     for (int m = 0; m < 2; ++m)
@@ -115,7 +131,7 @@ __device__ __forceinline__ void forward(const WeightsLds& w, WaveLds& l, const b
                 const __bf16 a = (__bf16)(z * sig(z));
                 const int s = m * 16 + g * 4 + r, k = t * 16 + c;
                 l.a1[s * P64 + k] = a;
-                if (TRAIN) l.a1t[k * P32 + s] = a;
+                if constexpr (TRAIN) l.a1t[k * P32 + s] = a;
             }
         }
     __syncthreads();
@@ -161,8 +177,8 @@ struct TrainArgs {
     int step;
     const float* params;
     const float* norm;
-    float* part;         /* [waves][NP] */
-    float* loss_part;    /* [waves] */
+    float* part;         /* [workgroups][NP] */
+    float* loss_part;    /* [workgroups] */
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -176,7 +192,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 __global__ __launch_bounds__(256) void k_train(TrainArgs a) {
     __shared__ WeightsLds w;
     __shared__ WaveLds wl[WAVES];
-    stage_weights(w, a.params, a.norm, true);
+    stage_weights<true>(w, a.params, a.norm);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
@@ -310,8 +326,13 @@ __global__ __launch_bounds__(256) void k_train(TrainArgs a) {
         __syncthreads();
     }
 
-    /* per-wave partial gradient, every parameter written once */
-    float* out = a.part + (int64_t)gw * NP;
+    /* per-wave partial gradient into LDS (the wave buffers are free now),
+     * every parameter written once; then the workgroup's partial, the sum
+     * of its waves in wave order */
+    static_assert(sizeof(WaveLds) * WAVES >= sizeof(float) * (WAVES * NP + WAVES), "reduction buffer");
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(&wl[0]);
+    float* out = red + wave * NP;
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
@@ -343,12 +364,26 @@ __global__ __launch_bounds__(256) void k_train(TrainArgs a) {
     }
     if (lane == 0) {
         out[OB3] = gb3;
-        a.loss_part[gw] = loss * inv_b;
+        red[WAVES * NP + wave] = loss * inv_b;
+    }
+    __syncthreads();
+    float* bout = a.part + (int64_t)blockIdx.x * NP;
+    for (int i = threadIdx.x; i < NP; i += blockDim.x) {
+        float v = red[i];
+#pragma unroll
+        for (int k = 1; k < WAVES; ++k) v += red[k * NP + i];
+        bout[i] = v;
+    }
+    if (threadIdx.x == 0) {
+        float L = red[WAVES * NP];
+        for (int k = 1; k < WAVES; ++k) L += red[WAVES * NP + k];
+        a.loss_part[blockIdx.x] = L;
     }
 }
 
-/* Sum of the per-wave partials in wave order, then one Adam update
- * (Kingma & Ba 2015, algorithm 1) when `lr` > 0. */
+/* Sum of the workgroup partials (four interleaved running sums, combined
+ * in a fixed order), then one Adam update (Kingma & Ba 2015, algorithm 1)
+ * when `lr` > 0. */
 __global__ void k_reduce_adam(const float* __restrict__ part, const float* __restrict__ loss_part, int waves,
                               float* __restrict__ p, float* __restrict__ mom1, float* __restrict__ mom2,
                               float* __restrict__ grad, float* __restrict__ loss_out, float lr, float bc1, float bc2) {
@@ -359,8 +394,16 @@ __global__ void k_reduce_adam(const float* __restrict__ part, const float* __res
         loss_out[0] = L;
     }
     if (i >= NP) return;
-    float gsum = 0.f;
-    for (int k = 0; k < waves; ++k) gsum += part[(int64_t)k * NP + i];
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int k = 0;
+    for (; k + 3 < waves; k += 4) {
+        s0 += part[(int64_t)k * NP + i];
+        s1 += part[(int64_t)(k + 1) * NP + i];
+        s2 += part[(int64_t)(k + 2) * NP + i];
+        s3 += part[(int64_t)(k + 3) * NP + i];
+    }
+    for (; k < waves; ++k) s0 += part[(int64_t)k * NP + i];
+    const float gsum = (s0 + s1) + (s2 + s3);
     grad[i] = gsum;
     if (lr <= 0.f) return;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -378,66 +421,85 @@ struct InferArgs {
     const float* params;
     const float* norm;
     float* d_out;         /* raw prediction per row (may be null) */
-    /* guess mode: row = sorted slot */
+    /* guess mode: row = sorted slot, or list[row] when `list` is set (then
+     * the row count is read from `count` on the device) */
+    const int32_t* list;
+    const int32_t* count;
     const uint2* sorted;
     const float* h;
     float max_h;
     float* guess;         /* h_start exp(clamp(d)), per sorted slot (may be null) */
 };
 
+/* Workgroups loop over 32-sample tiles (one per wave per round, the round
+ * count uniform over the workgroup because forward() has block barriers),
+ * so the weights are staged once per workgroup. */
 __global__ __launch_bounds__(256) void k_infer(InferArgs a) {
-    __shared__ WeightsLds w;
-    __shared__ WaveLds wl[WAVES];
-    stage_weights(w, a.params, a.norm, false);
+    __shared__ WeightsInf w;
+    __shared__ WaveInf wl[WAVES];
+    stage_weights<false>(w, a.params, a.norm);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
-    const int64_t base = ((int64_t)blockIdx.x * WAVES + wave) * S;
-    WaveLds& l = wl[wave];
-    bf16x8 xa[2];
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const int64_t row = base + m * 16 + c;
-        const bool ok = row < a.rows;
-        float x[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) x[e] = 0.f;
-        if (ok) {
-            if (g == 0) {
-                const float4 v0 = a.feats[row * 3], v1 = a.feats[row * 3 + 1];
-                const float raw[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) x[e] = (raw[e] - w.norm[e]) / w.norm[NI + e];
-            } else if (g == 1) {
-                const float4 v2 = a.feats[row * 3 + 2];
-                const float raw[4] = {v2.x, v2.y, v2.z, v2.w};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) x[e] = (raw[e] - w.norm[8 + e]) / w.norm[NI + 8 + e];
-            }
-        }
-        bf16x8 v;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = (__bf16)x[e];
-        xa[m] = v;
-    }
-    f32x4 z1[2][4], z2[2][4];
-    float y[2][4];
-    forward<false>(w, l, xa, c, g, z1, z2, y);
-    if (c != 0) return;
+    WaveInf& l = wl[wave];
     const float ymu = w.norm[2 * NI], ysig = w.norm[2 * NI + 1];
+    const int64_t rows = a.count ? (int64_t)a.count[0] : a.rows;
+    const int64_t tiles = (rows + S - 1) / S;
+    for (int64_t t0 = (int64_t)blockIdx.x * WAVES; t0 < tiles; t0 += (int64_t)gridDim.x * WAVES) {
+        const int64_t base = (t0 + wave) * S;
+        bf16x8 xa[2];
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < 2; ++m) {
+            const int64_t r0 = base + m * 16 + c;
+            const bool ok = r0 < rows;
+            const int64_t row = ok && a.list ? (int64_t)a.list[r0] : r0;
+            float x[8];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t row = base + m * 16 + g * 4 + r;
-            if (row >= a.rows) continue;
-            const float d = y[m][r] * ysig + ymu;
-            if (a.d_out) a.d_out[row] = d;
-            if (a.guess) {
-                const float h0 = a.h[a.sorted[row].y];
-                a.guess[row] = fminf(h0 * expf(fminf(fmaxf(d, -D_CLAMP), D_CLAMP)), a.max_h);
+            for (int e = 0; e < 8; ++e) x[e] = 0.f;
+            if (ok) {
+                if (g == 0) {
+                    const float4 v0 = a.feats[row * 3], v1 = a.feats[row * 3 + 1];
+                    const float raw[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) x[e] = (raw[e] - w.norm[e]) / w.norm[NI + e];
+                } else if (g == 1) {
+                    const float4 v2 = a.feats[row * 3 + 2];
+                    const float raw[4] = {v2.x, v2.y, v2.z, v2.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) x[e] = (raw[e] - w.norm[8 + e]) / w.norm[NI + 8 + e];
+                }
             }
+            bf16x8 v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (__bf16)x[e];
+            xa[m] = v;
         }
+        f32x4 z1[2][4], z2[2][4];
+        float y[2][4];
+        forward<false>(w, l, xa, c, g, z1, z2, y);
+        if (c == 0) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t r0 = base + m * 16 + g * 4 + r;
+                    if (r0 >= rows) continue;
+                    const int64_t row = a.list ? (int64_t)a.list[r0] : r0;
+                    const float d = y[m][r] * ysig + ymu;
+                    if (a.d_out) a.d_out[row] = d;
+                    if (a.guess) {
+                        const float h0 = a.h[a.sorted[row].y];
+                        a.guess[row] = fminf(h0 * expf(fminf(fmaxf(d, -D_CLAMP), D_CLAMP)), a.max_h);
+                    }
+                }
+        }
+        __syncthreads();  /* the next round rewrites the wave's a1 tile */
+    }
+}
+
+unsigned infer_blocks(int64_t rows) {
+    const int64_t tiles = (rows + S - 1) / S;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(INFER_BLOCKS, (tiles + WAVES - 1) / WAVES));
 }
 
 /* ------------------------------------------------------------- features */
@@ -462,11 +524,14 @@ struct FeatArgs {
     const int32_t* time;
     float dtmin1;
     float4* feats;        /* 3 per sorted slot */
+    const int32_t* list;  /* optional: only these sorted slots, count[0] of them */
+    const int32_t* count;
 };
 
 __global__ __launch_bounds__(256) void k_features(FeatArgs a) {
-    const int index = blockIdx.x * blockDim.x + threadIdx.x;
-    if (index >= a.n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (a.list ? a.count[0] : a.n)) return;
+    const int index = a.list ? a.list[t] : t;
     const uint2 kp = a.sorted[index];
     const int cx = (int)compact_bits(kp.x), cy = (int)compact_bits(kp.x >> 1), cz = (int)compact_bits(kp.x >> 2);
     float tot = 0.f, own = 0.f, face = 0.f, edge = 0.f, corner = 0.f, mx = 0.f, mn = 3.0e38f;
@@ -561,7 +626,7 @@ struct sph_mlp {
     float* mom2 = nullptr;
     float* norm = nullptr;
     float* grad = nullptr;
-    float* part = nullptr;       /* [waves][NP] */
+    float* part = nullptr;       /* [workgroups][NP] */
     float* loss_part = nullptr;
     float* loss = nullptr;       /* per training step */
     int loss_cap = 0;
@@ -626,13 +691,13 @@ int ensure_loss(sph_mlp* m, int count) {
     return SPH_OK;
 }
 
-int ensure_part(sph_mlp* m, int waves) {
-    if (waves <= m->part_waves) return SPH_OK;
+int ensure_part(sph_mlp* m, int parts) {
+    if (parts <= m->part_waves) return SPH_OK;
     dfree(m->part);
     dfree(m->loss_part);
-    MLP_TRY(dalloc(&m->part, (size_t)waves * NP));
-    MLP_TRY(dalloc(&m->loss_part, (size_t)waves));
-    m->part_waves = waves;
+    MLP_TRY(dalloc(&m->part, (size_t)parts * NP));
+    MLP_TRY(dalloc(&m->loss_part, (size_t)parts));
+    m->part_waves = parts;
     return SPH_OK;
 }
 
@@ -642,7 +707,7 @@ int run_step(sph_mlp* m, const float4* rec, int64_t count, int batch, int64_t no
     const int chunks = batch / S;
     const int blocks = std::min(TRAIN_BLOCKS, (chunks + WAVES - 1) / WAVES);
     const int waves = blocks * WAVES;
-    int rc = ensure_part(m, waves);
+    int rc = ensure_part(m, blocks);
     if (rc != SPH_OK) return rc;
     TrainArgs a;
     a.rec = rec;
@@ -665,7 +730,7 @@ int run_step(sph_mlp* m, const float4* rec, int64_t count, int batch, int64_t no
         bc1 = (float)(1.0 - std::pow(0.9, (double)m->adam_t));
         bc2 = (float)(1.0 - std::pow(0.999, (double)m->adam_t));
     }
-    k_reduce_adam<<<(NP + 255) / 256, 256, 0, m->stream>>>(m->part, m->loss_part, waves, m->params, m->mom1,
+    k_reduce_adam<<<(NP + 255) / 256, 256, 0, m->stream>>>(m->part, m->loss_part, blocks, m->params, m->mom1,
                                                            m->mom2, m->grad, loss_dst, lr, bc1, bc2);
     MLP_TRY(hipGetLastError());
     return SPH_OK;
@@ -788,8 +853,7 @@ int sph_mlp_forward(sph_mlp* m, const float* x, int64_t n, float* d_out) {
     a.params = m->params;
     a.norm = m->norm;
     a.d_out = m->scratch_d;
-    const int64_t blocks = (n + WAVES * S - 1) / (WAVES * S);
-    k_infer<<<(unsigned)blocks, WAVES * 64, 0, m->stream>>>(a);
+    k_infer<<<infer_blocks(n), WAVES * 64, 0, m->stream>>>(a);
     MLP_TRY(hipGetLastError());
     MLP_TRY(hipMemcpyAsync(d_out, m->scratch_d, (size_t)n * 4, hipMemcpyDeviceToHost, m->stream));
     MLP_TRY(hipStreamSynchronize(m->stream));
@@ -928,9 +992,39 @@ int sph_samples_clear(sph_samples* s) {
 }  // extern "C"
 
 /* --------------------------------------------------- density-pass hooks */
-int launch_features(sph_ctx* ctx) {
+/* Sorted slots of the active particles: one ballot and one counter update
+ * per wave (the list order across waves is not fixed; every row's result
+ * depends on its own slot only). */
+__global__ __launch_bounds__(256) void k_active_slots(int n, const uint2* __restrict__ sorted,
+                                                      const uint8_t* __restrict__ active, int32_t* list,
+                                                      int32_t* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < n && active[sorted[i].y];
+    const uint64_t b = __ballot(act);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0 && b) base = atomicAdd(count, (int)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (act) list[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = i;
+}
+
+__global__ void k_zero_i32(int32_t* p) { p[0] = 0; }
+
+int launch_features(sph_ctx* ctx, bool active_only = false) {
     if (!ctx->feats) SPH_HIP_TRY(hipMalloc((void**)&ctx->feats, sizeof(float4) * 3 * (size_t)ctx->n));
     FeatArgs a;
+    a.list = nullptr;
+    a.count = nullptr;
+    if (active_only) {
+        if (!ctx->slot_list) SPH_HIP_TRY(hipMalloc((void**)&ctx->slot_list, sizeof(int32_t) * (size_t)ctx->n));
+        if (!ctx->slot_count) SPH_HIP_TRY(hipMalloc((void**)&ctx->slot_count, sizeof(int32_t)));
+        k_zero_i32<<<1, 1, 0, ctx->stream>>>(ctx->slot_count);
+        k_active_slots<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, ctx->pairs_a, ctx->active,
+                                                                      ctx->slot_list, ctx->slot_count);
+        SPH_HIP_TRY(hipGetLastError());
+        a.list = ctx->slot_list;
+        a.count = ctx->slot_count;
+    }
     a.n = ctx->n;
     a.sorted = ctx->pairs_a;
     a.cell_se = ctx->cell_se;
@@ -953,7 +1047,10 @@ int launch_density_prologue(sph_ctx* ctx) {
     if (!ctx->dump && !ctx->learned) return SPH_OK;
     if (ctx->dump && ctx->dump->count + ctx->n > ctx->dump->cap)
         return ctx_fail(ctx, SPH_E_CAPACITY, "sample buffer full");
-    int rc = launch_features(ctx);
+    /* with individual timesteps only the active particles iterate; without
+     * a dump (which records every particle) only their guesses are needed */
+    const bool active_only = !ctx->dump && ctx->prm.activity_rule == SPH_ACTIVE_README;
+    int rc = launch_features(ctx, active_only);
     if (rc != SPH_OK) return rc;
     if (!ctx->learned) return SPH_OK;
     if (!ctx->guess) SPH_HIP_TRY(hipMalloc((void**)&ctx->guess, sizeof(float) * (size_t)ctx->n));
@@ -964,11 +1061,12 @@ int launch_density_prologue(sph_ctx* ctx) {
     a.params = m->params;
     a.norm = m->norm;
     a.sorted = ctx->pairs_a;
+    a.list = active_only ? ctx->slot_list : nullptr;
+    a.count = active_only ? ctx->slot_count : nullptr;
     a.h = ctx->h;
     a.max_h = ctx->prm.max_smoothing_length;
     a.guess = ctx->guess;
-    const int64_t blocks = ((int64_t)ctx->n + WAVES * S - 1) / (WAVES * S);
-    k_infer<<<(unsigned)blocks, WAVES * 64, 0, ctx->stream>>>(a);
+    k_infer<<<infer_blocks(ctx->n), WAVES * 64, 0, ctx->stream>>>(a);
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }

@@ -142,6 +142,8 @@ struct sph_ctx {
     /* learned smoothing-length guess (learned.hip) */
     float4* feats = nullptr;        /* 3 float4 (12 features) per sorted slot */
     float* guess = nullptr;         /* starting h per sorted slot */
+    int32_t* slot_list = nullptr;   /* sorted slots of the active particles (README rule) */
+    int32_t* slot_count = nullptr;  /* their number (device) */
     sph_mlp* learned = nullptr;     /* not owned */
     sph_samples* dump = nullptr;    /* not owned */
 };

@@ -181,3 +181,26 @@ def test_bad_guess_still_converges(sph, orc, tables256, impact):
           f"|dh/h| max {dh.max():.2e}", flush=True)
     assert np.quantile(dh, 0.999) < 2e-4
     assert learned["it"].mean() <= plain["it"].mean() + 1.0 + 1e-9
+
+
+def test_learned_guess_with_individual_timesteps(sph, orc, tables256, impact):
+    """README rule (up to 8 rungs): only the active particles' features and
+    guesses are computed (compacted sorted slots).  Active particles converge
+    to the plain iteration's h; inactive ones keep theirs in both runs."""
+    prm = sph.default_params(density_eta=1e-5, activity_rule=sph.SPH_ACTIVE_README, max_rungs=8,
+                             fixes=sph.SPH_FIX_INACTIVE_DRIFT)
+    ctx = sph.Context(prm, impact(20000, 13), tables256)
+    _run_steps(ctx, orc, prm, 3)
+    net = sph.Mlp(seed=3)
+    tree = orc.build_octree(ctx.download("pos"), prm.max_children_in_leaf, prm.max_tree_depth)
+    ctx.upload_tree(tree)
+    ctx.sort()
+    ctx.gravity()
+    act = ctx.download("active").astype(bool)
+    h0 = ctx.download("h")
+    plain, learned = _density_ab(ctx, net)
+    dh = np.abs(learned["h"].astype(np.float64) / plain["h"] - 1)
+    print(f"[learned README] active {act.mean():.4f}; |dh/h| active max {dh[act].max():.2e}", flush=True)
+    assert 0 < act.sum() < len(act)
+    assert np.array_equal(learned["h"][~act], h0[~act]) and np.array_equal(plain["h"][~act], h0[~act])
+    assert np.quantile(dh[act], 0.999) < 2e-4 and dh[act].max() < 1e-3
