@@ -1001,6 +1001,177 @@ __device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA&
     }
 }
 
+/* Targets of a near-field leaf (at most NEAR_K particles) in registers and
+ * their accumulated field (compensations and potential in the precision
+ * form). */
+template <int NEAR_K>
+struct NearTargets {
+    float tx[NEAR_K], ty[NEAR_K], tz[NEAR_K];
+    float gx[NEAR_K], gy[NEAR_K], gz[NEAR_K];
+    float cx[NEAR_K], cy[NEAR_K], cz[NEAR_K], gp[NEAR_K], cp[NEAR_K];
+};
+
+/* P2P of one rejected source leaf against the targets, with the M2L
+ * softening kernel (the near-field form's single pair force). */
+template <bool PREC, int NEAR_K>
+__device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>& T, int np, int4 ci) {
+    const int32_t* src = a.t.octree + ci.y + 2;
+    for (int q = 0; q < ci.x; ++q) {
+        const int pj = src[q];
+        const float4 xj = a.pos[pj];
+        const float mj = a.m[pj];
+#pragma unroll
+        for (int kk = 0; kk < NEAR_K; ++kk) {
+            const float dx = T.tx[kk] - xj.x, dy = T.ty[kk] - xj.y, dz = T.tz[kk] - xj.z;
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            if (kk >= np || r == 0.f) continue;
+            const float w = mj * dphi_dr_m2l(r, a.eps);
+            if constexpr (PREC) {
+                kadd(T.gx[kk], T.cx[kk], -(w * (dx / r)));
+                kadd(T.gy[kk], T.cy[kk], -(w * (dy / r)));
+                kadd(T.gz[kk], T.cz[kk], -(w * (dz / r)));
+                kadd(T.gp[kk], T.cp[kk], mj * pot_m2l(r, a.eps));
+            } else {
+                T.gx[kk] -= w * (dx / r);
+                T.gy[kk] -= w * (dy / r);
+                T.gz[kk] -= w * (dz / r);
+            }
+        }
+    }
+}
+
+/* Near-field form at a leaf of at most NEAR_K particles, with the work
+ * balanced over the GS lanes: every round each lane takes one node -- list
+ * entries first, then nodes from the group's LDS queue -- and either M2Ls it
+ * (MAC accepted), sums it pair by pair (rejected leaf) or queues its children
+ * (rejected branch).  Queue positions come from ballots in lane order, so
+ * the assignment of nodes to lanes, and with it every sum, is reproducible.
+ * When the queue is full a lane walks the branch's subtree itself in
+ * pre-order.  Same interactions as down_near_leaf; only the order of the
+ * additions differs. */
+template <int GS, bool PREC, int P, int QCAP>
+__device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const NodeA& A, LocalExp& L, float4 l0,
+                                 const int32_t* in, int lane, int wl, uint64_t gmask, int32_t* q) {
+    constexpr int NEAR_K = 4;
+    constexpr int NTP = nterms<P>();
+    const int np = inf.x;
+    const int32_t* pl = a.t.octree + inf.y + 2;
+    const uint64_t ltmask = wl == 0 ? 0ull : (~0ull >> (64 - wl));
+    float Lc[eNT3];
+#pragma unroll
+    for (int k = 0; k < NTP; ++k) Lc[k] = 0.f;
+    NearTargets<NEAR_K> T;
+#pragma unroll
+    for (int kk = 0; kk < NEAR_K; ++kk) {
+        const float4 x = kk < np ? a.pos[pl[kk]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        T.tx[kk] = x.x;
+        T.ty[kk] = x.y;
+        T.tz[kk] = x.z;
+        T.gx[kk] = T.gy[kk] = T.gz[kk] = 0.f;
+        T.cx[kk] = T.cy[kk] = T.cz[kk] = T.gp[kk] = T.cp[kk] = 0.f;
+    }
+    int lp = 0;            /* next list index (group-uniform, as are the rest) */
+    bool list_open = true;
+    int qh = 0, qt = 0;    /* queue head and tail (monotone; slot = value % QCAP) */
+    for (;;) {
+        int item = -1, nfill = 0;
+        if (list_open) {
+            const int i = lp + lane;
+            const int e = i < a.maxu ? in[i] : -1;
+            const uint64_t term = __ballot(e == -1) & gmask;
+            const uint64_t first = term & (~term + 1ull);
+            const bool valid = i < a.maxu && (term == 0ull || (1ull << wl) < first);
+            nfill = __popcll(__ballot(valid) & gmask);
+            if (valid) item = e;
+            lp += nfill;
+            if (term != 0ull || lp >= a.maxu) list_open = false;
+        }
+        const int take = min(qt - qh, GS - nfill);
+        if (lane >= nfill && lane - nfill < take) item = q[(qh + lane - nfill) % QCAP];
+        qh += take;
+        if (nfill + take == 0 && !list_open) break;
+        int nch = 0;
+        if (item >= 0 && item != id) {
+            const float4 b0 = a.t.far[3 * item], b1 = a.t.far[3 * item + 1], b2 = a.t.far[3 * item + 2];
+            if (mac_ok(a, A, item, b0, b1, b2)) {
+                if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, item, b0, b1, b2);
+                else m2l_add<P>(a, L, item, b0, b1, b2);
+            } else {
+                const int4 ci = a.t.info[item];
+                if (ci.x != 0) {
+                    near_p2p<PREC, NEAR_K>(a, T, np, ci);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) nch += a.t.child[8 * item + c] >= 0 ? 1 : 0;
+                }
+            }
+        }
+        /* group prefix sum of the child counts (0..8) from four bit ballots */
+        int before = 0, total = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint64_t m = __ballot((nch >> b) & 1) & gmask;
+            before += __popcll(m & ltmask) << b;
+            total += __popcll(m) << b;
+        }
+        if (total <= QCAP - (qt - qh)) {
+            if (nch > 0) {
+                int pos = qt + before;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int ch = a.t.child[8 * item + c];
+                    if (ch >= 0) q[(pos++) % QCAP] = ch;
+                }
+            }
+            qt += total;
+        } else if (nch > 0) {
+            /* queue full: walk the branch's subtree here, in pre-order */
+            int cur = item + 1;
+            const int end = a.t.sub_end[item];
+            while (cur < end) {
+                if (cur == id) {
+                    ++cur;
+                    continue;
+                }
+                const float4 b0 = a.t.far[3 * cur], b1 = a.t.far[3 * cur + 1], b2 = a.t.far[3 * cur + 2];
+                if (mac_ok(a, A, cur, b0, b1, b2)) {
+                    if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, cur, b0, b1, b2);
+                    else m2l_add<P>(a, L, cur, b0, b1, b2);
+                    cur = a.t.sub_end[cur];
+                    continue;
+                }
+                const int4 ci = a.t.info[cur];
+                ++cur;
+                if (ci.x != 0) near_p2p<PREC, NEAR_K>(a, T, np, ci);
+            }
+        }
+        /* the queue writes of this round are read by other lanes next round */
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int kk = 0; kk < NEAR_K; ++kk) {
+        T.gx[kk] = group_sum<GS>(T.gx[kk]);
+        T.gy[kk] = group_sum<GS>(T.gy[kk]);
+        T.gz[kk] = group_sum<GS>(T.gz[kk]);
+        if constexpr (PREC) {
+            T.gx[kk] += group_sum<GS>(T.cx[kk]);
+            T.gy[kk] += group_sum<GS>(T.cy[kk]);
+            T.gz[kk] += group_sum<GS>(T.cz[kk]);
+            T.gp[kk] = group_sum<GS>(T.gp[kk]) + group_sum<GS>(T.cp[kk]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NTP; ++k) {
+        L.e[k] = group_sum<GS>(L.e[k]);
+        if constexpr (PREC) L.e[k] += group_sum<GS>(Lc[k]);
+    }
+    add_incoming<P>(a, id, l0, L);
+#pragma unroll
+    for (int kk = 0; kk < NEAR_K; ++kk)
+        if (lane == kk && kk < np) leaf_particle<PREC, P>(a, L, pl, np, kk, T.gx[kk], T.gy[kk], T.gz[kk], T.gp[kk]);
+}
+
 /* Down pass for GRAVITY_MAX_RECURSION == 1 with GS lanes per node.
  *
  * Lanes test GS list entries at a time against the node (MAC); accepted
@@ -1049,8 +1220,14 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     for (int k = 0; k < NTP; ++k) L.e[k] = Lc[k] = 0.f;
     const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
     if constexpr (NEAR) {
+        constexpr int QCAP = 512;
+        __shared__ int32_t near_q[GPB * QCAP];
         if (live && np != 0) {
-            down_near_leaf<GS, PREC, P>(a, id, inf, A, L, l0, in, lane, wl, gmask);
+            if (np <= 4)
+                down_near_leaf_q<GS, PREC, P, QCAP>(a, id, inf, A, L, l0, in, lane, wl, gmask,
+                                                    near_q + (tid / GS) * QCAP);
+            else
+                down_near_leaf<GS, PREC, P>(a, id, inf, A, L, l0, in, lane, wl, gmask);
             return;
         }
     }
