@@ -891,116 +891,6 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
-/* Near-field form at a leaf (GS lanes, one leaf).  Each lane walks the
- * subtrees of its list entries in pre-order without a stack: a node is
- * M2L'd if the MAC accepts it (skip its subtree), a rejected leaf is summed
- * particle by particle (P2P), a rejected branch is opened (next id).  The
- * targets are the leaf's particles, NEAR_K at a time in registers; the M2L
- * part runs in the first pass only.  Partial sums are reduced over the group,
- * the incoming local expansion is added, and lane k finishes particle k. */
-template <int GS, bool PREC, int P>
-__device__ void down_near_leaf(const DownArgs& a, int id, int4 inf, const NodeA& A, LocalExp& L, float4 l0,
-                               const int32_t* in, int lane, int wl, uint64_t gmask) {
-    constexpr int NEAR_K = 4;
-    constexpr int NTP = nterms<P>();
-    const int np = inf.x;
-    const int32_t* pl = a.t.octree + inf.y + 2;
-    float Lc[eNT3];
-#pragma unroll
-    for (int k = 0; k < NTP; ++k) Lc[k] = 0.f;
-    for (int k0 = 0; k0 < np; k0 += NEAR_K) {
-        float tx[NEAR_K], ty[NEAR_K], tz[NEAR_K], gx[NEAR_K], gy[NEAR_K], gz[NEAR_K];
-        /* precision form: compensations and the pair potential */
-        float cx[NEAR_K], cy[NEAR_K], cz[NEAR_K], gp[NEAR_K], cp[NEAR_K];
-#pragma unroll
-        for (int kk = 0; kk < NEAR_K; ++kk) {
-            const float4 x = k0 + kk < np ? a.pos[pl[k0 + kk]] : make_float4(0.f, 0.f, 0.f, 0.f);
-            tx[kk] = x.x;
-            ty[kk] = x.y;
-            tz[kk] = x.z;
-            gx[kk] = gy[kk] = gz[kk] = 0.f;
-            cx[kk] = cy[kk] = cz[kk] = gp[kk] = cp[kk] = 0.f;
-        }
-        bool more = true;
-        for (int base = 0; more; base += GS) {
-            const int i = base + lane;
-            const int e = i < a.maxu ? in[i] : -1;
-            const uint64_t term = __ballot(e == -1) & gmask;
-            const uint64_t first = term & (~term + 1ull);
-            const bool valid = i < a.maxu && (term == 0ull || (1ull << wl) < first);
-            more = term == 0ull && base + GS < a.maxu;
-            if (!valid) continue;
-            int cur = e;
-            const int end = a.t.sub_end[e];
-            while (cur < end) {
-                if (cur == id) {
-                    ++cur;
-                    continue;
-                }
-                const float4 b0 = a.t.far[3 * cur], b1 = a.t.far[3 * cur + 1], b2 = a.t.far[3 * cur + 2];
-                if (mac_ok(a, A, cur, b0, b1, b2)) {
-                    if (k0 == 0) {
-                        if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, cur, b0, b1, b2);
-                        else m2l_add<P>(a, L, cur, b0, b1, b2);
-                    }
-                    cur = a.t.sub_end[cur];
-                    continue;
-                }
-                const int4 ci = a.t.info[cur];
-                ++cur;
-                if (ci.x == 0) continue;
-                const int32_t* src = a.t.octree + ci.y + 2;
-                for (int q = 0; q < ci.x; ++q) {
-                    const int pj = src[q];
-                    const float4 xj = a.pos[pj];
-                    const float mj = a.m[pj];
-#pragma unroll
-                    for (int kk = 0; kk < NEAR_K; ++kk) {
-                        const float dx = tx[kk] - xj.x, dy = ty[kk] - xj.y, dz = tz[kk] - xj.z;
-                        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-                        if (k0 + kk >= np || r == 0.f) continue;
-                        const float w = mj * dphi_dr_m2l(r, a.eps);
-                        if constexpr (PREC) {
-                            kadd(gx[kk], cx[kk], -(w * (dx / r)));
-                            kadd(gy[kk], cy[kk], -(w * (dy / r)));
-                            kadd(gz[kk], cz[kk], -(w * (dz / r)));
-                            kadd(gp[kk], cp[kk], mj * pot_m2l(r, a.eps));
-                        } else {
-                            gx[kk] -= w * (dx / r);
-                            gy[kk] -= w * (dy / r);
-                            gz[kk] -= w * (dz / r);
-                        }
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int kk = 0; kk < NEAR_K; ++kk) {
-            gx[kk] = group_sum<GS>(gx[kk]);
-            gy[kk] = group_sum<GS>(gy[kk]);
-            gz[kk] = group_sum<GS>(gz[kk]);
-            if constexpr (PREC) {
-                gx[kk] += group_sum<GS>(cx[kk]);
-                gy[kk] += group_sum<GS>(cy[kk]);
-                gz[kk] += group_sum<GS>(cz[kk]);
-                gp[kk] = group_sum<GS>(gp[kk]) + group_sum<GS>(cp[kk]);
-            }
-        }
-        if (k0 == 0) {
-#pragma unroll
-            for (int k = 0; k < NTP; ++k) {
-                L.e[k] = group_sum<GS>(L.e[k]);
-                if constexpr (PREC) L.e[k] += group_sum<GS>(Lc[k]);
-            }
-            add_incoming<P>(a, id, l0, L);
-        }
-#pragma unroll
-        for (int kk = 0; kk < NEAR_K; ++kk)
-            if (lane == kk && k0 + kk < np)
-                leaf_particle<PREC, P>(a, L, pl, np, k0 + kk, gx[kk], gy[kk], gz[kk], gp[kk]);
-    }
-}
-
 /* Targets of a near-field leaf (at most NEAR_K particles) in registers and
  * their accumulated field (compensations and potential in the precision
  * form). */
@@ -1040,15 +930,17 @@ __device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>&
     }
 }
 
-/* Near-field form at a leaf of at most NEAR_K particles, with the work
- * balanced over the GS lanes: every round each lane takes one node -- list
- * entries first, then nodes from the group's LDS queue -- and either M2Ls it
- * (MAC accepted), sums it pair by pair (rejected leaf) or queues its children
- * (rejected branch).  Queue positions come from ballots in lane order, so
- * the assignment of nodes to lanes, and with it every sum, is reproducible.
- * When the queue is full a lane walks the branch's subtree itself in
- * pre-order.  Same interactions as down_near_leaf; only the order of the
- * additions differs. */
+/* Near-field form at a leaf, with the work balanced over the GS lanes: every
+ * round each lane takes one node -- list entries first, then nodes from the
+ * group's LDS queue -- and either M2Ls it (MAC accepted), sums it pair by
+ * pair (rejected leaf) or queues its children (rejected branch).  Queue
+ * positions come from ballots in lane order, so the assignment of nodes to
+ * lanes, and with it every sum, is reproducible.  When the queue is full a
+ * lane walks the branch's subtree itself in pre-order (node ids are
+ * pre-order: a subtree is [id, subtree_end[id])).  The targets are the
+ * leaf's particles, NEAR_K at a time in registers; the M2L part runs in the
+ * first pass only.  Partial sums are reduced over the group, the incoming
+ * local expansion is added, and lane k finishes particle k of the pass. */
 template <int GS, bool PREC, int P, int QCAP>
 __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const NodeA& A, LocalExp& L, float4 l0,
                                  const int32_t* in, int lane, int wl, uint64_t gmask, int32_t* q) {
@@ -1060,116 +952,127 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
     float Lc[eNT3];
 #pragma unroll
     for (int k = 0; k < NTP; ++k) Lc[k] = 0.f;
-    NearTargets<NEAR_K> T;
+    for (int k0 = 0; k0 < np; k0 += NEAR_K) {
+        const int nk = min(NEAR_K, np - k0);
+        NearTargets<NEAR_K> T;
 #pragma unroll
-    for (int kk = 0; kk < NEAR_K; ++kk) {
-        const float4 x = kk < np ? a.pos[pl[kk]] : make_float4(0.f, 0.f, 0.f, 0.f);
-        T.tx[kk] = x.x;
-        T.ty[kk] = x.y;
-        T.tz[kk] = x.z;
-        T.gx[kk] = T.gy[kk] = T.gz[kk] = 0.f;
-        T.cx[kk] = T.cy[kk] = T.cz[kk] = T.gp[kk] = T.cp[kk] = 0.f;
-    }
-    int lp = 0;            /* next list index (group-uniform, as are the rest) */
-    bool list_open = true;
-    int qh = 0, qt = 0;    /* queue head and tail (monotone; slot = value % QCAP) */
-    for (;;) {
-        int item = -1, nfill = 0;
-        if (list_open) {
-            const int i = lp + lane;
-            const int e = i < a.maxu ? in[i] : -1;
-            const uint64_t term = __ballot(e == -1) & gmask;
-            const uint64_t first = term & (~term + 1ull);
-            const bool valid = i < a.maxu && (term == 0ull || (1ull << wl) < first);
-            nfill = __popcll(__ballot(valid) & gmask);
-            if (valid) item = e;
-            lp += nfill;
-            if (term != 0ull || lp >= a.maxu) list_open = false;
+        for (int kk = 0; kk < NEAR_K; ++kk) {
+            const float4 x = kk < nk ? a.pos[pl[k0 + kk]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            T.tx[kk] = x.x;
+            T.ty[kk] = x.y;
+            T.tz[kk] = x.z;
+            T.gx[kk] = T.gy[kk] = T.gz[kk] = 0.f;
+            T.cx[kk] = T.cy[kk] = T.cz[kk] = T.gp[kk] = T.cp[kk] = 0.f;
         }
-        const int take = min(qt - qh, GS - nfill);
-        if (lane >= nfill && lane - nfill < take) item = q[(qh + lane - nfill) % QCAP];
-        qh += take;
-        if (nfill + take == 0 && !list_open) break;
-        int nch = 0;
-        if (item >= 0 && item != id) {
-            const float4 b0 = a.t.far[3 * item], b1 = a.t.far[3 * item + 1], b2 = a.t.far[3 * item + 2];
-            if (mac_ok(a, A, item, b0, b1, b2)) {
-                if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, item, b0, b1, b2);
-                else m2l_add<P>(a, L, item, b0, b1, b2);
-            } else {
-                const int4 ci = a.t.info[item];
-                if (ci.x != 0) {
-                    near_p2p<PREC, NEAR_K>(a, T, np, ci);
+        const bool do_m2l = k0 == 0;
+        int lp = 0;            /* next list index (group-uniform, as are the rest) */
+        bool list_open = true;
+        int qh = 0, qt = 0;    /* queue head and tail (monotone; slot = value % QCAP) */
+        for (;;) {
+            int item = -1, nfill = 0;
+            if (list_open) {
+                const int i = lp + lane;
+                const int e = i < a.maxu ? in[i] : -1;
+                const uint64_t term = __ballot(e == -1) & gmask;
+                const uint64_t first = term & (~term + 1ull);
+                const bool valid = i < a.maxu && (term == 0ull || (1ull << wl) < first);
+                nfill = __popcll(__ballot(valid) & gmask);
+                if (valid) item = e;
+                lp += nfill;
+                if (term != 0ull || lp >= a.maxu) list_open = false;
+            }
+            const int take = min(qt - qh, GS - nfill);
+            if (lane >= nfill && lane - nfill < take) item = q[(qh + lane - nfill) % QCAP];
+            qh += take;
+            if (nfill + take == 0 && !list_open) break;
+            int nch = 0;
+            if (item >= 0 && item != id) {
+                const float4 b0 = a.t.far[3 * item], b1 = a.t.far[3 * item + 1], b2 = a.t.far[3 * item + 2];
+                if (mac_ok(a, A, item, b0, b1, b2)) {
+                    if (do_m2l) {
+                        if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, item, b0, b1, b2);
+                        else m2l_add<P>(a, L, item, b0, b1, b2);
+                    }
                 } else {
+                    const int4 ci = a.t.info[item];
+                    if (ci.x != 0) {
+                        near_p2p<PREC, NEAR_K>(a, T, nk, ci);
+                    } else {
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) nch += a.t.child[8 * item + c] >= 0 ? 1 : 0;
+                        for (int c = 0; c < 8; ++c) nch += a.t.child[8 * item + c] >= 0 ? 1 : 0;
+                    }
                 }
             }
-        }
-        /* group prefix sum of the child counts (0..8) from four bit ballots */
-        int before = 0, total = 0;
+            /* group prefix sum of the child counts (0..8) from four bit ballots */
+            int before = 0, total = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint64_t m = __ballot((nch >> b) & 1) & gmask;
-            before += __popcll(m & ltmask) << b;
-            total += __popcll(m) << b;
-        }
-        if (total <= QCAP - (qt - qh)) {
-            if (nch > 0) {
-                int pos = qt + before;
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int ch = a.t.child[8 * item + c];
-                    if (ch >= 0) q[(pos++) % QCAP] = ch;
-                }
+            for (int b = 0; b < 4; ++b) {
+                const uint64_t m = __ballot((nch >> b) & 1) & gmask;
+                before += __popcll(m & ltmask) << b;
+                total += __popcll(m) << b;
             }
-            qt += total;
-        } else if (nch > 0) {
-            /* queue full: walk the branch's subtree here, in pre-order */
-            int cur = item + 1;
-            const int end = a.t.sub_end[item];
-            while (cur < end) {
-                if (cur == id) {
+            if (total <= QCAP - (qt - qh)) {
+                if (nch > 0) {
+                    int pos = qt + before;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int ch = a.t.child[8 * item + c];
+                        if (ch >= 0) q[(pos++) % QCAP] = ch;
+                    }
+                }
+                qt += total;
+            } else if (nch > 0) {
+                /* queue full: walk the branch's subtree here, in pre-order */
+                int cur = item + 1;
+                const int end = a.t.sub_end[item];
+                while (cur < end) {
+                    if (cur == id) {
+                        ++cur;
+                        continue;
+                    }
+                    const float4 b0 = a.t.far[3 * cur], b1 = a.t.far[3 * cur + 1], b2 = a.t.far[3 * cur + 2];
+                    if (mac_ok(a, A, cur, b0, b1, b2)) {
+                        if (do_m2l) {
+                            if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, cur, b0, b1, b2);
+                            else m2l_add<P>(a, L, cur, b0, b1, b2);
+                        }
+                        cur = a.t.sub_end[cur];
+                        continue;
+                    }
+                    const int4 ci = a.t.info[cur];
                     ++cur;
-                    continue;
+                    if (ci.x != 0) near_p2p<PREC, NEAR_K>(a, T, nk, ci);
                 }
-                const float4 b0 = a.t.far[3 * cur], b1 = a.t.far[3 * cur + 1], b2 = a.t.far[3 * cur + 2];
-                if (mac_ok(a, A, cur, b0, b1, b2)) {
-                    if constexpr (PREC) m2l_add_comp<P>(a, L, Lc, cur, b0, b1, b2);
-                    else m2l_add<P>(a, L, cur, b0, b1, b2);
-                    cur = a.t.sub_end[cur];
-                    continue;
-                }
-                const int4 ci = a.t.info[cur];
-                ++cur;
-                if (ci.x != 0) near_p2p<PREC, NEAR_K>(a, T, np, ci);
+            }
+            /* the queue writes of this round are read by other lanes next round */
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+#pragma unroll
+        for (int kk = 0; kk < NEAR_K; ++kk) {
+            T.gx[kk] = group_sum<GS>(T.gx[kk]);
+            T.gy[kk] = group_sum<GS>(T.gy[kk]);
+            T.gz[kk] = group_sum<GS>(T.gz[kk]);
+            if constexpr (PREC) {
+                T.gx[kk] += group_sum<GS>(T.cx[kk]);
+                T.gy[kk] += group_sum<GS>(T.cy[kk]);
+                T.gz[kk] += group_sum<GS>(T.cz[kk]);
+                T.gp[kk] = group_sum<GS>(T.gp[kk]) + group_sum<GS>(T.cp[kk]);
             }
         }
-        /* the queue writes of this round are read by other lanes next round */
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-    }
+        if (do_m2l) {
 #pragma unroll
-    for (int kk = 0; kk < NEAR_K; ++kk) {
-        T.gx[kk] = group_sum<GS>(T.gx[kk]);
-        T.gy[kk] = group_sum<GS>(T.gy[kk]);
-        T.gz[kk] = group_sum<GS>(T.gz[kk]);
-        if constexpr (PREC) {
-            T.gx[kk] += group_sum<GS>(T.cx[kk]);
-            T.gy[kk] += group_sum<GS>(T.cy[kk]);
-            T.gz[kk] += group_sum<GS>(T.cz[kk]);
-            T.gp[kk] = group_sum<GS>(T.gp[kk]) + group_sum<GS>(T.cp[kk]);
+            for (int k = 0; k < NTP; ++k) {
+                L.e[k] = group_sum<GS>(L.e[k]);
+                if constexpr (PREC) L.e[k] += group_sum<GS>(Lc[k]);
+            }
+            add_incoming<P>(a, id, l0, L);
         }
-    }
 #pragma unroll
-    for (int k = 0; k < NTP; ++k) {
-        L.e[k] = group_sum<GS>(L.e[k]);
-        if constexpr (PREC) L.e[k] += group_sum<GS>(Lc[k]);
+        for (int kk = 0; kk < NEAR_K; ++kk)
+            if (lane == kk && kk < nk)
+                leaf_particle<PREC, P>(a, L, pl, np, k0 + kk, T.gx[kk], T.gy[kk], T.gz[kk], T.gp[kk]);
     }
-    add_incoming<P>(a, id, l0, L);
-#pragma unroll
-    for (int kk = 0; kk < NEAR_K; ++kk)
-        if (lane == kk && kk < np) leaf_particle<PREC, P>(a, L, pl, np, kk, T.gx[kk], T.gy[kk], T.gz[kk], T.gp[kk]);
 }
 
 /* Down pass for GRAVITY_MAX_RECURSION == 1 with GS lanes per node.
@@ -1186,8 +1089,8 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
  * partial local expansion; the group reduces them with shuffles (a fixed
  * order, so results are reproducible), adds the incoming local expansion and
  * runs L2L (lane c -> child c) or L2P + P2P (lane k -> particle k). */
-template <int GS, bool NEAR, bool PREC, int P>
-__global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
+template <int GS, bool NEAR, bool PREC, int P, int MODE>
+__device__ __forceinline__ void down_group_body(const DownArgs& a) {
     constexpr int NTP = nterms<P>();
     constexpr int GPB = 256 / GS;
     constexpr int EPR = GS / 8;
@@ -1219,17 +1122,19 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
 #pragma unroll
     for (int k = 0; k < NTP; ++k) L.e[k] = Lc[k] = 0.f;
     const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
-    if constexpr (NEAR) {
+    /* MODE 1: branches only, MODE 2: leaves only (the near-field form runs
+     * the two as separate launches, so each has the registers of its own
+     * path); MODE 0: every node */
+    if constexpr (MODE == 1) {
+        if (np != 0) return;
+    }
+    if constexpr (MODE == 2) {
+        static_assert(NEAR, "leaf launches are for the near-field form");
         constexpr int QCAP = 512;
         __shared__ int32_t near_q[GPB * QCAP];
-        if (live && np != 0) {
-            if (np <= 4)
-                down_near_leaf_q<GS, PREC, P, QCAP>(a, id, inf, A, L, l0, in, lane, wl, gmask,
-                                                    near_q + (tid / GS) * QCAP);
-            else
-                down_near_leaf<GS, PREC, P>(a, id, inf, A, L, l0, in, lane, wl, gmask);
-            return;
-        }
+        if (live && np != 0)
+            down_near_leaf_q<GS, PREC, P, QCAP>(a, id, inf, A, L, l0, in, lane, wl, gmask, near_q + (tid / GS) * QCAP);
+        return;
     }
     const uint32_t outbase = (uint32_t)s * (uint32_t)a.maxu;
     int running = 0;
@@ -1309,6 +1214,19 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     } else {
         leaf_field_lanes<PREC, P>(a, L, inf, lane, GS);
     }
+}
+
+template <int GS, bool NEAR, bool PREC, int P, int MODE>
+__global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
+    down_group_body<GS, NEAR, PREC, P, MODE>(a);
+}
+
+/* The near-field leaves of a level.  Two waves per SIMD are asked for: the
+ * P = 3 precision walk otherwise needs more registers than that allows and
+ * would run one wave per SIMD. */
+template <int GS, bool PREC, int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_down_leaves(DownArgs a) {
+    down_group_body<GS, true, PREC, P, 2>(a);
 }
 
 /* Level 0.  zeroFirstBranches (helpers.h:15-32); the root has node id 0.
@@ -1476,14 +1394,28 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
             constexpr int GS = 32;
             const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
             const dim3 g(xcd_grid(blocks));
+            /* near-field form: branches (list processing, L2L) and leaves
+             * (balanced walk, L2P) as two launches over the level */
             if (order3) {
-                if (a.phi) k_down_group<GS, true, true, 3><<<g, threads, 0, ctx->stream>>>(a);
-                else if (a.near) k_down_group<GS, true, false, 3><<<g, threads, 0, ctx->stream>>>(a);
-                else k_down_group<GS, false, false, 3><<<g, threads, 0, ctx->stream>>>(a);
+                if (a.phi) {
+                    k_down_group<GS, true, true, 3, 1><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves<GS, true, 3><<<g, threads, 0, ctx->stream>>>(a);
+                } else if (a.near) {
+                    k_down_group<GS, true, false, 3, 1><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves<GS, false, 3><<<g, threads, 0, ctx->stream>>>(a);
+                } else {
+                    k_down_group<GS, false, false, 3, 0><<<g, threads, 0, ctx->stream>>>(a);
+                }
             } else {
-                if (a.phi) k_down_group<GS, true, true, 2><<<g, threads, 0, ctx->stream>>>(a);
-                else if (a.near) k_down_group<GS, true, false, 2><<<g, threads, 0, ctx->stream>>>(a);
-                else k_down_group<GS, false, false, 2><<<g, threads, 0, ctx->stream>>>(a);
+                if (a.phi) {
+                    k_down_group<GS, true, true, 2, 1><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves<GS, true, 2><<<g, threads, 0, ctx->stream>>>(a);
+                } else if (a.near) {
+                    k_down_group<GS, true, false, 2, 1><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves<GS, false, 2><<<g, threads, 0, ctx->stream>>>(a);
+                } else {
+                    k_down_group<GS, false, false, 2, 0><<<g, threads, 0, ctx->stream>>>(a);
+                }
             }
         } else {
             k_down<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
