@@ -465,10 +465,12 @@ __device__ __forceinline__ bool mac_accept3(float ax, float ay, float az, float 
 
 /* M2L.metal:12-108 + derivatives.metal:59-156: the local coefficients one
  * source node adds at the expansion centre (ten at P = 2; at P = 3 also the
- * octupole's potential, the quadrupole's field and the third derivatives). */
-template <int P>
-__device__ __forceinline__ void m2l_terms(const LocalExp& L, float4 b0, float4 b1, float4 b2, const float4* oct,
-                                          float eps, float T[eNT3]) {
+ * octupole's potential, the quadrupole's field and the third derivatives).
+ * Each coefficient is handed to `emit(k, value)` as soon as it is final, so
+ * that the accumulation (plain or compensated) does not hold all of them. */
+template <int P, class Emit>
+__device__ __forceinline__ void m2l_emit(const LocalExp& L, float4 b0, float4 b1, float4 b2, const float4* oct,
+                                         float eps, Emit emit) {
     const float vx = L.px - b0.x, vy = L.py - b0.y, vz = L.pz - b0.z;
     const float r2 = vx * vx + vy * vy + vz * vz;
     const float r = sqrtf(r2);
@@ -499,25 +501,27 @@ __device__ __forceinline__ void m2l_terms(const LocalExp& L, float4 b0, float4 b
         if (P == 3) Dt_4 = -5.f * Dt_3 * r_inv;
     }
     const float rx = vx * r_inv, ry = vy * r_inv, rz = vz * r_inv;
-    const float DX = rx * Dt_2, DY = ry * Dt_2, DZ = rz * Dt_2;
     const float D2 = Dt_2 * r_inv;
-    const float DXX = rx * rx * Dt_3 + D2, DXY = rx * ry * Dt_3, DXZ = rx * rz * Dt_3;
-    const float DYY = ry * ry * Dt_3 + D2, DYZ = ry * rz * Dt_3, DZZ = rz * rz * Dt_3 + D2;
     const float M0 = b1.x;
-    float lm = M0 * Dt_1;
-    lm += b1.y * DXX + b2.x * DYY + b2.z * DZZ;
-    lm += b1.z * DXY + b1.w * DXZ + b2.y * DYZ;
-    T[eM] = lm;
-    T[eX] = M0 * DX;
-    T[eY] = M0 * DY;
-    T[eZ] = M0 * DZ;
-    T[eXX] = M0 * DXX;
-    T[eXY] = M0 * DXY;
-    T[eXZ] = M0 * DXZ;
-    T[eYY] = M0 * DYY;
-    T[eYZ] = M0 * DYZ;
-    T[eZZ] = M0 * DZZ;
-    if (P == 3) {
+    {
+        const float DXX = rx * rx * Dt_3 + D2, DXY = rx * ry * Dt_3, DXZ = rx * rz * Dt_3;
+        const float DYY = ry * ry * Dt_3 + D2, DYZ = ry * rz * Dt_3, DZZ = rz * rz * Dt_3 + D2;
+        emit(eXX, M0 * DXX);
+        emit(eXY, M0 * DXY);
+        emit(eXZ, M0 * DXZ);
+        emit(eYY, M0 * DYY);
+        emit(eYZ, M0 * DYZ);
+        emit(eZZ, M0 * DZZ);
+        float lm = M0 * Dt_1;
+        lm += b1.y * DXX + b2.x * DYY + b2.z * DZZ;
+        lm += b1.z * DXY + b1.w * DXZ + b2.y * DYZ;
+        if (P != 3) {
+            emit(eM, lm);
+            emit(eX, M0 * (rx * Dt_2));
+            emit(eY, M0 * (ry * Dt_2));
+            emit(eZ, M0 * (rz * Dt_2));
+            return;
+        }
         /* derivatives.metal:140-154, M2L.metal:58-107 */
         const float D3 = Dt_3 * r_inv;
         const float rx2 = rx * rx, ry2 = ry * ry, rz2 = rz * rz;
@@ -533,30 +537,30 @@ __device__ __forceinline__ void m2l_terms(const LocalExp& L, float4 b0, float4 b
         lo += o0.y * DXXY + o0.z * DXXZ + o0.w * DXYY;
         lo += o1.w * DYYZ + o1.y * DXZZ + o2.x * DYZZ;
         lo += o1.x * DXYZ;
-        T[eM] += lo;
+        emit(eM, lm + lo);
         const float QXX = b1.y, QXY = b1.z, QXZ = b1.w, QYY = b2.x, QYZ = b2.y, QZZ = b2.z;
-        T[eX] += QXX * DXXX + QYY * DXYY + QZZ * DXZZ + (QXY * DXXY + QXZ * DXXZ + QYZ * DXYZ);
-        T[eY] += QXX * DXXY + QYY * DYYY + QZZ * DYZZ + (QXY * DXYY + QXZ * DXYZ + QYZ * DYYZ);
-        T[eZ] += QXX * DXXZ + QYY * DYYZ + QZZ * DZZZ + (QXY * DXYZ + QXZ * DXZZ + QYZ * DYZZ);
-        T[eXXX] = M0 * DXXX;
-        T[eXXY] = M0 * DXXY;
-        T[eXXZ] = M0 * DXXZ;
-        T[eXYY] = M0 * DXYY;
-        T[eXYZ] = M0 * DXYZ;
-        T[eXZZ] = M0 * DXZZ;
-        T[eYYY] = M0 * DYYY;
-        T[eYYZ] = M0 * DYYZ;
-        T[eYZZ] = M0 * DYZZ;
-        T[eZZZ] = M0 * DZZZ;
+        emit(eX, M0 * (rx * Dt_2) +
+                     (QXX * DXXX + QYY * DXYY + QZZ * DXZZ + (QXY * DXXY + QXZ * DXXZ + QYZ * DXYZ)));
+        emit(eY, M0 * (ry * Dt_2) +
+                     (QXX * DXXY + QYY * DYYY + QZZ * DYZZ + (QXY * DXYY + QXZ * DXYZ + QYZ * DYYZ)));
+        emit(eZ, M0 * (rz * Dt_2) +
+                     (QXX * DXXZ + QYY * DYYZ + QZZ * DZZZ + (QXY * DXYZ + QXZ * DXZZ + QYZ * DYZZ)));
+        emit(eXXX, M0 * DXXX);
+        emit(eXXY, M0 * DXXY);
+        emit(eXXZ, M0 * DXXZ);
+        emit(eXYY, M0 * DXYY);
+        emit(eXYZ, M0 * DXYZ);
+        emit(eXZZ, M0 * DXZZ);
+        emit(eYYY, M0 * DYYY);
+        emit(eYYZ, M0 * DYYZ);
+        emit(eYZZ, M0 * DYZZ);
+        emit(eZZZ, M0 * DZZZ);
     }
 }
 
 template <int P>
 __device__ __forceinline__ void m2l_add(const DownArgs& a, LocalExp& L, int node, float4 b0, float4 b1, float4 b2) {
-    float T[eNT3];
-    m2l_terms<P>(L, b0, b1, b2, a.t.oct + 3 * node, a.eps, T);
-#pragma unroll
-    for (int k = 0; k < nterms<P>(); ++k) L.e[k] += T[k];
+    m2l_emit<P>(L, b0, b1, b2, a.t.oct + 3 * node, a.eps, [&](int k, float v) { L.e[k] += v; });
 }
 
 /* Neumaier's compensated sum (Neumaier 1974): s + c carries the sum of the
@@ -571,10 +575,7 @@ __device__ __forceinline__ void kadd(float& s, float& c, float x) {
 template <int P>
 __device__ __forceinline__ void m2l_add_comp(const DownArgs& a, LocalExp& L, float Lc[eNT3], int node, float4 b0,
                                              float4 b1, float4 b2) {
-    float T[eNT3];
-    m2l_terms<P>(L, b0, b1, b2, a.t.oct + 3 * node, a.eps, T);
-#pragma unroll
-    for (int k = 0; k < nterms<P>(); ++k) kadd(L.e[k], Lc[k], T[k]);
+    m2l_emit<P>(L, b0, b1, b2, a.t.oct + 3 * node, a.eps, [&](int k, float v) { kadd(L.e[k], Lc[k], v); });
 }
 
 /* Softened 1/r whose derivatives M2L uses (derivatives.metal:12-57,
