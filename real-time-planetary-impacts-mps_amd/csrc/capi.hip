@@ -486,6 +486,7 @@ int sph_tree_upload_done(sph_ctx* ctx) {
 
 int sph_sort(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
+    ctx->slot_list_valid = false;
     int rc = ctx_begin_pass(ctx, SPH_PASS_SORT);
     if (rc == SPH_OK) rc = launch_sort(ctx);
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_SORT);
@@ -497,6 +498,7 @@ int sph_gravity(sph_ctx* ctx) {
     if (!ctx->have_tree) return ctx_fail(ctx, SPH_E_NOTREE, "gravity before any tree upload");
     /* the near-field form checks the node numbering in the up pass */
     if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ctx->tree_check_pending = true;
+    ctx->slot_list_valid = false; /* the up pass sets the activity */
     int rc = ctx_begin_pass(ctx, SPH_PASS_GRAVITY_UP);
     /* With gravNorm = 0 every MAC test rejects (minGrav = 0, Q6).  As written
      * the leaves then force-accept; in the near-field form every pair would be
@@ -643,6 +645,7 @@ int sph_upload(sph_ctx* ctx, int field, const void* src) {
     void* dst = field_ptr(ctx, field);
     if (!dst) return ctx_fail(ctx, SPH_E_ARG, "field not uploadable");
     if (field == SPH_F_GRAVNORM) ctx->gravnorm_known = true;
+    ctx->slot_list_valid = false;
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     SPH_HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
     return SPH_OK;

@@ -131,6 +131,9 @@ struct DensityArgs {
     int fix_cache;
     int dt_reset;
     const float* guess;   /* learned starting h per sorted slot, or null */
+    /* README rule: the inactive particles' EOS runs in k_inactive_eos (in
+     * original order, coalesced) and their threads here return at once */
+    int skip_inactive;
 };
 
 constexpr float kGuessTrust = 0.05f;
@@ -145,6 +148,7 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     const uint32_t ind = a.sorted[index].y;
     const float2* table = a.mat[ind] == 400 ? a.eos0 : a.eos1;
     if (!a.active[ind]) {
+        if (a.skip_inactive) return;
         const float rho = a.rho[ind];
         const float2 pc = eos_sample(table, a.eos_res, a.u[ind], rho, a.min_u, a.max_u, a.min_rho, a.max_rho);
         a.P[ind] = pc.x;
@@ -344,6 +348,7 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     const uint32_t ind = a.sorted[index].y;
     const float2* table = a.mat[ind] == 400 ? a.eos0 : a.eos1;
     if (!a.active[ind]) {
+        if (a.skip_inactive) return;
         if (lane == 0) {
             const float rho = a.rho[ind];
             const float2 pc = eos_sample(table, a.eos_res, a.u[ind], rho, a.min_u, a.max_u, a.min_rho, a.max_rho);
@@ -515,6 +520,38 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     a.B[ind] = bal;
     a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
 }
+
+/* density.metal:40-55 for the inactive particles, in original order
+ * (coalesced) instead of through the sorted slots. */
+__global__ __launch_bounds__(256) void k_inactive_eos(DensityArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n || a.active[i]) return;
+    const float2* table = a.mat[i] == 400 ? a.eos0 : a.eos1;
+    const float2 pc = eos_sample(table, a.eos_res, a.u[i], a.rho[i], a.min_u, a.max_u, a.min_rho, a.max_rho);
+    a.P[i] = pc.x;
+    a.c[i] = pc.y;
+    a.nr_iters[i] = 0;
+    a.work[i] = make_int4(0, 0, 0, 0);
+}
+
+/* Sorted slots of the active particles (for the learned guess's features
+ * and inference): one ballot and one counter update per wave (the list order
+ * across waves is not fixed; each particle's result depends on its own slot
+ * only). */
+__global__ __launch_bounds__(256) void k_active_slots(int n, const uint2* __restrict__ sorted,
+                                                      const uint8_t* __restrict__ active, int32_t* list,
+                                                      int32_t* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < n && active[sorted[i].y];
+    const uint64_t b = __ballot(act);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0 && b) base = atomicAdd(count, (int)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (act) list[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = i;
+}
+
+__global__ void k_zero_i32(int32_t* p) { p[0] = 0; }
 
 /* Morton-ordered copies of what the acceleration loop reads for every
  * candidate, packed so that the range test needs one 16-byte load:
@@ -832,6 +869,20 @@ Cells make_cells(const sph_ctx* ctx) {
 
 }  // namespace
 
+/* The sorted slots of the active particles, rebuilt only after a sort or a
+ * gravity pass (which set the order and the activity). */
+int launch_active_list(sph_ctx* ctx) {
+    if (ctx->slot_list_valid) return SPH_OK;
+    if (!ctx->slot_list) SPH_HIP_TRY(hipMalloc((void**)&ctx->slot_list, sizeof(int32_t) * (size_t)ctx->n));
+    if (!ctx->slot_count) SPH_HIP_TRY(hipMalloc((void**)&ctx->slot_count, sizeof(int32_t)));
+    k_zero_i32<<<1, 1, 0, ctx->stream>>>(ctx->slot_count);
+    k_active_slots<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, ctx->pairs_a, ctx->active, ctx->slot_list,
+                                                                  ctx->slot_count);
+    SPH_HIP_TRY(hipGetLastError());
+    ctx->slot_list_valid = true;
+    return SPH_OK;
+}
+
 int launch_density(sph_ctx* ctx) {
     const sph_params& p = ctx->prm;
     DensityArgs a;
@@ -868,6 +919,15 @@ int launch_density(sph_ctx* ctx) {
     const float dtmin1 = p.dt0 / (float)(1 << p.r_max);
     a.dt_reset = (int)(p.dt0 * (1.f / dtmin1));
     a.guess = ctx->learned ? ctx->guess : nullptr;
+    /* With individual timesteps most particles are inactive: their EOS is
+     * cheaper in original order than through the sorted slots.  (A compacted
+     * list of the active slots would leave too few waves to hide the
+     * neighbour loops' latency.) */
+    a.skip_inactive = p.activity_rule == SPH_ACTIVE_README ? 1 : 0;
+    if (a.skip_inactive) {
+        k_inactive_eos<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(a);
+        SPH_HIP_TRY(hipGetLastError());
+    }
     const int threads = 256;
     const int dsel = p.density_lanes * 2 + (p.lane_split ? 1 : 0);
     switch (dsel) {

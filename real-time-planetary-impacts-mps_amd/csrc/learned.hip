@@ -992,36 +992,14 @@ int sph_samples_clear(sph_samples* s) {
 }  // extern "C"
 
 /* --------------------------------------------------- density-pass hooks */
-/* Sorted slots of the active particles: one ballot and one counter update
- * per wave (the list order across waves is not fixed; every row's result
- * depends on its own slot only). */
-__global__ __launch_bounds__(256) void k_active_slots(int n, const uint2* __restrict__ sorted,
-                                                      const uint8_t* __restrict__ active, int32_t* list,
-                                                      int32_t* count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < n && active[sorted[i].y];
-    const uint64_t b = __ballot(act);
-    const int lane = threadIdx.x & 63;
-    int base = 0;
-    if (lane == 0 && b) base = atomicAdd(count, (int)__popcll(b));
-    base = __shfl(base, 0, 64);
-    if (act) list[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = i;
-}
-
-__global__ void k_zero_i32(int32_t* p) { p[0] = 0; }
-
 int launch_features(sph_ctx* ctx, bool active_only = false) {
     if (!ctx->feats) SPH_HIP_TRY(hipMalloc((void**)&ctx->feats, sizeof(float4) * 3 * (size_t)ctx->n));
     FeatArgs a;
     a.list = nullptr;
     a.count = nullptr;
     if (active_only) {
-        if (!ctx->slot_list) SPH_HIP_TRY(hipMalloc((void**)&ctx->slot_list, sizeof(int32_t) * (size_t)ctx->n));
-        if (!ctx->slot_count) SPH_HIP_TRY(hipMalloc((void**)&ctx->slot_count, sizeof(int32_t)));
-        k_zero_i32<<<1, 1, 0, ctx->stream>>>(ctx->slot_count);
-        k_active_slots<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, ctx->pairs_a, ctx->active,
-                                                                      ctx->slot_list, ctx->slot_count);
-        SPH_HIP_TRY(hipGetLastError());
+        const int rc = launch_active_list(ctx);
+        if (rc != SPH_OK) return rc;
         a.list = ctx->slot_list;
         a.count = ctx->slot_count;
     }

@@ -144,6 +144,7 @@ struct sph_ctx {
     float* guess = nullptr;         /* starting h per sorted slot */
     int32_t* slot_list = nullptr;   /* sorted slots of the active particles (README rule) */
     int32_t* slot_count = nullptr;  /* their number (device) */
+    bool slot_list_valid = false;   /* cleared by a sort and by a gravity pass */
     sph_mlp* learned = nullptr;     /* not owned */
     sph_samples* dump = nullptr;    /* not owned */
 };
@@ -166,6 +167,8 @@ int launch_density_epilogue(sph_ctx* ctx);
 /* gravity.hip */
 int launch_tree_convert(sph_ctx* ctx);
 int launch_gravity_up(sph_ctx* ctx);
+/* sorted slots of the active particles into slot_list / slot_count (README rule) */
+int launch_active_list(sph_ctx* ctx);
 /* bootstrap: the as-written P = 2 down pass (leaves force-accept), used by
  * the near-field form to estimate |a| before its first evaluation */
 int launch_gravity_down(sph_ctx* ctx, bool bootstrap = false);
