@@ -408,6 +408,8 @@ def main():
     if passes:
         per_pass = {k: v["ms"] / max(v["calls"], 1) for k, v in passes.items()}
         out["pass_ms"] = {k: round(v, 4) for k, v in per_pass.items()}
+        out["pass_ms_note"] = ("gravity_down runs on its own stream, concurrently with density; its events span "
+                               "that overlap, so the passes do not add up to ms_per_step")
         out["density_ms"] = round(per_pass["density"], 4)
         # dominant kernel: the density pass is one kernel
         kname = (f"k_density_group<{int(prm.density_lanes)}, {int(prm.lane_split)}>" if prm.density_lanes > 0

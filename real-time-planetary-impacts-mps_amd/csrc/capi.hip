@@ -79,6 +79,7 @@ void free_tree(sph_tree_dev& T) {
 
 void destroy_ctx(sph_ctx* ctx) {
     if (!ctx) return;
+    if (ctx->gstream) (void)hipStreamSynchronize(ctx->gstream);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
     float4** f4[] = {&ctx->pos,    &ctx->vel,    &ctx->acc,   &ctx->s_posm, &ctx->s_vel,
@@ -113,6 +114,9 @@ void destroy_ctx(sph_ctx* ctx) {
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (auto e : ctx->ev_open)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->ev_up) (void)hipEventDestroy(ctx->ev_up);
+    if (ctx->ev_down) (void)hipEventDestroy(ctx->ev_down);
+    if (ctx->gstream) (void)hipStreamDestroy(ctx->gstream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     delete ctx;
@@ -212,7 +216,15 @@ int ctx_end_pass(sph_ctx* ctx, int pass) {
     return SPH_OK;
 }
 
+int join_gravity(sph_ctx* ctx) {
+    if (!ctx->grav_pending) return SPH_OK;
+    ctx->grav_pending = false;
+    SPH_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_down, 0));
+    return SPH_OK;
+}
+
 int ctx_fold_times(sph_ctx* ctx) {
+    SPH_TRY_RC(join_gravity(ctx));
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (auto& t : ctx->ev_pending) {
         float ms = 0.f;
@@ -256,6 +268,9 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(hipGetDevice(&ctx->device));
     TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     TRY(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    TRY(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
+    TRY(hipEventCreateWithFlags(&ctx->ev_up, hipEventDisableTiming));
+    TRY(hipEventCreateWithFlags(&ctx->ev_down, hipEventDisableTiming));
     TRY(hipEventCreateWithFlags(&ctx->snap_done, hipEventDisableTiming));
     TRY(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming));
     for (auto& e : ctx->tree.released) TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -351,6 +366,7 @@ void sph_host_free(void* p) {
 
 int sph_snapshot_positions(sph_ctx* ctx, float* pinned_dst4) {
     if (!ctx || !pinned_dst4) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     const size_t bytes = 16 * (size_t)ctx->n;
     SPH_HIP_TRY(hipMemcpyAsync(ctx->snap, ctx->pos, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     SPH_HIP_TRY(hipEventRecord(ctx->snap_done, ctx->stream));
@@ -370,6 +386,8 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const i
                     const int32_t* level_counts, int n_levels, int32_t node_count) {
     if (!ctx || !octree || !level_nodes || !level_counts || n_levels < 1 || node_count < 1 || n_ints < 2)
         return SPH_E_ARG;
+    /* the tree buffers' release events must cover a down pass still running */
+    SPH_TRY_RC(join_gravity(ctx));
     sph_tree_dev& T = ctx->tree;
     int64_t total = 0;
     int32_t max_nonlast = 0;
@@ -486,6 +504,7 @@ int sph_tree_upload_done(sph_ctx* ctx) {
 
 int sph_sort(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     ctx->slot_list_valid = false;
     int rc = ctx_begin_pass(ctx, SPH_PASS_SORT);
     if (rc == SPH_OK) rc = launch_sort(ctx);
@@ -496,6 +515,7 @@ int sph_sort(sph_ctx* ctx) {
 int sph_gravity(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
     if (!ctx->have_tree) return ctx_fail(ctx, SPH_E_NOTREE, "gravity before any tree upload");
+    SPH_TRY_RC(join_gravity(ctx));
     /* the near-field form checks the node numbering in the up pass */
     if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ctx->tree_check_pending = true;
     ctx->slot_list_valid = false; /* the up pass sets the activity */
@@ -511,10 +531,22 @@ int sph_gravity(sph_ctx* ctx) {
     }
     if (rc == SPH_OK) rc = launch_gravity_up(ctx);
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_GRAVITY_UP);
-    if (rc == SPH_OK) rc = ctx_begin_pass(ctx, SPH_PASS_GRAVITY_DOWN);
+    /* the down pass on gstream, after the up pass; the density pass may run
+     * meanwhile on the context stream */
+    if (rc == SPH_OK) rc = (int)hipEventRecord(ctx->ev_up, ctx->stream);
+    if (rc == SPH_OK) rc = (int)hipStreamWaitEvent(ctx->gstream, ctx->ev_up, 0);
+    if (rc != SPH_OK) return rc;
+    hipStream_t main_stream = ctx->stream;
+    ctx->stream = ctx->gstream;
+    rc = ctx_begin_pass(ctx, SPH_PASS_GRAVITY_DOWN);
     if (rc == SPH_OK) rc = launch_gravity_down(ctx);
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_GRAVITY_DOWN);
-    if (rc == SPH_OK) ctx->gravnorm_known = true;
+    ctx->stream = main_stream;
+    if (rc == SPH_OK) rc = (int)hipEventRecord(ctx->ev_down, ctx->gstream);
+    if (rc == SPH_OK) {
+        ctx->grav_pending = true;
+        ctx->gravnorm_known = true;
+    }
     return rc;
 }
 
@@ -531,6 +563,7 @@ int sph_density(sph_ctx* ctx) {
 
 int sph_acceleration(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     if (!ctx->have_prev) return ctx_fail(ctx, SPH_E_STATE, "acceleration before any sort");
     int rc = ctx_begin_pass(ctx, SPH_PASS_ACCELERATION);
     if (rc == SPH_OK) rc = launch_acceleration(ctx);
@@ -540,6 +573,7 @@ int sph_acceleration(sph_ctx* ctx) {
 
 int sph_step(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     int rc = ctx_begin_pass(ctx, SPH_PASS_STEP);
     if (rc == SPH_OK) rc = launch_step(ctx);
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_STEP);
@@ -548,6 +582,7 @@ int sph_step(sph_ctx* ctx) {
 
 int sph_sync(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     SPH_HIP_TRY(hipGetLastError());
     if (ctx->tree_check_pending) {
@@ -574,6 +609,7 @@ int sph_download(sph_ctx* ctx, int field, void* dst) {
     if (!ctx || !dst) return SPH_E_ARG;
     const int64_t bytes = field_bytes(ctx, field);
     if (bytes < 0) return ctx_fail(ctx, SPH_E_ARG, "unknown field");
+    SPH_TRY_RC(join_gravity(ctx));
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (field == SPH_F_CELL_START || field == SPH_F_CELL_END) {
         std::vector<uint2> se((size_t)ctx->ncells);
@@ -636,6 +672,7 @@ int sph_download(sph_ctx* ctx, int field, void* dst) {
 
 int sph_upload(sph_ctx* ctx, int field, const void* src) {
     if (!ctx || !src) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     const int64_t bytes = field_bytes(ctx, field);
     if (bytes < 0) return ctx_fail(ctx, SPH_E_ARG, "unknown field");
     if (field == SPH_F_CELL_ARRAY || field == SPH_F_CELL_START || field == SPH_F_CELL_END ||

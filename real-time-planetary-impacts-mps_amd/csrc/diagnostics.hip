@@ -58,6 +58,7 @@ extern "C" {
 
 int sph_conservation(sph_ctx* ctx, double* out) {
     if (!ctx || !out) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     double* part = nullptr;
     SPH_HIP_TRY(hipMalloc((void**)&part, sizeof(double) * kBlocks * kTerms));
     k_conserve<<<kBlocks, kThreads, 0, ctx->stream>>>(ctx->n, ctx->pos, ctx->vel, ctx->m, ctx->u, ctx->phi, part);
@@ -77,6 +78,7 @@ int sph_conservation(sph_ctx* ctx, double* out) {
 
 int sph_time(sph_ctx* ctx, double* seconds) {
     if (!ctx || !seconds) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     int32_t ticks = 0;
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     SPH_HIP_TRY(hipMemcpy(&ticks, ctx->time, 4, hipMemcpyDeviceToHost));

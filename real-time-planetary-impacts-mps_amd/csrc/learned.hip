@@ -1078,6 +1078,7 @@ int sph_set_learned_guess(sph_ctx* ctx, sph_mlp* m) {
 
 int sph_features(sph_ctx* ctx, float* dst) {
     if (!ctx || !dst) return SPH_E_ARG;
+    SPH_TRY_RC(join_gravity(ctx));
     if (!ctx->have_prev) return ctx_fail(ctx, SPH_E_STATE, "features before any sort");
     int rc = launch_features(ctx);
     if (rc != SPH_OK) return rc;

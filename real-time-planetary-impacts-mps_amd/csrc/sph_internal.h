@@ -40,6 +40,12 @@
 
 #include "sph_capi.h"
 
+/* propagates an SPH_* return code */
+#define SPH_TRY_RC(expr)                   \
+    do {                                   \
+        const int _rc = (expr);            \
+        if (_rc != SPH_OK) return _rc;     \
+    } while (0)
 #define SPH_HIP_TRY(expr)                                              \
     do {                                                               \
         hipError_t _e = (expr);                                        \
@@ -81,6 +87,12 @@ struct sph_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;
+    /* the gravity down pass runs on its own stream, concurrently with the
+     * density pass (which neither reads nor writes what it does); every other
+     * call first makes `stream` wait for it (join_gravity) */
+    hipStream_t gstream = nullptr;
+    hipEvent_t ev_up = nullptr, ev_down = nullptr;
+    bool grav_pending = false;
     std::string err;
 
     /* particle state, original order */
@@ -167,6 +179,8 @@ int launch_density_epilogue(sph_ctx* ctx);
 /* gravity.hip */
 int launch_tree_convert(sph_ctx* ctx);
 int launch_gravity_up(sph_ctx* ctx);
+/* makes ctx->stream wait for a gravity down pass still running on gstream */
+int join_gravity(sph_ctx* ctx);
 /* sorted slots of the active particles into slot_list / slot_count (README rule) */
 int launch_active_list(sph_ctx* ctx);
 /* bootstrap: the as-written P = 2 down pass (leaves force-accept), used by
