@@ -336,6 +336,38 @@ __device__ __forceinline__ int gsum_i(int v) {
  * The terms are the reference's; only the order of the additions differs
  * (per-cell partial sums, then a tree), which changes results at the
  * rounding level. */
+/* Calls body(key) for the cells first, first + STEP, ... of the box
+ * [mn, mx] in x -> y -> z order (z fastest), with the Morton key of the
+ * periodic cell.  The position in the box advances by carries instead of a
+ * division per cell, and the x/y part of the key is recomputed only when x
+ * or y changes. */
+template <int STEP, class Body>
+__device__ __forceinline__ void box_cells(const int mn[3], const int mx[3], int first, int mask, Body body) {
+    const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
+    const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
+    if (first >= ncell) return;
+    int iz = first % nz;
+    const int t = first / nz;
+    int iy = t % ny, ix = t / ny;
+    uint32_t exy = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
+                   (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1);
+    for (int c = first; c < ncell; c += STEP) {
+        body(exy | (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2));
+        iz += STEP;
+        if (iz >= nz) {
+            do {
+                iz -= nz;
+                if (++iy >= ny) {
+                    iy = 0;
+                    ++ix;
+                }
+            } while (iz >= nz);
+            exy = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
+                  (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1);
+        }
+    }
+}
+
 template <int G, int SPLIT>
 __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     constexpr int PPB = 256 / G;
@@ -388,18 +420,11 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
         const float hsum = h_i + h_i;
         int mn[3], mx[3];
         cell_box(position, cs, hb, mask + 1, mn, mx);
-        const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
-        const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
         float part = 0.f;
         int cnt = 0;
-        for (int c = SPLIT ? 0 : lane; c < ncell; c += SPLIT ? 1 : G) {
-            const int iz = c % nz, t = c / nz;
-            const int iy = t % ny, ix = t / ny;
-            const uint32_t key = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
-                                 (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1) |
-                                 (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
+        box_cells<SPLIT ? 1 : G>(mn, mx, SPLIT ? 0 : lane, mask, [&](uint32_t key) {
             const uint2 se = cse[key];
-            if (se.x == 0xFFFFFFFFu) continue;
+            if (se.x == 0xFFFFFFFFu) return;
             if (!SPLIT || lane == 0) w_cand += (int)(se.y - se.x + 1);
             for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) {
                 const float4 q = spm[k];
@@ -415,7 +440,7 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
                 }
                 part += kW(r, h1) * q.w;
             }
-        }
+        });
         density = gsum<G>(part);
         if (!use_cache) {
             nN = min(gsum_i<G>(cnt), a.ne);
@@ -458,16 +483,9 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
         const float hsum = h_i + h_i;
         int mn[3], mx[3];
         cell_box(position, cs, hb, mask + 1, mn, mx);
-        const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
-        const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
-        for (int c = SPLIT ? 0 : lane; c < ncell; c += SPLIT ? 1 : G) {
-            const int iz = c % nz, t = c / nz;
-            const int iy = t % ny, ix = t / ny;
-            const uint32_t key = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
-                                 (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1) |
-                                 (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
+        box_cells<SPLIT ? 1 : G>(mn, mx, SPLIT ? 0 : lane, mask, [&](uint32_t key) {
             const uint2 se = cse[key];
-            if (se.x == 0xFFFFFFFFu) continue;
+            if (se.x == 0xFFFFFFFFu) return;
             if (!SPLIT || lane == 0) f_cand += (int)(se.y - se.x + 1);
             for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) {
                 const float4 q = spm[k];
@@ -492,7 +510,7 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
                     cz += mass * (vx * gy - vy * gx);
                 }
             }
-        }
+        });
     }
     /* as written, a rescan adds to the last NR density (Q7) unless fixed */
     if (!cached) density = (a.fix_cache ? 0.f : density) + gsum<G>(dpart);
