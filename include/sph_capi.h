@@ -192,6 +192,24 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints,
 /* 1 once the last uploaded tree's host buffers may be reused, else 0. */
 int sph_tree_upload_done(sph_ctx* ctx);
 
+/* Builds the octree on the GPU from the particles' current positions, in
+ * place of a CPU build (octree.cpp:14-141) and its upload: no counterpart in
+ * the reference.  The tree splits the positions' bounding cube at its
+ * midpoints (Morton keys of 10 bits per axis) rather than octree.cpp's
+ * particle means, with the same leaf size (max_children_in_leaf) and depth
+ * limit (max_tree_depth, at most 10 here), and is laid out as octree.cpp lays
+ * out its own: flat records, pre-order node ids, one offset list per level.
+ * Passes enqueued after this call use it.  SPH_E_ARG for a non-finite
+ * position or max_tree_depth > 10.  Blocks until the tree's size is known. */
+int sph_build_tree(sph_ctx* ctx);
+/* Size of the current tree: octreeData ints, nodes, levels, and the first
+ * `level_cap` level sizes into `level_counts` (may be NULL). */
+int sph_tree_info(sph_ctx* ctx, int64_t* n_ints, int32_t* node_count, int32_t* n_levels,
+                  int32_t* level_counts, int32_t level_cap);
+/* Copies the current tree to the host: `octree` (n_ints ints) and the
+ * concatenated level lists `level_nodes` (node_count ints). */
+int sph_download_tree(sph_ctx* ctx, int32_t* octree, int32_t* level_nodes);
+
 /* ----------------------------------------------------------------- passes */
 int sph_sort(sph_ctx* ctx);          /* Compute::sort               Compute.cpp:265-326 */
 int sph_gravity(sph_ctx* ctx);       /* Compute::gravitationalPass  Compute.cpp:328-377 */

@@ -86,6 +86,7 @@ void sph_octree_free(sph_octree* t);
 #define SPH_TREE_ASYNC 0   /* rebuild on a CPU thread, swap when ready (Compute.cpp:456-508) */
 #define SPH_TREE_EVERY 1   /* rebuild from the current positions before every step */
 #define SPH_TREE_FIXED 2   /* keep the first tree */
+#define SPH_TREE_GPU 3     /* rebuild on the GPU (sph_build_tree) before every step; no counterpart in the reference */
 
 typedef struct sph_headless sph_headless;
 

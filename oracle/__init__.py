@@ -83,6 +83,8 @@ def lib() -> ctypes.CDLL:
                                      ctypes.c_float, _P(ctypes.c_float)]
         L.orc_eos_sample.restype = None
         L.orc_build_octree.argtypes = [_P(ctypes.c_float), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(OrcTree)]
+        L.orc_build_octree_morton.argtypes = [_P(ctypes.c_float), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              _P(OrcTree)]
         L.orc_free_tree.argtypes = [_P(OrcTree)]
         L.orc_free_tree.restype = None
         for name in ("orc_gravity_up", "orc_gravity_down"):
@@ -244,6 +246,27 @@ def build_octree(pos4: np.ndarray, max_leaf: int = 4, max_depth: int = 7) -> Dic
     t = OrcTree()
     if L.orc_build_octree(_p(pos4), pos4.shape[0], max_leaf, max_depth, ctypes.byref(t)) != 0:
         raise ValueError("build_octree: non-finite particle position")
+    try:
+        counts = np.ctypeslib.as_array(t.level_counts, shape=(t.n_levels,)).copy()
+        return {
+            "octree": np.ctypeslib.as_array(t.octree, shape=(t.n_ints,)).copy(),
+            "level_nodes": np.ctypeslib.as_array(t.level_nodes, shape=(int(counts.sum()),)).copy(),
+            "level_counts": counts,
+            "node_count": int(t.node_count),
+        }
+    finally:
+        L.orc_free_tree(ctypes.byref(t))
+
+
+def build_octree_morton(pos4: np.ndarray, max_leaf: int = 4, max_depth: int = 7) -> Dict[str, object]:
+    """The build's GPU octree restated (Morton keys of the bounding cube), in
+    octree.cpp's flat layout."""
+    L = lib()
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    t = OrcTree()
+    rc = L.orc_build_octree_morton(_p(pos4), pos4.shape[0], max_leaf, max_depth, ctypes.byref(t))
+    if rc != 0:
+        raise ValueError(f"build_octree_morton failed ({rc})")
     try:
         counts = np.ctypeslib.as_array(t.level_counts, shape=(t.n_levels,)).copy()
         return {

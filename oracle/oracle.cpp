@@ -1334,6 +1334,105 @@ int orc_build_octree(const float* pos4, int32_t n, int32_t max_leaf, int32_t max
     return 0;
 }
 
+/* Octree over Morton keys (the build's GPU tree, not the reference's): the
+ * positions' bounding cube is quantised to 2^10 cells per axis (FP64, no
+ * fused operations), each particle gets the 30-bit key x | y << 1 | z << 2
+ * interleaved, particles are sorted by (key, index), and a node at level L
+ * holds the particles sharing the key's top 3 L bits.  A node is a leaf when
+ * it holds at most max_leaf particles or L == max_depth; children are the
+ * octants of level L + 1 in key order.  The flat array, the pre-order node
+ * numbering and the level lists are laid out as octree.cpp lays out its own
+ * tree. */
+namespace {
+uint32_t spread10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+int morton_build(const std::vector<uint32_t>& key, const std::vector<int>& idx, int s, int e, int level,
+                 int maxLeaf, int maxDepth, std::vector<int>& data, std::vector<std::vector<int>>& levels,
+                 int& nodeValues) {
+    const int nodeStart = (int)data.size();
+    if (level >= (int)levels.size()) levels.resize(level + 1);
+    levels[level].push_back(nodeStart);
+    if (e - s <= maxLeaf || level == maxDepth) {
+        data.push_back(e - s);
+        data.push_back(nodeValues++);
+        for (int j = s; j < e; ++j) data.push_back(idx[j]);
+        return nodeStart;
+    }
+    for (int i = 0; i < 10; i++) data.push_back(-1);
+    data[nodeStart] = 0;
+    data[nodeStart + 1] = nodeValues++;
+    const int shift = 30 - 3 * (level + 1);
+    int j = s;
+    for (int c = 0; c < 8; ++c) {
+        const int b = j;
+        while (j < e && (int)((key[j] >> shift) & 7u) == c) ++j;
+        if (j > b) data[nodeStart + 2 + c] = morton_build(key, idx, b, j, level + 1, maxLeaf, maxDepth, data, levels,
+                                                          nodeValues);
+    }
+    return nodeStart;
+}
+}  // namespace
+
+int orc_build_octree_morton(const float* pos4, int32_t n, int32_t max_leaf, int32_t max_depth, orc_tree* out) {
+    std::memset(out, 0, sizeof(*out));
+    if (n <= 0) return 0;
+    if (max_depth < 0 || max_depth > 10 || max_leaf < 1) return -2;
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = pos4[a];
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const float v = pos4[4 * i + a];
+            if (!std::isfinite(v)) return -1;
+            lo[a] = std::min(lo[a], v);
+            hi[a] = std::max(hi[a], v);
+        }
+    float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(ext > 0.f)) ext = 1.f;
+    const double scale = 1024.0 / (double)ext;
+    std::vector<uint32_t> key0(n);
+    for (int i = 0; i < n; ++i) {
+        uint32_t q[3];
+        for (int a = 0; a < 3; ++a) {
+            const double t = ((double)pos4[4 * i + a] - (double)lo[a]) * scale;
+            const int v = (int)std::floor(t);
+            q[a] = (uint32_t)std::min(std::max(v, 0), 1023);
+        }
+        key0[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key0[a] < key0[b]; });
+    std::vector<uint32_t> key(n);
+    for (int i = 0; i < n; ++i) key[i] = key0[idx[i]];
+    std::vector<int> data;
+    std::vector<std::vector<int>> levels;
+    int nodeValues = 0;
+    morton_build(key, idx, 0, n, 0, max_leaf, max_depth, data, levels, nodeValues);
+    out->n_ints = (int64_t)data.size();
+    out->octree = (int32_t*)std::malloc(sizeof(int32_t) * data.size());
+    std::memcpy(out->octree, data.data(), sizeof(int32_t) * data.size());
+    out->n_levels = (int32_t)levels.size();
+    out->level_counts = (int32_t*)std::malloc(sizeof(int32_t) * levels.size());
+    size_t total = 0;
+    for (auto& l : levels) total += l.size();
+    out->level_nodes = (int32_t*)std::malloc(sizeof(int32_t) * total);
+    size_t off = 0;
+    for (size_t l = 0; l < levels.size(); ++l) {
+        out->level_counts[l] = (int32_t)levels[l].size();
+        std::memcpy(out->level_nodes + off, levels[l].data(), sizeof(int32_t) * levels[l].size());
+        off += levels[l].size();
+    }
+    out->node_count = nodeValues;
+    return 0;
+}
+
 void orc_free_tree(orc_tree* t) {
     std::free(t->octree);
     std::free(t->level_nodes);

@@ -91,6 +91,10 @@ typedef struct orc_tree {
 /* Builds into library-owned storage; free with orc_free_tree. */
 int orc_build_octree(const float* pos4, int32_t n, int32_t max_leaf, int32_t max_depth, orc_tree* out);
 void orc_free_tree(orc_tree* t);
+/* The build's GPU tree restated: an octree over 30-bit Morton keys of the
+ * bounding cube (max_depth <= 10), in octree.cpp's flat layout.  -1 for a
+ * non-finite coordinate, -2 for bad limits. */
+int orc_build_octree_morton(const float* pos4, int32_t n, int32_t max_leaf, int32_t max_depth, orc_tree* out);
 
 /* ---- gravity/** ---- */
 /* Records: 40 floats per multipole and 32 per local; the first 24 / 16 are

@@ -32,7 +32,7 @@ SPH_FIX_INACTIVE_DRIFT = 64
 SPH_FIX_ALL = 0x7F
 SPH_ACTIVE_AS_WRITTEN = 0
 SPH_ACTIVE_README = 1
-TREE_ASYNC, TREE_EVERY, TREE_FIXED = 0, 1, 2
+TREE_ASYNC, TREE_EVERY, TREE_FIXED, TREE_GPU = 0, 1, 2, 3
 
 FIELDS = [
     "pos", "vel", "acc", "rho", "u", "mass", "pressure", "h", "mat", "gradterm",
@@ -112,6 +112,7 @@ CAPI_SYMBOLS = [
     "sph_mlp_loss_grad", "sph_mlp_fit_norm", "sph_mlp_train", "sph_samples_create", "sph_samples_destroy",
     "sph_samples_count", "sph_samples_read", "sph_samples_append", "sph_samples_clear", "sph_set_sample_dump",
     "sph_set_learned_guess", "sph_features", "sph_conservation", "sph_time",
+    "sph_build_tree", "sph_tree_info", "sph_download_tree",
 ]
 HOST_SYMBOLS = [
     "sph_tillotson_granite", "sph_tillotson_iron", "sph_make_eos_table", "sph_impact_config_default",
@@ -152,7 +153,7 @@ def libs():
         h.sph_last_error.restype = ctypes.c_char_p
         h.sph_num_particles.argtypes = [ctypes.c_void_p]
         for name in ("sph_sort", "sph_gravity", "sph_density", "sph_acceleration", "sph_step", "sph_sync",
-                     "sph_wait_snapshot", "sph_tree_upload_done", "sph_reset_pass_times"):
+                     "sph_wait_snapshot", "sph_tree_upload_done", "sph_reset_pass_times", "sph_build_tree"):
             getattr(h, name).argtypes = [ctypes.c_void_p]
         h.sph_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         h.sph_pass_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
@@ -163,6 +164,9 @@ def libs():
         h.sph_upload_tree.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int,
                                       ctypes.c_int32]
+        I32 = ctypes.POINTER(ctypes.c_int32)
+        h.sph_tree_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), I32, I32, I32, ctypes.c_int32]
+        h.sph_download_tree.argtypes = [ctypes.c_void_p, I32, I32]
         h.sph_snapshot_positions.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         h.sph_host_alloc.argtypes = [ctypes.c_size_t]
         h.sph_host_alloc.restype = ctypes.c_void_p
@@ -359,6 +363,24 @@ class Context:
                                      _fp(lc, ctypes.c_int32), lc.size, int(tree["node_count"]))
         self._check(rc, "sph_upload_tree")
         self.sync()  # pageable source buffers: wait until the copy has landed
+
+    def build_tree(self):
+        """Octree built on the GPU from the current positions (sph_build_tree)."""
+        self._check(self._h.sph_build_tree(self.ptr), "sph_build_tree")
+
+    def download_tree(self) -> Dict[str, object]:
+        """The current tree in the layout upload_tree takes."""
+        n_ints, node_count, n_levels = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        lc = np.zeros(64, np.int32)
+        rc = self._h.sph_tree_info(self.ptr, ctypes.byref(n_ints), ctypes.byref(node_count), ctypes.byref(n_levels),
+                                   _fp(lc, ctypes.c_int32), lc.size)
+        self._check(rc, "sph_tree_info")
+        oc = np.zeros(n_ints.value, np.int32)
+        ln = np.zeros(node_count.value, np.int32)
+        self._check(self._h.sph_download_tree(self.ptr, _fp(oc, ctypes.c_int32), _fp(ln, ctypes.c_int32)),
+                    "sph_download_tree")
+        return {"octree": oc, "level_nodes": ln, "level_counts": lc[:min(n_levels.value, lc.size)].copy(),
+                "node_count": int(node_count.value)}
 
     def _shape_dtype(self, field: str, nbytes: int):
         if field in _F4:

@@ -16,17 +16,6 @@
 
 namespace {
 
-template <typename T>
-hipError_t dalloc(T** p, size_t count) {
-    return hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
-}
-
-template <typename T>
-void dfree(T*& p) {
-    if (p) (void)hipFree((void*)p);
-    p = nullptr;
-}
-
 bool valid_params(const sph_params* p, std::string& why) {
     if (p->cell_power < 1 || p->cell_power > 8) { why = "cell_power must be 1..8"; return false; }
     if (!(p->cell_width > 0.f)) { why = "cell_width must be > 0"; return false; }
@@ -105,6 +94,7 @@ void destroy_ctx(sph_ctx* ctx) {
     dfree(ctx->eos[0]);
     dfree(ctx->eos[1]);
     free_tree(ctx->tree);
+    sph_treebuild_free(ctx->tb);
     if (ctx->snap_done) (void)hipEventDestroy(ctx->snap_done);
     if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
     for (auto& t : ctx->ev_pending) {
@@ -382,12 +372,10 @@ int sph_wait_snapshot(sph_ctx* ctx) {
     return SPH_OK;
 }
 
-int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const int32_t* level_nodes,
-                    const int32_t* level_counts, int n_levels, int32_t node_count) {
-    if (!ctx || !octree || !level_nodes || !level_counts || n_levels < 1 || node_count < 1 || n_ints < 2)
-        return SPH_E_ARG;
-    /* the tree buffers' release events must cover a down pass still running */
-    SPH_TRY_RC(join_gravity(ctx));
+}  // extern "C"
+
+int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int n_levels, int32_t node_count,
+                 int* slot) {
     sph_tree_dev& T = ctx->tree;
     int64_t total = 0;
     int32_t max_nonlast = 0;
@@ -463,14 +451,15 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const i
             ctx->list_cap = cap;
         }
     }
-    /* upload into the buffer the current tree does not use, on the copy
-     * stream, once the compute stream has released it */
-    const int k = ctx->have_tree ? (T.up_cur ^ 1) : T.up_cur;
-    SPH_HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, T.released[k], 0));
-    SPH_HIP_TRY(hipMemcpyAsync(T.octree_up[k], octree, 4 * (size_t)n_ints, hipMemcpyHostToDevice, ctx->copy_stream));
-    SPH_HIP_TRY(hipMemcpyAsync(T.level_tp_up[k], level_nodes, 4 * (size_t)total, hipMemcpyHostToDevice, ctx->copy_stream));
-    SPH_HIP_TRY(hipEventRecord(ctx->upload_done, ctx->copy_stream));
-    SPH_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->upload_done, 0));
+    /* the buffer the current tree does not use */
+    *slot = ctx->have_tree ? (T.up_cur ^ 1) : T.up_cur;
+    return SPH_OK;
+}
+
+int tree_commit(sph_ctx* ctx, int k, int64_t n_ints, const int32_t* level_counts, int n_levels, int32_t node_count) {
+    sph_tree_dev& T = ctx->tree;
+    int32_t max_nonlast = 0;
+    for (int l = 0; l + 1 < n_levels; ++l) max_nonlast = std::max(max_nonlast, level_counts[l]);
     /* every pass enqueued from here on uses buffer k */
     if (ctx->have_tree && k != T.up_cur) SPH_HIP_TRY(hipEventRecord(T.released[T.up_cur], ctx->stream));
     T.up_cur = k;
@@ -485,8 +474,30 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const i
     int rc = launch_tree_convert(ctx);
     if (rc != SPH_OK) return rc;
     ctx->have_tree = true;
-    ctx->upload_pending = true;
     ctx->tree_check_pending = true;
+    return SPH_OK;
+}
+
+extern "C" {
+
+int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const int32_t* level_nodes,
+                    const int32_t* level_counts, int n_levels, int32_t node_count) {
+    if (!ctx || !octree || !level_nodes || !level_counts || n_levels < 1 || node_count < 1 || n_ints < 2)
+        return SPH_E_ARG;
+    /* the tree buffers' release events must cover a down pass still running */
+    SPH_TRY_RC(join_gravity(ctx));
+    sph_tree_dev& T = ctx->tree;
+    int k = 0;
+    SPH_TRY_RC(tree_reserve(ctx, n_ints, level_counts, n_levels, node_count, &k));
+    /* upload on the copy stream, once the compute stream has released buffer k */
+    SPH_HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, T.released[k], 0));
+    SPH_HIP_TRY(hipMemcpyAsync(T.octree_up[k], octree, 4 * (size_t)n_ints, hipMemcpyHostToDevice, ctx->copy_stream));
+    SPH_HIP_TRY(hipMemcpyAsync(T.level_tp_up[k], level_nodes, 4 * (size_t)node_count, hipMemcpyHostToDevice,
+                               ctx->copy_stream));
+    SPH_HIP_TRY(hipEventRecord(ctx->upload_done, ctx->copy_stream));
+    SPH_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->upload_done, 0));
+    SPH_TRY_RC(tree_commit(ctx, k, n_ints, level_counts, n_levels, node_count));
+    ctx->upload_pending = true;
     return SPH_OK;
 }
 

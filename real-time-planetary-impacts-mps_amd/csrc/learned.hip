@@ -660,16 +660,6 @@ int mlp_fail(sph_mlp* m, int code, const std::string& msg) {
         if (_e != hipSuccess) return mlp_fail(m, (int)_e, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
-template <typename T>
-hipError_t dalloc(T** p, size_t count) {
-    return hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
-}
-template <typename T>
-void dfree(T*& p) {
-    if (p) (void)hipFree((void*)p);
-    p = nullptr;
-}
-
 int ensure_scratch(sph_mlp* m, int64_t rows) {
     if (rows <= m->scratch_cap) return SPH_OK;
     dfree(m->scratch_rec);

@@ -229,6 +229,30 @@ int launch_fill_cells(sph_ctx* ctx) {
     return SPH_OK;
 }
 
+/* LSD radix sort of (key, index) pairs by the low `passes` bytes of the key,
+ * stable (ties keep their order), ping-ponging between a and b.  Returns the
+ * buffer that holds the result through `out`. */
+int launch_sort_pairs(sph_ctx* ctx, uint2* a, uint2* b, int n, int passes, uint2** out) {
+    const int ntiles = (n + kTile - 1) / kTile;
+    if (ntiles > ctx->ntiles) return ctx_fail(ctx, SPH_E_STATE, "sort tile table too small");
+    hipStream_t s = ctx->stream;
+    uint2* bufs[2] = {a, b};
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        k_hist<<<ntiles, kThreads, 0, s>>>(bufs[cur], n, shift, ctx->tile_hist, ntiles);
+        SPH_HIP_TRY(hipGetLastError());
+        k_scan_rows<<<256, kThreads, 0, s>>>(ctx->tile_hist, ntiles, ctx->digit_total);
+        SPH_HIP_TRY(hipGetLastError());
+        k_scatter<<<ntiles, kThreads, 0, s>>>(bufs[cur], bufs[cur ^ 1], n, shift, ctx->tile_hist, ntiles,
+                                              ctx->digit_total);
+        SPH_HIP_TRY(hipGetLastError());
+        cur ^= 1;
+    }
+    *out = bufs[cur];
+    return SPH_OK;
+}
+
 int launch_sort(sph_ctx* ctx) {
     const int n = ctx->n;
     const int ntiles = (n + kTile - 1) / kTile;

@@ -52,6 +52,34 @@
         if (_e != hipSuccess) return ctx_fail_hip(ctx, _e, #expr);     \
     } while (0)
 
+template <typename T>
+inline hipError_t dalloc(T** p, size_t count) {
+    return hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+}
+
+template <typename T>
+inline void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+/* work buffers of the GPU octree build (treebuild.hip) */
+struct sph_treebuild {
+    uint2* pairs[2] = {nullptr, nullptr}; /* (Morton key, particle) */
+    float* bbox_part = nullptr;           /* 6 per workgroup */
+    float* box = nullptr;                 /* lo xyz, hi xyz */
+    double* scale = nullptr;              /* 1024 / cube side */
+    int2* nodes = nullptr;                /* [start, end) of every node, level L at L * n */
+    int32_t *cnt = nullptr, *scn = nullptr; /* children per node, their exclusive scan */
+    int32_t *lmin = nullptr, *lleaf = nullptr, *lcnt = nullptr; /* per sorted position */
+    int32_t *psize = nullptr, *pcount = nullptr, *pbase = nullptr, *pnbase = nullptr;
+    int32_t* bsum = nullptr;              /* scan block sums */
+    int32_t* scalars = nullptr;           /* flag, totals, scan length, level sizes */
+    int bbox_blocks = 0, bsum_cap = 0, n_cap = 0;
+    int64_t node_cap = 0;
+};
+void sph_treebuild_free(sph_treebuild& B);
+
 struct sph_tree_dev {
     int32_t* octree = nullptr;      /* raw octreeData (current) */
     int32_t* octree_up[2] = {nullptr, nullptr}; /* upload double buffer */
@@ -133,6 +161,8 @@ struct sph_ctx {
     /* gravnorm holds a field (a gravity pass ran or it was uploaded) */
     bool gravnorm_known = false;
 
+    sph_treebuild tb; /* GPU octree build */
+
     /* octree handoff */
     float4* snap = nullptr;
     hipEvent_t snap_done = nullptr;
@@ -181,6 +211,14 @@ int launch_tree_convert(sph_ctx* ctx);
 int launch_gravity_up(sph_ctx* ctx);
 /* makes ctx->stream wait for a gravity down pass still running on gstream */
 int join_gravity(sph_ctx* ctx);
+/* stable LSD radix sort of (key, index) pairs by the low `passes` key bytes */
+int launch_sort_pairs(sph_ctx* ctx, uint2* a, uint2* b, int n, int passes, uint2** out);
+/* device tree buffers for a tree of the given size: checks, grows, and picks
+ * the upload slot; tree_commit makes that slot current and converts it */
+int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int n_levels, int32_t node_count,
+                 int* slot);
+int tree_commit(sph_ctx* ctx, int slot, int64_t n_ints, const int32_t* level_counts, int n_levels,
+                int32_t node_count);
 /* sorted slots of the active particles into slot_list / slot_count (README rule) */
 int launch_active_list(sph_ctx* ctx);
 /* bootstrap: the as-written P = 2 down pass (leaves force-accept), used by

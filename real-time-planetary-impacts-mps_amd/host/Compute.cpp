@@ -45,7 +45,18 @@ Compute::Compute(const sph_params& params, const DataStruct& data, const std::ve
         return;
     }
     /* Compute.cpp:82-83: first tree built synchronously from the ICs */
-    buildAndUpload(data.positions.data());
+    if (treeMode_ == SPH_TREE_GPU)
+        buildOnDevice();
+    else
+        buildAndUpload(data.positions.data());
+}
+
+void Compute::buildOnDevice() {
+    const auto t0 = std::chrono::steady_clock::now();
+    check(sph_build_tree(ctx), "sph_build_tree");
+    lastBuildMs_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    haveTree_ = true;
+    treesSwapped_++;
 }
 
 Compute::~Compute() {
@@ -118,6 +129,11 @@ void Compute::updateOctreeData() {
 void Compute::updateOctreeBuffer() {
     if (!ctx || status_ != 0) return;
     if (treeMode_ == SPH_TREE_FIXED) {
+        return;
+    }
+    if (treeMode_ == SPH_TREE_GPU) {
+        /* rebuilt on the GPU from the current positions before the step */
+        buildOnDevice();
         return;
     }
     if (treeMode_ == SPH_TREE_EVERY) {

@@ -75,6 +75,7 @@ public:
 private:
     void check(int rc, const char* what);
     void buildAndUpload(const float* pos4);
+    void buildOnDevice(); /* sph_build_tree (SPH_TREE_GPU) */
     void updateOctreeData(); /* body of the octree thread (Compute.cpp:510-516) */
     void ensureStaging(int64_t nInts, int64_t nNodes);
 

@@ -16,7 +16,7 @@ extern "C" {
 int sph_headless_create(const sph_params* prm, const sph_impact_config* cfg, int eos_res, int tree_mode,
                         int tree_threads, sph_headless** out) {
     if (!prm || !cfg || !out || eos_res < 2 || cfg->n_total < 2) return SPH_E_ARG;
-    if (tree_mode < SPH_TREE_ASYNC || tree_mode > SPH_TREE_FIXED) return SPH_E_ARG;
+    if (tree_mode < SPH_TREE_ASYNC || tree_mode > SPH_TREE_GPU) return SPH_E_ARG;
     *out = nullptr;
     sph_headless* w = new sph_headless();
     w->h = new Headless(*prm, *cfg, eos_res, tree_mode, tree_threads);

@@ -448,6 +448,67 @@ def test_drift_fix_bounds_the_log_rate_per_own_step(sph, orc, tables256):
     assert worst[0] > 0.25
 
 
+def _morton_keys(pos):
+    """Independent numpy restatement of the GPU tree's keys: the bounding cube
+    quantised to 1024 cells per axis in FP64, bits interleaved x, y, z."""
+    p = pos[:, :3].astype(np.float32)
+    lo, hi = p.min(axis=0), p.max(axis=0)
+    ext = np.float32(max(hi - lo))
+    ext = ext if ext > 0 else np.float32(1.0)
+    q = np.floor((p.astype(np.float64) - lo.astype(np.float64)) * (1024.0 / float(ext))).astype(np.int64)
+    q = np.clip(q, 0, 1023)
+    key = np.zeros(len(p), np.int64)
+    for b in range(10):
+        for a in range(3):
+            key |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    return key
+
+
+@pytest.mark.parametrize("n,leaf,depth", [(1, 4, 7), (4, 4, 7), (5, 4, 7), (3000, 4, 7), (20000, 4, 10), (500, 1, 3)])
+def test_morton_octree_structure(sph, orc, n, leaf, depth):
+    """The GPU tree's restatement: every particle once, in key order; leaves
+    over max_leaf only at max_depth; a node's particles share its key prefix
+    and a child sits in the slot of its octant; pre-order numbering and level
+    lists as octree.cpp lays them out."""
+    rng = np.random.default_rng(n + depth)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.normal(size=(n, 3)) * np.array([3.0, 1.0, 0.5])
+    if n >= 500:
+        pos[: n // 10, :3] = pos[0, :3]      # a clump of coincident particles
+    tree = orc.build_octree_morton(pos, leaf, depth)
+    oc = tree["octree"]
+    key = _morton_keys(pos)
+    seen = np.zeros(n, np.int64)
+    order, ids, lvl_of = [], [], {}
+
+    def rec(tp, level, prefix):
+        order.append(tp)
+        ids.append(int(oc[tp + 1]))
+        lvl_of[tp] = level
+        shift = 30 - 3 * level
+        if oc[tp] == 0:
+            assert level < depth
+            for c in range(8):
+                ch = int(oc[tp + 2 + c])
+                if ch >= 0:
+                    rec(ch, level + 1, (prefix << 3) | c)
+        else:
+            ps = oc[tp + 2:tp + 2 + oc[tp]]
+            assert oc[tp] <= leaf or level == depth
+            assert np.all((key[ps] >> shift) == prefix)
+            assert np.all(np.diff(key[ps]) >= 0)
+            seen[ps] += 1
+    rec(0, 0, 0)
+    assert np.all(seen == 1)
+    assert ids == list(range(len(ids))) and tree["node_count"] == len(ids)
+    counts = tree["level_counts"]
+    assert counts[0] == 1 and counts.sum() == len(order)
+    off = np.r_[0, np.cumsum(counts)]
+    for L in range(len(counts)):
+        want = [tp for tp in order if lvl_of[tp] == L]
+        assert list(tree["level_nodes"][off[L]:off[L + 1]]) == want
+
+
 def test_octree_rejects_non_finite_positions(sph, orc):
     pos = np.random.default_rng(3).normal(size=(100, 4)).astype(np.float32)
     pos[17, 1] = np.nan
