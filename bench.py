@@ -50,7 +50,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=None, help="particles (default 1e6; 4e6 with --learned-precision)")
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--tree", choices=["async", "every", "fixed"], default="async")
+    ap.add_argument("--tree", choices=["async", "every", "fixed", "gpu"], default="async",
+                    help="octree: CPU thread (async, as the reference), CPU before every step, the first one kept, "
+                         "or built on the GPU before every step")
     ap.add_argument("--tree-threads", type=int, default=8)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--no-timing", action="store_true", help="do not record per-pass HIP events")
@@ -347,7 +349,8 @@ def main():
     prm = apply_overrides(sph.default_params(fixes=args.fixes), args)
     tables = sph.make_eos_tables(prm, 256)
     cfg = sph.impact_config(args.n, args.seed, headon=args.headon)
-    tree_mode = {"async": sph.TREE_ASYNC, "every": sph.TREE_EVERY, "fixed": sph.TREE_FIXED}[args.tree]
+    tree_mode = {"async": sph.TREE_ASYNC, "every": sph.TREE_EVERY, "fixed": sph.TREE_FIXED,
+                 "gpu": sph.TREE_GPU}[args.tree]
     t_setup = time.perf_counter()
     hl = sph.Headless(prm, cfg, eos_res=256, tree_mode=tree_mode, tree_threads=args.tree_threads)
     ctx = hl.ctx
@@ -391,7 +394,8 @@ def main():
                          "(89% target / 11% impactor), plain "
                          "Newton-Raphson smoothing length, FP32, "
                          + ("reference step as written" if args.fixes == 0 else f"corrections 0x{args.fixes:x}")
-                         + f", CPU octree rebuilt on a thread ({args.tree})"),
+                         + (", octree rebuilt on the GPU before every step (Morton keys)" if args.tree == "gpu"
+                            else f", CPU octree rebuilt on a thread ({args.tree})")),
             "fixes": args.fixes,
             "param_overrides": args.param,
             "n_particles": args.n,
@@ -406,7 +410,7 @@ def main():
         },
     }
     if passes:
-        per_pass = {k: v["ms"] / max(v["calls"], 1) for k, v in passes.items()}
+        per_pass = {k: v["ms"] / v["calls"] for k, v in passes.items() if v["calls"]}
         out["pass_ms"] = {k: round(v, 4) for k, v in per_pass.items()}
         out["pass_ms_note"] = ("gravity_down runs on its own stream, concurrently with density; its events span "
                                "that overlap, so the passes do not add up to ms_per_step")

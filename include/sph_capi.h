@@ -270,7 +270,7 @@ int sph_upload(sph_ctx* ctx, int field, const void* src);
 /* ------------------------------------------------------------------ timing */
 enum sph_pass {
     SPH_PASS_SORT = 0, SPH_PASS_GRAVITY_UP, SPH_PASS_GRAVITY_DOWN, SPH_PASS_DENSITY,
-    SPH_PASS_ACCELERATION, SPH_PASS_STEP, SPH_PASS_COUNT
+    SPH_PASS_ACCELERATION, SPH_PASS_STEP, SPH_PASS_TREE_BUILD, SPH_PASS_COUNT
 };
 /* Enables HIP-event timing of every pass (0 = off, the default). */
 int sph_set_timing(sph_ctx* ctx, int enabled);

@@ -44,7 +44,7 @@ FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 _F4 = {"pos", "vel", "acc"}
 _I32 = {"mat", "next_active", "global_time", "dt", "nr_iters"}
 _U32 = {"cell_array", "cell_start", "cell_end"}
-PASSES = ["sort", "gravity_up", "gravity_down", "density", "acceleration", "step"]
+PASSES = ["sort", "gravity_up", "gravity_down", "density", "acceleration", "step", "tree_build"]
 
 
 class SphParams(ctypes.Structure):

@@ -229,29 +229,32 @@ int launch_fill_cells(sph_ctx* ctx) {
     return SPH_OK;
 }
 
-/* LSD radix sort of (key, index) pairs by the low `passes` bytes of the key,
- * stable (ties keep their order), ping-ponging between a and b.  Returns the
+/* LSD radix sort of (key, value) pairs by the low `passes` bytes of the key,
+ * stable (ties keep their order), ping-ponging between a and b, with the
+ * caller's tile table (256 x tile_cap) and digit totals (256).  Returns the
  * buffer that holds the result through `out`. */
-int launch_sort_pairs(sph_ctx* ctx, uint2* a, uint2* b, int n, int passes, uint2** out) {
+int launch_sort_pairs(sph_ctx* ctx, uint2* a, uint2* b, int n, int passes, uint32_t* tile_hist, int tile_cap,
+                      uint32_t* digit_total, uint2** out) {
     const int ntiles = (n + kTile - 1) / kTile;
-    if (ntiles > ctx->ntiles) return ctx_fail(ctx, SPH_E_STATE, "sort tile table too small");
+    if (ntiles > tile_cap) return ctx_fail(ctx, SPH_E_STATE, "sort tile table too small");
     hipStream_t s = ctx->stream;
     uint2* bufs[2] = {a, b};
     int cur = 0;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        k_hist<<<ntiles, kThreads, 0, s>>>(bufs[cur], n, shift, ctx->tile_hist, ntiles);
+        k_hist<<<ntiles, kThreads, 0, s>>>(bufs[cur], n, shift, tile_hist, ntiles);
         SPH_HIP_TRY(hipGetLastError());
-        k_scan_rows<<<256, kThreads, 0, s>>>(ctx->tile_hist, ntiles, ctx->digit_total);
+        k_scan_rows<<<256, kThreads, 0, s>>>(tile_hist, ntiles, digit_total);
         SPH_HIP_TRY(hipGetLastError());
-        k_scatter<<<ntiles, kThreads, 0, s>>>(bufs[cur], bufs[cur ^ 1], n, shift, ctx->tile_hist, ntiles,
-                                              ctx->digit_total);
+        k_scatter<<<ntiles, kThreads, 0, s>>>(bufs[cur], bufs[cur ^ 1], n, shift, tile_hist, ntiles, digit_total);
         SPH_HIP_TRY(hipGetLastError());
         cur ^= 1;
     }
     *out = bufs[cur];
     return SPH_OK;
 }
+
+int sort_tile_size() { return kTile; }
 
 int launch_sort(sph_ctx* ctx) {
     const int n = ctx->n;

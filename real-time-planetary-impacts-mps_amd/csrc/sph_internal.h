@@ -69,14 +69,15 @@ struct sph_treebuild {
     float* bbox_part = nullptr;           /* 6 per workgroup */
     float* box = nullptr;                 /* lo xyz, hi xyz */
     double* scale = nullptr;              /* 1024 / cube side */
-    int2* nodes = nullptr;                /* [start, end) of every node, level L at L * n */
-    int32_t *cnt = nullptr, *scn = nullptr; /* children per node, their exclusive scan */
-    int32_t *lmin = nullptr, *lleaf = nullptr, *lcnt = nullptr; /* per sorted position */
-    int32_t *psize = nullptr, *pcount = nullptr, *pbase = nullptr, *pnbase = nullptr;
+    int4* chain = nullptr;                /* per sorted position: top level, leaf level, leaf size */
+    int32_t *pbase = nullptr, *pnbase = nullptr; /* per position: flat-array offset, node id */
     int32_t* bsum = nullptr;              /* scan block sums */
     int32_t* scalars = nullptr;           /* flag, totals, scan length, level sizes */
-    int bbox_blocks = 0, bsum_cap = 0, n_cap = 0;
-    int64_t node_cap = 0;
+    uint2* lv[2] = {nullptr, nullptr};    /* (level, offset) per node, sorted by level */
+    uint32_t* tile_hist = nullptr;        /* radix sort tile table, 256 x tile_cap */
+    uint32_t* digit_total = nullptr;      /* 256 */
+    int bbox_blocks = 0, bsum_cap = 0, n_cap = 0, tile_cap = 0;
+    int64_t lv_cap = 0;
 };
 void sph_treebuild_free(sph_treebuild& B);
 
@@ -212,7 +213,9 @@ int launch_gravity_up(sph_ctx* ctx);
 /* makes ctx->stream wait for a gravity down pass still running on gstream */
 int join_gravity(sph_ctx* ctx);
 /* stable LSD radix sort of (key, index) pairs by the low `passes` key bytes */
-int launch_sort_pairs(sph_ctx* ctx, uint2* a, uint2* b, int n, int passes, uint2** out);
+int launch_sort_pairs(sph_ctx* ctx, uint2* a, uint2* b, int n, int passes, uint32_t* tile_hist, int tile_cap,
+                      uint32_t* digit_total, uint2** out);
+int sort_tile_size(); /* pairs per tile of the radix sort */
 /* device tree buffers for a tree of the given size: checks, grows, and picks
  * the upload slot; tree_commit makes that slot current and converts it */
 int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int n_levels, int32_t node_count,

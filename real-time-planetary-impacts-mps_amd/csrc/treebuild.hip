@@ -8,19 +8,34 @@
  *
  *   keys      the particles' bounding cube is quantised to 1024 cells per
  *             axis (FP64 arithmetic), key = x | y << 1 | z << 2 interleaved
- *             (30 bits), pairs (key, index) sorted by the LSD radix sort
- *   levels    a node at level L holds the particles sharing the key's top
- *             3 L bits; it is a leaf when it holds at most max_leaf particles
- *             or L == max_depth; level L + 1 is emitted from level L with one
- *             thread per node (binary searches for the octant boundaries) and
- *             an exclusive scan of the child counts, so every level lists its
- *             nodes in key order
+ *             (30 bits = 10 octal digits, digit t selects the octant at level
+ *             t), pairs (key, particle) sorted by the LSD radix sort
+ *   nodes     a node at level L holds the particles whose keys share the
+ *             first L digits; it is a leaf when it holds at most max_leaf
+ *             particles or L == max_depth
+ *   chains    the nodes that start at one sorted position i are a chain of
+ *             first children, from level Lmin(i) down to a leaf, and each is
+ *             found from neighbouring keys alone (common(a, b) = the number
+ *             of leading digits a and b share):
+ *               d      = common(key[i-1], key[i]): i starts a segment at
+ *                        every level above d; the level-d segment holding
+ *                        i - 1 and i is the parent of the top node
+ *               exists iff i == 0, or d < max_depth and the parent holds
+ *                        more than max_leaf particles (some window
+ *                        [j, j + max_leaf] around i lies inside it)
+ *               Lmin   = 0 for i == 0, else d + 1
+ *               Lleaf  = min(max(Lmin, D + 1), max_depth) with
+ *                        D = common(key[i], key[i + max_leaf])
+ *             so one thread per position produces its chain, and the whole
+ *             tree takes a fixed number of launches whatever its depth
  *   layout    in pre-order a node precedes another iff it starts at an
- *             earlier sorted position, or at the same one and a lower level
- *             (the nodes starting at one position are a chain of first
- *             children ending in a leaf).  Scans over the positions of the
- *             chains' sizes (10 per branch, 2 + n per leaf) and node counts
- *             give every node's offset and id.
+ *             earlier sorted position, or at the same one and a lower level:
+ *             exclusive scans over the positions of the chains' sizes (10 per
+ *             branch, 2 + n per leaf) and node counts give every node's
+ *             offset and id.  A chain's first children are its own next
+ *             nodes; every other child links itself into its parent (one
+ *             binary search for the parent's start).  The level lists are the
+ *             (level, offset) pairs in pre-order, stably sorted by level.
  */
 #include <algorithm>
 #include <climits>
@@ -31,9 +46,10 @@ namespace {
 
 constexpr int kMaxDepth = 10; /* 30-bit keys */
 /* device scalars: [0] non-finite flag, [1] flat-array ints, [2] nodes,
- * [kLevelCounts + L] nodes in level L */
+ * [3] positions (scan length), [kLevelCounts + L] nodes in level L */
 constexpr int kLevelCounts = 4;
 constexpr int kScalars = kLevelCounts + kMaxDepth + 2;
+constexpr int kGridCap = 2048; /* grid-stride launches: at most 2048 workgroups */
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     v &= 0x3FFu;
@@ -42,6 +58,20 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     v = (v | (v << 4)) & 0x030C30C3u;
     v = (v | (v << 2)) & 0x09249249u;
     return v;
+}
+
+/* leading octal digits two 30-bit keys share (10 if equal) */
+__device__ __forceinline__ int common_digits(uint32_t a, uint32_t b) {
+    const uint32_t x = a ^ b;
+    return x == 0u ? kMaxDepth : (__clz(x) - 2) / 3;
+}
+
+/* octant of a key at level L + 1 (digit L + 1) */
+__device__ __forceinline__ int digit_below(uint32_t key, int L) { return (int)((key >> (30 - 3 * (L + 1))) & 7u); }
+
+__global__ void k_tb_init(int32_t* scalars, int n) {
+    const int t = threadIdx.x;
+    if (t < kScalars) scalars[t] = t == 3 ? n : 0;
 }
 
 /* Per-workgroup bounds of the positions; any non-finite coordinate sets
@@ -115,139 +145,132 @@ __global__ __launch_bounds__(256) void k_tb_keys(const float4* __restrict__ pos,
     pairs[i] = make_uint2(spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2), (uint32_t)i);
 }
 
-__global__ void k_tb_init(int2* nodes, int n, int32_t* scalars) {
-    nodes[0] = make_int2(0, n);
-    scalars[0] = 0;
-    scalars[kLevelCounts] = 1;
-}
-
-/* First index in [lo, hi) whose octant at `shift` is >= k (the octants are
- * non-decreasing there: the keys share every higher bit). */
-__device__ __forceinline__ int octant_bound(const uint2* __restrict__ sorted, int lo, int hi, int shift, uint32_t k) {
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (((sorted[mid].x >> shift) & 7u) < k) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-/* Number of children of each node of level `level` (0 for a leaf); the
- * level's size is read from the device. */
-__global__ __launch_bounds__(256) void k_tb_split(const int2* __restrict__ nodes, const int32_t* __restrict__ count,
-                                                  int level, int max_leaf, int max_depth,
-                                                  const uint2* __restrict__ sorted, int32_t* __restrict__ cnt) {
-    const int c = *count;
-    const int shift = 30 - 3 * (level + 1);
-    for (int j = blockIdx.x * 256 + threadIdx.x; j < c; j += gridDim.x * 256) {
-        const int2 nd = nodes[j];
-        if (nd.y - nd.x <= max_leaf || level >= max_depth) {
-            cnt[j] = 0;
+/* The chain of nodes starting at each sorted position: (Lmin, Lleaf, leaf
+ * size, 0), or Lmin = -1 when no node starts there; its flat-array ints and
+ * node count (scanned in place into offsets and ids afterwards), and the
+ * nodes per level. */
+__global__ __launch_bounds__(256) void k_tb_chain(const uint2* __restrict__ sorted, int n, int max_leaf, int max_depth,
+                                                  int4* __restrict__ chain, int32_t* __restrict__ psize,
+                                                  int32_t* __restrict__ pcount, int32_t* __restrict__ level_counts) {
+    __shared__ int hist[kMaxDepth + 1];
+    if (threadIdx.x <= kMaxDepth) hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const uint32_t key = sorted[i].x;
+        int lmin = 0;
+        bool has = true;
+        if (i > 0) {
+            const int d = common_digits(sorted[i - 1].x, key);
+            has = false;
+            if (d < max_depth) {
+                /* the level-d segment holding i holds more than max_leaf
+                 * particles iff some window of max_leaf + 1 around i lies in it */
+                for (int j = max(0, i - max_leaf); j <= i && j + max_leaf < n; ++j)
+                    if (common_digits(sorted[j].x, sorted[j + max_leaf].x) >= d) {
+                        has = true;
+                        break;
+                    }
+            }
+            lmin = d + 1;
+        }
+        if (!has) {
+            chain[i] = make_int4(-1, -1, 0, 0);
+            psize[i] = 0;
+            pcount[i] = 0;
             continue;
         }
-        int b = nd.x, k_c = 0;
-        for (uint32_t k = 1; k <= 8; ++k) {
-            const int e = k < 8 ? octant_bound(sorted, b, nd.y, shift, k) : nd.y;
-            k_c += e > b ? 1 : 0;
-            b = e;
+        const int D = i + max_leaf < n ? common_digits(key, sorted[i + max_leaf].x) : -1;
+        const int lleaf = min(max(lmin, D + 1), max_depth);
+        int cnt;
+        if (D < lleaf) {
+            /* at most max_leaf particles share the leaf's prefix */
+            cnt = 1;
+            while (cnt < max_leaf && i + cnt < n && common_digits(key, sorted[i + cnt].x) >= lleaf) ++cnt;
+        } else {
+            /* a depth-limited leaf: its end by binary search */
+            int lo = i + max_leaf + 1, hi = n;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (common_digits(key, sorted[mid].x) >= lleaf) lo = mid + 1;
+                else hi = mid;
+            }
+            cnt = lo - i;
         }
-        cnt[j] = k_c;
+        chain[i] = make_int4(lmin, lleaf, cnt, 0);
+        psize[i] = 10 * (lleaf - lmin) + 2 + cnt;
+        pcount[i] = lleaf - lmin + 1;
+        for (int L = lmin; L <= lleaf; ++L) atomicAdd(&hist[L], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x <= kMaxDepth && hist[threadIdx.x]) atomicAdd(&level_counts[threadIdx.x], hist[threadIdx.x]);
+}
+
+/* The records of every chain, and its (level, offset) pairs at the nodes'
+ * ids.  A branch gets its first child (the chain's next node) and -1 in the
+ * other slots; k_tb_link fills those.  A record that would not fit in n_ints
+ * is skipped (the tree check of the next gravity pass then reports the
+ * tree). */
+__global__ __launch_bounds__(256) void k_tb_fill(const uint2* __restrict__ sorted, int n,
+                                                 const int4* __restrict__ chain, const int32_t* __restrict__ pbase,
+                                                 const int32_t* __restrict__ pnbase, int32_t* __restrict__ octree,
+                                                 int64_t n_ints, uint2* __restrict__ lv, int32_t node_count) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const int4 c = chain[i];
+        if (c.x < 0) continue;
+        const uint32_t key = sorted[i].x;
+        const int base = pbase[i], nbase = pnbase[i];
+        for (int L = c.x; L <= c.y; ++L) {
+            const int off = base + 10 * (L - c.x), id = nbase + (L - c.x);
+            const bool leaf = L == c.y;
+            if (off < 0 || off + (int64_t)(leaf ? 2 + c.z : 10) > n_ints || id >= node_count) break;
+            lv[id] = make_uint2((uint32_t)L, (uint32_t)off);
+            octree[off + 1] = id;
+            if (leaf) {
+                octree[off] = c.z;
+                for (int q = 0; q < c.z; ++q) octree[off + 2 + q] = (int32_t)sorted[i + q].y;
+            } else {
+                octree[off] = 0;
+                const int first = digit_below(key, L);
+                for (int k = 0; k < 8; ++k) octree[off + 2 + k] = k == first ? off + 10 : -1;
+            }
+        }
     }
 }
 
-/* The children of each node, in octant order, at the scanned offsets. */
-__global__ __launch_bounds__(256) void k_tb_emit(const int2* __restrict__ nodes, const int32_t* __restrict__ cnt,
-                                                 const int32_t* __restrict__ scn, const int32_t* __restrict__ count,
-                                                 int level, const uint2* __restrict__ sorted,
-                                                 int2* __restrict__ next) {
-    const int c = *count;
-    const int shift = 30 - 3 * (level + 1);
-    for (int j = blockIdx.x * 256 + threadIdx.x; j < c; j += gridDim.x * 256) {
-        if (cnt[j] == 0) continue;
-        const int2 nd = nodes[j];
-        int o = scn[j], b = nd.x;
-        for (uint32_t k = 1; k <= 8; ++k) {
-            const int e = k < 8 ? octant_bound(sorted, b, nd.y, shift, k) : nd.y;
-            if (e > b) next[o++] = make_int2(b, e);
-            b = e;
+/* Every chain but the root's links its top node into its parent: the node
+ * at level Lmin - 1 that starts where that level's segment holding i
+ * starts. */
+__global__ __launch_bounds__(256) void k_tb_link(const uint2* __restrict__ sorted, int n,
+                                                 const int4* __restrict__ chain, const int32_t* __restrict__ pbase,
+                                                 int32_t* __restrict__ octree, int64_t n_ints) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const int lmin = chain[i].x;
+        if (i == 0 || lmin < 1) continue;
+        const uint32_t key = sorted[i].x;
+        const int d = lmin - 1;
+        int lo = 0, hi = i; /* first position sharing d digits with i */
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (common_digits(sorted[mid].x, key) >= d) hi = mid;
+            else lo = mid + 1;
         }
+        const int4 pc = chain[lo];
+        if (pc.x < 0 || pc.x > d || pc.y <= d) continue;
+        const int64_t poff = pbase[lo] + 10 * (d - pc.x);
+        if (poff < 0 || poff + 10 > n_ints) continue;
+        octree[poff + 2 + digit_below(key, d)] = pbase[i];
     }
 }
 
-__global__ __launch_bounds__(256) void k_tb_fill_i32(int32_t* p, int n, int v) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = v;
-}
-
-/* Per sorted position: the top level of the chain of nodes starting there,
- * and the leaf that ends the chain. */
-__global__ __launch_bounds__(256) void k_tb_mark(const int2* __restrict__ nodes, const int32_t* __restrict__ cnt,
-                                                 const int32_t* __restrict__ count, int level,
-                                                 int32_t* __restrict__ lmin, int32_t* __restrict__ lleaf,
-                                                 int32_t* __restrict__ lcnt) {
-    const int c = *count;
-    for (int j = blockIdx.x * 256 + threadIdx.x; j < c; j += gridDim.x * 256) {
-        const int2 nd = nodes[j];
-        atomicMin(&lmin[nd.x], level);
-        if (cnt[j] == 0) {
-            lleaf[nd.x] = level;
-            lcnt[nd.x] = nd.y - nd.x;
-        }
-    }
-}
-
-/* Flat-array ints and node count of the chain starting at each position. */
-__global__ __launch_bounds__(256) void k_tb_chain(int n, const int32_t* __restrict__ lmin,
-                                                  const int32_t* __restrict__ lleaf, const int32_t* __restrict__ lcnt,
-                                                  int32_t* __restrict__ psize, int32_t* __restrict__ pcount) {
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
-        const int lm = lmin[p];
-        if (lm == INT_MAX) {
-            psize[p] = 0;
-            pcount[p] = 0;
-            continue;
-        }
-        const int d = lleaf[p] - lm;
-        psize[p] = 10 * d + 2 + lcnt[p];
-        pcount[p] = d + 1;
-    }
-}
-
-/* The flat records of one level, and its list of node offsets.  A record
- * that would not fit in n_ints is skipped (the tree check of the next
- * gravity pass then reports the tree). */
-__global__ __launch_bounds__(256) void k_tb_fill(const int2* __restrict__ nodes, const int32_t* __restrict__ cnt,
-                                                 int count, int level, const int32_t* __restrict__ lmin,
-                                                 const int32_t* __restrict__ pbase, const int32_t* __restrict__ pnbase,
-                                                 const uint2* __restrict__ sorted, int32_t* __restrict__ octree,
-                                                 int64_t n_ints, int32_t* __restrict__ level_nodes) {
-    const int shift = 30 - 3 * (level + 1);
-    for (int j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) {
-        const int2 nd = nodes[j];
-        const int d = level - lmin[nd.x];
-        const int off = pbase[nd.x] + 10 * d;
-        level_nodes[j] = off;
-        const int64_t size = cnt[j] == 0 ? 2 + (nd.y - nd.x) : 10;
-        if (off < 0 || off + size > n_ints) continue;
-        octree[off + 1] = pnbase[nd.x] + d;
-        if (cnt[j] == 0) {
-            octree[off] = nd.y - nd.x;
-            for (int q = nd.x; q < nd.y; ++q) octree[off + 2 + (q - nd.x)] = (int32_t)sorted[q].y;
-            continue;
-        }
-        octree[off] = 0;
-        int b = nd.x;
-        for (uint32_t k = 1; k <= 8; ++k) {
-            const int e = k < 8 ? octant_bound(sorted, b, nd.y, shift, k) : nd.y;
-            octree[off + 2 + (int)(k - 1)] = e > b ? pbase[b] + 10 * (level + 1 - lmin[b]) : -1;
-            b = e;
-        }
-    }
+__global__ __launch_bounds__(256) void k_tb_levels(const uint2* __restrict__ lv, int count,
+                                                   int32_t* __restrict__ level_nodes) {
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) level_nodes[j] = (int32_t)lv[j].y;
 }
 
 /* Exclusive scan of int32 in three launches: 1024-element blocks (4 per
  * thread), a scan of the block sums by one workgroup in chunks, and the
- * block offsets added back.  The element count is read from the device;
- * the grid covers the most it can be.  total[0] = the sum. */
+ * block offsets added back.  In place is allowed.  The element count is read
+ * from the device; the grid covers the most it can be.  total[0] = the sum. */
 constexpr int kScanBlock = 1024;
 
 __device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int& sum) {
@@ -266,8 +289,8 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int& sum) {
     return incl - v;
 }
 
-__global__ __launch_bounds__(256) void k_scan_local(const int32_t* __restrict__ in, int32_t* __restrict__ out,
-                                                    const int32_t* __restrict__ count, int32_t* __restrict__ bsum) {
+__global__ __launch_bounds__(256) void k_scan_local(const int32_t* in, int32_t* out, const int32_t* __restrict__ count,
+                                                    int32_t* __restrict__ bsum) {
     __shared__ int sh[256];
     const int n = *count;
     if ((int)blockIdx.x * kScanBlock >= n) return;
@@ -312,33 +335,26 @@ __global__ __launch_bounds__(256) void k_scan_add(int32_t* __restrict__ out, con
         if (base + k < n) out[base + k] += add;
 }
 
-}  // namespace
-
-namespace {
-
-constexpr int kGridCap = 2048; /* grid-stride launches: at most 2048 workgroups */
-
 int grid_for(int64_t items) { return (int)std::max<int64_t>(1, std::min<int64_t>((items + 255) / 256, kGridCap)); }
 
-/* exclusive scan of in[0, *count) into out; the grid covers max_count */
-int exclusive_scan(sph_ctx* ctx, const int32_t* in, int32_t* out, const int32_t* count, int max_count,
-                   int32_t* total) {
+/* in-place exclusive scan of a[0, *count); the grid covers max_count */
+int exclusive_scan(sph_ctx* ctx, int32_t* a, const int32_t* count, int max_count, int32_t* total) {
     sph_treebuild& B = ctx->tb;
-    const int nb = (max_count + kScanBlock - 1) / kScanBlock;
+    const int nb = std::max((max_count + kScanBlock - 1) / kScanBlock, 1);
     if (nb > B.bsum_cap) return ctx_fail(ctx, SPH_E_STATE, "tree build scan buffer too small");
     hipStream_t s = ctx->stream;
-    k_scan_local<<<std::max(nb, 1), 256, 0, s>>>(in, out, count, B.bsum);
+    k_scan_local<<<nb, 256, 0, s>>>(a, a, count, B.bsum);
     k_scan_sums<<<1, 256, 0, s>>>(B.bsum, count, total);
-    k_scan_add<<<std::max(nb, 1), 256, 0, s>>>(out, count, B.bsum);
+    k_scan_add<<<nb, 256, 0, s>>>(a, count, B.bsum);
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }
 
+/* buffers sized by the particle count */
 int ensure_buffers(sph_ctx* ctx) {
     sph_treebuild& B = ctx->tb;
     const int n = ctx->n;
-    const int64_t node_cap = (int64_t)(kMaxDepth + 1) * n;
-    if (B.n_cap >= n && B.node_cap >= node_cap) return SPH_OK;
+    if (B.n_cap >= n) return SPH_OK;
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     sph_treebuild_free(B);
     B.bbox_blocks = std::min((n + 255) / 256, 1024);
@@ -347,16 +363,37 @@ int ensure_buffers(sph_ctx* ctx) {
     SPH_HIP_TRY(dalloc(&B.bbox_part, (size_t)B.bbox_blocks * 6));
     SPH_HIP_TRY(dalloc(&B.box, 8));
     SPH_HIP_TRY(dalloc(&B.scale, 1));
-    SPH_HIP_TRY(dalloc(&B.nodes, (size_t)node_cap));
-    SPH_HIP_TRY(dalloc(&B.cnt, (size_t)node_cap));
-    SPH_HIP_TRY(dalloc(&B.scn, (size_t)node_cap));
-    int32_t** pn[] = {&B.lmin, &B.lleaf, &B.lcnt, &B.psize, &B.pcount, &B.pbase, &B.pnbase};
-    for (auto p : pn) SPH_HIP_TRY(dalloc(p, (size_t)n));
+    SPH_HIP_TRY(dalloc(&B.chain, (size_t)n));
+    SPH_HIP_TRY(dalloc(&B.pbase, (size_t)n));
+    SPH_HIP_TRY(dalloc(&B.pnbase, (size_t)n));
     B.bsum_cap = (n + kScanBlock - 1) / kScanBlock;
     SPH_HIP_TRY(dalloc(&B.bsum, (size_t)B.bsum_cap));
     SPH_HIP_TRY(dalloc(&B.scalars, kScalars));
+    SPH_HIP_TRY(dalloc(&B.digit_total, 256));
     B.n_cap = n;
-    B.node_cap = node_cap;
+    return SPH_OK;
+}
+
+/* buffers sized by the node count (grown by 1.2x) */
+int ensure_node_buffers(sph_ctx* ctx, int64_t nodes) {
+    sph_treebuild& B = ctx->tb;
+    const int64_t items = std::max<int64_t>(nodes, ctx->n);
+    const int tiles = (int)((items + sort_tile_size() - 1) / sort_tile_size());
+    if (nodes > B.lv_cap || tiles > B.tile_cap) SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (nodes > B.lv_cap) {
+        const int64_t cap = (int64_t)(nodes * 1.2) + 64;
+        dfree(B.lv[0]);
+        dfree(B.lv[1]);
+        SPH_HIP_TRY(dalloc(&B.lv[0], (size_t)cap));
+        SPH_HIP_TRY(dalloc(&B.lv[1], (size_t)cap));
+        B.lv_cap = cap;
+    }
+    if (tiles > B.tile_cap) {
+        const int cap = (int)(tiles * 1.2) + 4;
+        dfree(B.tile_hist);
+        SPH_HIP_TRY(dalloc(&B.tile_hist, (size_t)256 * cap));
+        B.tile_cap = cap;
+    }
     return SPH_OK;
 }
 
@@ -368,14 +405,19 @@ void sph_treebuild_free(sph_treebuild& B) {
     dfree(B.bbox_part);
     dfree(B.box);
     dfree(B.scale);
-    dfree(B.nodes);
-    dfree(B.cnt);
-    dfree(B.scn);
-    int32_t** pn[] = {&B.lmin, &B.lleaf, &B.lcnt, &B.psize, &B.pcount, &B.pbase, &B.pnbase, &B.bsum, &B.scalars};
-    for (auto p : pn) dfree(*p);
+    dfree(B.chain);
+    dfree(B.pbase);
+    dfree(B.pnbase);
+    dfree(B.bsum);
+    dfree(B.scalars);
+    dfree(B.lv[0]);
+    dfree(B.lv[1]);
+    dfree(B.tile_hist);
+    dfree(B.digit_total);
     B.n_cap = 0;
-    B.node_cap = 0;
     B.bsum_cap = 0;
+    B.tile_cap = 0;
+    B.lv_cap = 0;
 }
 
 extern "C" {
@@ -387,38 +429,26 @@ int sph_build_tree(sph_ctx* ctx) {
         return ctx_fail(ctx, SPH_E_ARG, "the GPU tree needs max_tree_depth 0..10");
     SPH_TRY_RC(join_gravity(ctx));
     SPH_TRY_RC(ensure_buffers(ctx));
+    SPH_TRY_RC(ensure_node_buffers(ctx, ctx->tb.lv_cap)); /* tile table for the Morton sort */
+    SPH_TRY_RC(ctx_begin_pass(ctx, SPH_PASS_TREE_BUILD));
     sph_treebuild& B = ctx->tb;
     const int n = ctx->n;
-    const int64_t stride = n; /* level L's nodes at nodes + L * n (a level holds at most n) */
     hipStream_t s = ctx->stream;
-    int32_t* lvc = B.scalars + kLevelCounts;
+    const int g = grid_for(n);
     /* bounds (and the non-finite check), keys, stable sort by key */
-    k_tb_init<<<1, 1, 0, s>>>(B.nodes, n, B.scalars);
+    k_tb_init<<<1, 64, 0, s>>>(B.scalars, n);
     k_tb_bbox_part<<<B.bbox_blocks, 256, 0, s>>>(ctx->pos, n, B.bbox_part, B.scalars);
     k_tb_bbox_final<<<1, 256, 0, s>>>(B.bbox_part, B.bbox_blocks, B.box, B.scale);
     k_tb_keys<<<(n + 255) / 256, 256, 0, s>>>(ctx->pos, n, B.box, B.scale, B.pairs[0]);
     SPH_HIP_TRY(hipGetLastError());
     uint2* sorted = nullptr;
-    SPH_TRY_RC(launch_sort_pairs(ctx, B.pairs[0], B.pairs[1], n, 4, &sorted));
-    /* levels top down; every level's size stays on the device */
-    const int g = grid_for(n);
-    for (int L = 0; L <= max_depth; ++L) {
-        int2* lv = B.nodes + L * stride;
-        int32_t* cnt = B.cnt + L * stride;
-        k_tb_split<<<g, 256, 0, s>>>(lv, lvc + L, L, max_leaf, max_depth, sorted, cnt);
-        if (L == max_depth) break;
-        SPH_TRY_RC(exclusive_scan(ctx, cnt, B.scn + L * stride, lvc + L, n, lvc + L + 1));
-        k_tb_emit<<<g, 256, 0, s>>>(lv, cnt, B.scn + L * stride, lvc + L, L, sorted, lv + stride);
-    }
-    /* chains of nodes per sorted position, then offsets and ids */
-    k_tb_fill_i32<<<g, 256, 0, s>>>(B.lmin, n, INT_MAX);
-    for (int L = 0; L <= max_depth; ++L)
-        k_tb_mark<<<g, 256, 0, s>>>(B.nodes + L * stride, B.cnt + L * stride, lvc + L, L, B.lmin, B.lleaf, B.lcnt);
-    k_tb_chain<<<g, 256, 0, s>>>(n, B.lmin, B.lleaf, B.lcnt, B.psize, B.pcount);
+    SPH_TRY_RC(launch_sort_pairs(ctx, B.pairs[0], B.pairs[1], n, 4, B.tile_hist, B.tile_cap, B.digit_total, &sorted));
+    /* chains, then offsets and ids */
+    k_tb_chain<<<g, 256, 0, s>>>(sorted, n, max_leaf, max_depth, B.chain, B.pbase, B.pnbase,
+                                 B.scalars + kLevelCounts);
     SPH_HIP_TRY(hipGetLastError());
-    k_tb_fill_i32<<<1, 256, 0, s>>>(B.scalars + 3, 1, n); /* scan length */
-    SPH_TRY_RC(exclusive_scan(ctx, B.psize, B.pbase, B.scalars + 3, n, B.scalars + 1));
-    SPH_TRY_RC(exclusive_scan(ctx, B.pcount, B.pnbase, B.scalars + 3, n, B.scalars + 2));
+    SPH_TRY_RC(exclusive_scan(ctx, B.pbase, B.scalars + 3, n, B.scalars + 1));
+    SPH_TRY_RC(exclusive_scan(ctx, B.pnbase, B.scalars + 3, n, B.scalars + 2));
     /* the one host synchronisation: flag, totals and level sizes */
     int32_t h[kScalars] = {};
     SPH_HIP_TRY(hipMemcpyAsync(h, B.scalars, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -429,17 +459,21 @@ int sph_build_tree(sph_ctx* ctx) {
     const int32_t* counts = h + kLevelCounts;
     const int64_t n_ints = h[1];
     const int32_t node_count = h[2];
+    SPH_TRY_RC(ensure_node_buffers(ctx, node_count));
     int k = 0;
     SPH_TRY_RC(tree_reserve(ctx, n_ints, counts, n_levels, node_count, &k));
     /* buffer k: the compute stream is past its release (recorded on it) */
-    int64_t lo = 0;
-    for (int L = 0; L < n_levels; ++L) {
-        k_tb_fill<<<grid_for(counts[L]), 256, 0, s>>>(B.nodes + L * stride, B.cnt + L * stride, counts[L], L, B.lmin,
-                                                      B.pbase, B.pnbase, sorted, ctx->tree.octree_up[k], n_ints,
-                                                      ctx->tree.level_tp_up[k] + lo);
-        lo += counts[L];
-    }
+    int32_t* octree = ctx->tree.octree_up[k];
+    k_tb_fill<<<g, 256, 0, s>>>(sorted, n, B.chain, B.pbase, B.pnbase, octree, n_ints, B.lv[0], node_count);
+    k_tb_link<<<g, 256, 0, s>>>(sorted, n, B.chain, B.pbase, octree, n_ints);
     SPH_HIP_TRY(hipGetLastError());
+    /* level lists: (level, offset) in pre-order, stably sorted by level */
+    uint2* by_level = nullptr;
+    SPH_TRY_RC(launch_sort_pairs(ctx, B.lv[0], B.lv[1], node_count, 1, B.tile_hist, B.tile_cap, B.digit_total,
+                                 &by_level));
+    k_tb_levels<<<grid_for(node_count), 256, 0, s>>>(by_level, node_count, ctx->tree.level_tp_up[k]);
+    SPH_HIP_TRY(hipGetLastError());
+    SPH_TRY_RC(ctx_end_pass(ctx, SPH_PASS_TREE_BUILD));
     return tree_commit(ctx, k, n_ints, counts, n_levels, node_count);
 }
 
