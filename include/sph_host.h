@@ -82,6 +82,49 @@ int sph_octree_build(const float* pos4, int32_t n, int32_t max_leaf, int32_t max
                      int32_t nthreads, sph_octree* out);
 void sph_octree_free(sph_octree* t);
 
+/* ------------------------------------------------------------ HDF5 files */
+/* DataStruct (hdfHandler.hpp:15-26) as library-owned arrays; pos4 / vel4 hold
+ * 4 floats per particle (w = 0).  Free with sph_particles_free. */
+typedef struct sph_particles {
+    int32_t n;
+    float* pos4;
+    float* vel4;
+    float* rho;
+    float* u;
+    float* m;
+    int32_t* mat;
+    float* p;
+    float* h;
+} sph_particles;
+
+/* readHDFFile (hdfHandler.cpp:12-136): PartType0/{Coordinates, Densities,
+ * InternalEnergies, Masses, MaterialIDs, Pressures, SmoothingLengths,
+ * Velocities} of an HDF5 file, converted as H5Dread converts to
+ * H5T_NATIVE_FLOAT / H5T_NATIVE_INT32.  Reads superblock 0/1 files with
+ * symbol-table groups and contiguous or compact, unfiltered storage (the
+ * layout of the reference's files) without libhdf5.  A missing, malformed or
+ * unsupported file returns SPH_E_ARG and sph_hdf_last_error() says why (the
+ * reference exits). */
+int sph_read_hdf(const char* path, sph_particles* out);
+void sph_particles_free(sph_particles* p);
+/* writeHDFFile (hdfHandler.cpp:139-254): PartType0 with the DataStruct
+ * datasets, Entropies (empty), Potentials (`potentials`, or empty when NULL)
+ * and ParticleIDs 0..n-1. */
+int sph_write_hdf(const char* path, const sph_ic_view* ic, const float* potentials);
+/* Checkpoint and resume (no counterpart: the reference never saves its
+ * state).  The file is a valid initial-conditions file (PartType0 as
+ * sph_write_hdf writes it) plus a Checkpoint group with what a step carries
+ * over: accelerations, du/dt, dh/dt and its log-rate, grad-h terms, sound
+ * speeds, Balsara switches, gravity norms, activity, next active times, the
+ * global time and step.  sph_checkpoint_read restores it into a context of
+ * the same particle count; with a tree rebuilt before every step the run then
+ * continues bit-identically. */
+int sph_checkpoint_write(sph_ctx* ctx, const char* path);
+int sph_checkpoint_read(sph_ctx* ctx, const char* path);
+/* Message of the last failing sph_read_hdf / sph_write_hdf / checkpoint call
+ * on this thread. */
+const char* sph_hdf_last_error(void);
+
 /* -------------------------------------------------------------- headless */
 #define SPH_TREE_ASYNC 0   /* rebuild on a CPU thread, swap when ready (Compute.cpp:456-508) */
 #define SPH_TREE_EVERY 1   /* rebuild from the current positions before every step */
@@ -95,6 +138,11 @@ typedef struct sph_headless sph_headless;
  * and the initial tree. */
 int sph_headless_create(const sph_params* prm, const sph_impact_config* cfg, int eos_res,
                         int tree_mode, int tree_threads, sph_headless** out);
+/* Headless::Headless(filepath) (Headless.cpp:19-23): initial conditions read
+ * from an HDF5 file as sph_read_hdf reads them; Tillotson EOS tables (the
+ * reference's ANEOS files are not available). */
+int sph_headless_create_from_file(const sph_params* prm, const char* path, int eos_res, int tree_mode,
+                                  int tree_threads, sph_headless** out);
 void sph_headless_destroy(sph_headless* h);
 /* Headless::step (Headless.cpp:37-55): updateOctreeBuffer, then sort,
  * gravity, density, acceleration, step, then wait. */

@@ -101,6 +101,13 @@ class SphOctree(ctypes.Structure):
     ]
 
 
+class SphParticles(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32)] + [(k, ctypes.POINTER(ctypes.c_float)) for k in
+                                          ("pos4", "vel4", "rho", "u", "m")] + [
+        ("mat", ctypes.POINTER(ctypes.c_int32)), ("p", ctypes.POINTER(ctypes.c_float)),
+        ("h", ctypes.POINTER(ctypes.c_float))]
+
+
 # C-ABI symbols of include/sph_capi.h and include/sph_host.h
 CAPI_SYMBOLS = [
     "sph_params_default", "sph_create", "sph_destroy", "sph_last_error", "sph_num_particles",
@@ -119,6 +126,8 @@ HOST_SYMBOLS = [
     "sph_impact_config_headon", "sph_make_impact", "sph_octree_build", "sph_octree_free",
     "sph_headless_create", "sph_headless_destroy", "sph_headless_step", "sph_headless_enqueue_step",
     "sph_headless_ctx", "sph_headless_tree_stats", "sph_headless_last_error",
+    "sph_read_hdf", "sph_particles_free", "sph_write_hdf", "sph_checkpoint_write", "sph_checkpoint_read",
+    "sph_hdf_last_error", "sph_headless_create_from_file",
 ]
 
 _hip: Optional[ctypes.CDLL] = None
@@ -224,6 +233,16 @@ def libs():
                                               ctypes.POINTER(ctypes.c_double)]
         o.sph_headless_last_error.argtypes = [ctypes.c_void_p]
         o.sph_headless_last_error.restype = ctypes.c_char_p
+        o.sph_read_hdf.argtypes = [ctypes.c_char_p, ctypes.POINTER(SphParticles)]
+        o.sph_particles_free.argtypes = [ctypes.POINTER(SphParticles)]
+        o.sph_particles_free.restype = None
+        o.sph_write_hdf.argtypes = [ctypes.c_char_p, ctypes.POINTER(SphIcView), ctypes.POINTER(ctypes.c_float)]
+        o.sph_checkpoint_write.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        o.sph_checkpoint_read.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        o.sph_hdf_last_error.argtypes = []
+        o.sph_hdf_last_error.restype = ctypes.c_char_p
+        o.sph_headless_create_from_file.argtypes = [ctypes.POINTER(SphParams), ctypes.c_char_p, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     return _hip, _host
 
 
@@ -282,6 +301,37 @@ def make_impact(params: SphParams, cfg: SphImpactConfig, tables: Dict[int, np.nd
     if rc != SPH_OK:
         raise SphError(f"sph_make_impact failed: {rc}")
     return ic
+
+
+def read_hdf(path: str) -> Dict[str, np.ndarray]:
+    """readHDFFile (hdfHandler.cpp:12-136) through sph_read_hdf: the
+    DataStruct datasets of PartType0, as float32 / int32 arrays (pos and vel
+    with 4 floats per particle)."""
+    _, o = libs()
+    t = SphParticles()
+    if o.sph_read_hdf(os.fsencode(path), ctypes.byref(t)) != SPH_OK:
+        raise SphError(f"sph_read_hdf: {o.sph_hdf_last_error().decode()}")
+    try:
+        n = t.n
+        f = lambda p, shape: np.ctypeslib.as_array(p, shape=shape).copy()
+        return {"pos": f(t.pos4, (n, 4)), "vel": f(t.vel4, (n, 4)), "rho": f(t.rho, (n,)), "u": f(t.u, (n,)),
+                "mass": f(t.m, (n,)), "mat": f(t.mat, (n,)), "pressure": f(t.p, (n,)), "h": f(t.h, (n,))}
+    finally:
+        o.sph_particles_free(ctypes.byref(t))
+
+
+def write_hdf(path: str, ic: Dict[str, np.ndarray], potentials: Optional[np.ndarray] = None):
+    """writeHDFFile (hdfHandler.cpp:139-254) through sph_write_hdf."""
+    _, o = libs()
+    keep = {k: np.ascontiguousarray(ic[k], np.int32 if k == "mat" else np.float32)
+            for k in ("pos", "vel", "rho", "u", "mass", "mat", "pressure", "h")}
+    n = int(keep["mass"].shape[0])
+    view = SphIcView(n, _fp(keep["pos"]), _fp(keep["vel"]), _fp(keep["rho"]), _fp(keep["u"]), _fp(keep["mass"]),
+                     _fp(keep["mat"], ctypes.c_int32), _fp(keep["pressure"]), _fp(keep["h"]))
+    pot = None if potentials is None else np.ascontiguousarray(potentials, np.float32)
+    rc = o.sph_write_hdf(os.fsencode(path), ctypes.byref(view), None if pot is None else _fp(pot))
+    if rc != SPH_OK:
+        raise SphError(f"sph_write_hdf: {o.sph_hdf_last_error().decode()}")
 
 
 def octree_build(pos4: np.ndarray, max_leaf: int = 4, max_depth: int = 7, nthreads: int = 1) -> Dict[str, object]:
@@ -363,6 +413,18 @@ class Context:
                                      _fp(lc, ctypes.c_int32), lc.size, int(tree["node_count"]))
         self._check(rc, "sph_upload_tree")
         self.sync()  # pageable source buffers: wait until the copy has landed
+
+    def checkpoint_write(self, path: str):
+        """sph_checkpoint_write: the context's state as an HDF5 checkpoint."""
+        o = libs()[1]
+        if o.sph_checkpoint_write(self.ptr, os.fsencode(path)) != SPH_OK:
+            raise SphError(f"sph_checkpoint_write: {o.sph_hdf_last_error().decode()}")
+
+    def checkpoint_read(self, path: str):
+        """sph_checkpoint_read: restores a checkpoint into this context."""
+        o = libs()[1]
+        if o.sph_checkpoint_read(self.ptr, os.fsencode(path)) != SPH_OK:
+            raise SphError(f"sph_checkpoint_read: {o.sph_hdf_last_error().decode()}")
 
     def build_tree(self):
         """Octree built on the GPU from the current positions (sph_build_tree)."""
@@ -583,6 +645,25 @@ class Headless:
             raise SphError(f"sph_headless_create failed: {rc}")
         self.ptr = ptr
         self.ctx = Context.from_handle(o.sph_headless_ctx(ptr), params)
+
+    @classmethod
+    def from_file(cls, params: SphParams, path: str, eos_res: int = 256, tree_mode: int = TREE_ASYNC,
+                  tree_threads: int = 8):
+        """Headless::Headless(filepath) (Headless.cpp:19-23): initial
+        conditions from an HDF5 file."""
+        _, o = libs()
+        obj = cls.__new__(cls)
+        obj._o = o
+        obj.params = params
+        ptr = ctypes.c_void_p()
+        rc = o.sph_headless_create_from_file(ctypes.byref(params), os.fsencode(path), eos_res, tree_mode,
+                                             tree_threads, ctypes.byref(ptr))
+        if rc != SPH_OK:
+            obj.ptr = None
+            raise SphError(f"sph_headless_create_from_file failed ({rc}): {o.sph_hdf_last_error().decode()}")
+        obj.ptr = ptr
+        obj.ctx = Context.from_handle(o.sph_headless_ctx(ptr), params)
+        return obj
 
     def step(self):
         rc = self._o.sph_headless_step(self.ptr)

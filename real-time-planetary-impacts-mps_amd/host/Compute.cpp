@@ -214,10 +214,22 @@ Headless::Headless(const sph_params& params, const sph_impact_config& config, in
     compute = new Compute(params, data, tables, treeMode, treeThreads);
 }
 
+Headless::Headless(const sph_params& params, const std::string& path, int eosResolution, int treeMode,
+                   int treeThreads) {
+    DataStruct data;
+    if (!readHDFFile(path, data, error_)) {
+        status_ = SPH_E_ARG;
+        return;
+    }
+    std::vector<EOSTable> tables = makeEOSTables(params, eosResolution);
+    compute = new Compute(params, data, tables, treeMode, treeThreads);
+}
+
 Headless::~Headless() { delete compute; }
 
 void Headless::enqueueStep() {
     /* Headless.cpp:43-51 (drawSnapshot is visualisation and is not run) */
+    if (!compute) return;
     compute->updateOctreeBuffer();
     for (int i = 0; i < STEPS_PER_FRAME; i++) {
         compute->sort();
@@ -230,7 +242,7 @@ void Headless::enqueueStep() {
 
 void Headless::step() {
     enqueueStep();
-    if (compute->ctx && compute->status() == 0) {
+    if (compute && compute->ctx && compute->status() == 0) {
         int rc = sph_sync(compute->ctx);
         if (rc != SPH_OK && status_ == 0) {
             status_ = rc;

@@ -107,6 +107,10 @@ class Headless {
 public:
     Headless(const sph_params& params, const sph_impact_config& config, int eosResolution, int treeMode,
              int treeThreads);
+    /* Headless::Headless(filepath) (Headless.cpp:19-23): initial conditions
+     * from an HDF5 file (readHDFFile); the EOS tables are the Tillotson ones
+     * (the ANEOS files are not available). */
+    Headless(const sph_params& params, const std::string& path, int eosResolution, int treeMode, int treeThreads);
     ~Headless();
     /* updateOctreeBuffer, sort, gravity, density, acceleration, step, wait */
     void step();
@@ -121,6 +125,23 @@ private:
     int status_ = 0;
     std::string error_;
 };
+
+/* readHDFFile (hdfHandler.cpp:12-136): PartType0/{Coordinates, Densities,
+ * InternalEnergies, Masses, MaterialIDs, Pressures, SmoothingLengths,
+ * Velocities}, converted to float / int32 as H5Dread converts them.  Errors
+ * are returned (the reference exits). */
+bool readHDFFile(const std::string& path, DataStruct& out, std::string& err);
+/* writeHDFFile (hdfHandler.cpp:139-254): the same datasets plus Entropies,
+ * Potentials and ParticleIDs = 0..n-1; empty vectors give empty datasets. */
+bool writeHDFFile(const std::string& path, const DataStruct& data, std::string& err);
+/* Checkpoint of a context: writeHDFFile's datasets from the device state plus
+ * a Checkpoint group with the rest of the state a step carries over (no
+ * counterpart: the reference never writes its state). */
+int writeCheckpoint(sph_ctx* ctx, const std::string& path, std::string& err);
+/* Restores a checkpoint into a context of the same particle count. */
+int readCheckpoint(sph_ctx* ctx, const std::string& path, std::string& err);
+/* sets what sph_hdf_last_error() reports on this thread */
+void setHdfError(const std::string& msg);
 
 /* Synthetic inputs of the configs. */
 std::vector<EOSTable> makeEOSTables(const sph_params& params, int resolution);

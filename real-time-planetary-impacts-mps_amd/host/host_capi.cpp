@@ -2,6 +2,7 @@
  * host_capi.cpp -- flat C entry points of the headless driver (sph_host.h)
  * so that Python tests and bench.py can drive Headless through ctypes.
  */
+#include <exception>
 #include <string>
 
 #include "Compute.hpp"
@@ -22,6 +23,29 @@ int sph_headless_create(const sph_params* prm, const sph_impact_config* cfg, int
     w->h = new Headless(*prm, *cfg, eos_res, tree_mode, tree_threads);
     const int st = w->h->status();
     if (st != 0) {
+        delete w->h;
+        delete w;
+        return st;
+    }
+    *out = w;
+    return SPH_OK;
+}
+
+int sph_headless_create_from_file(const sph_params* prm, const char* path, int eos_res, int tree_mode,
+                                  int tree_threads, sph_headless** out) {
+    if (!prm || !path || !out || eos_res < 2) return SPH_E_ARG;
+    if (tree_mode < SPH_TREE_ASYNC || tree_mode > SPH_TREE_GPU) return SPH_E_ARG;
+    *out = nullptr;
+    sph_headless* w = new sph_headless();
+    try {
+        w->h = new Headless(*prm, std::string(path), eos_res, tree_mode, tree_threads);
+    } catch (const std::exception&) {
+        delete w;
+        return SPH_E_CAPACITY;
+    }
+    const int st = w->h->status();
+    if (st != 0) {
+        setHdfError(w->h->error());
         delete w->h;
         delete w;
         return st;
