@@ -33,6 +33,18 @@ void sph_tillotson_iron(sph_tillotson* out);
  * is deterministic and adds no noise. */
 int sph_make_eos_table(const sph_params* prm, const sph_tillotson* t, int res, uint64_t seed, float* pc_out);
 
+/* loadANEOSDataFromFile (ANEOS.cpp:21-175): reads an ANEOS text table (13
+ * header lines, "numRho numT", the densities, the temperatures, then
+ * numRho * numT lines "u P c" in SI with the density index fastest) and
+ * resamples it into `pc_out` (2*res*res floats, the layout of
+ * sph_make_eos_table) on the axes of `prm` (eos_min/max_rho/u).  `fixed` = 0
+ * reproduces the reference's resampler, quirks included (Q13: interpolation
+ * weights inverted; the upper row's search bounded by the lower row's index);
+ * 1 interpolates bilinearly.  SPH_E_ARG with sph_aneos_last_error() for a
+ * missing or malformed file. */
+int sph_load_aneos(const char* path, const sph_params* prm, int res, int fixed, float* pc_out);
+const char* sph_aneos_last_error(void);
+
 /* ------------------------------------------------------ initial conditions */
 typedef struct sph_impact_config {
     int32_t n_total;

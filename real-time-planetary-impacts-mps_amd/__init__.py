@@ -127,7 +127,7 @@ HOST_SYMBOLS = [
     "sph_headless_create", "sph_headless_destroy", "sph_headless_step", "sph_headless_enqueue_step",
     "sph_headless_ctx", "sph_headless_tree_stats", "sph_headless_last_error",
     "sph_read_hdf", "sph_particles_free", "sph_write_hdf", "sph_checkpoint_write", "sph_checkpoint_read",
-    "sph_hdf_last_error", "sph_headless_create_from_file",
+    "sph_hdf_last_error", "sph_headless_create_from_file", "sph_load_aneos", "sph_aneos_last_error",
 ]
 
 _hip: Optional[ctypes.CDLL] = None
@@ -239,6 +239,10 @@ def libs():
         o.sph_write_hdf.argtypes = [ctypes.c_char_p, ctypes.POINTER(SphIcView), ctypes.POINTER(ctypes.c_float)]
         o.sph_checkpoint_write.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         o.sph_checkpoint_read.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        o.sph_load_aneos.argtypes = [ctypes.c_char_p, ctypes.POINTER(SphParams), ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_float)]
+        o.sph_aneos_last_error.argtypes = []
+        o.sph_aneos_last_error.restype = ctypes.c_char_p
         o.sph_hdf_last_error.argtypes = []
         o.sph_hdf_last_error.restype = ctypes.c_char_p
         o.sph_headless_create_from_file.argtypes = [ctypes.POINTER(SphParams), ctypes.c_char_p, ctypes.c_int,
@@ -301,6 +305,16 @@ def make_impact(params: SphParams, cfg: SphImpactConfig, tables: Dict[int, np.nd
     if rc != SPH_OK:
         raise SphError(f"sph_make_impact failed: {rc}")
     return ic
+
+
+def load_aneos(path: str, params: SphParams, res: int, fixed: bool = False) -> np.ndarray:
+    """loadANEOSDataFromFile (ANEOS.cpp:21-175) through sph_load_aneos: the
+    (P, c) table, shape (res, res, 2), row = density sample."""
+    _, o = libs()
+    out = np.zeros((res, res, 2), np.float32)
+    if o.sph_load_aneos(os.fsencode(path), ctypes.byref(params), res, int(fixed), _fp(out)) != SPH_OK:
+        raise SphError(f"sph_load_aneos: {o.sph_aneos_last_error().decode()}")
+    return out
 
 
 def read_hdf(path: str) -> Dict[str, np.ndarray]:

@@ -143,6 +143,12 @@ int readCheckpoint(sph_ctx* ctx, const std::string& path, std::string& err);
 /* sets what sph_hdf_last_error() reports on this thread */
 void setHdfError(const std::string& msg);
 
+/* loadANEOSDataFromFile (ANEOS.cpp:21-175): an ANEOS text table resampled
+ * into `out` (res x res (P, c), code units) on the axes of `prm`; `fixed`
+ * selects the corrected resampler instead of the reference's (Q13). */
+bool loadANEOSDataFromFile(const std::string& path, const sph_params& prm, int resolution, bool fixed,
+                           EOSTable& out, std::string& err);
+
 /* Synthetic inputs of the configs. */
 std::vector<EOSTable> makeEOSTables(const sph_params& params, int resolution);
 DataStruct makeImpact(const sph_params& params, const sph_impact_config& config, const std::vector<EOSTable>& tables);
