@@ -155,6 +155,13 @@ int sph_headless_create(const sph_params* prm, const sph_impact_config* cfg, int
  * reference's ANEOS files are not available). */
 int sph_headless_create_from_file(const sph_params* prm, const char* path, int eos_res, int tree_mode,
                                   int tree_threads, sph_headless** out);
+/* The same with the EOS tables of materials 400 and 402 read from ANEOS
+ * text files (sph_load_aneos at eos_res, `fixed_resampler` as there), as the
+ * reference's constructor reads ANEOS_forsterite_S19.txt and
+ * ANEOS_Fe85Si15_S20.txt. */
+int sph_headless_create_with_aneos(const sph_params* prm, const char* path, const char* aneos_400,
+                                   const char* aneos_402, int eos_res, int fixed_resampler, int tree_mode,
+                                   int tree_threads, sph_headless** out);
 void sph_headless_destroy(sph_headless* h);
 /* Headless::step (Headless.cpp:37-55): updateOctreeBuffer, then sort,
  * gravity, density, acceleration, step, then wait. */

@@ -128,6 +128,7 @@ HOST_SYMBOLS = [
     "sph_headless_ctx", "sph_headless_tree_stats", "sph_headless_last_error",
     "sph_read_hdf", "sph_particles_free", "sph_write_hdf", "sph_checkpoint_write", "sph_checkpoint_read",
     "sph_hdf_last_error", "sph_headless_create_from_file", "sph_load_aneos", "sph_aneos_last_error",
+    "sph_headless_create_with_aneos",
 ]
 
 _hip: Optional[ctypes.CDLL] = None
@@ -239,6 +240,9 @@ def libs():
         o.sph_write_hdf.argtypes = [ctypes.c_char_p, ctypes.POINTER(SphIcView), ctypes.POINTER(ctypes.c_float)]
         o.sph_checkpoint_write.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         o.sph_checkpoint_read.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        o.sph_headless_create_with_aneos.argtypes = [ctypes.POINTER(SphParams), ctypes.c_char_p, ctypes.c_char_p,
+                                                     ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         o.sph_load_aneos.argtypes = [ctypes.c_char_p, ctypes.POINTER(SphParams), ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_float)]
         o.sph_aneos_last_error.argtypes = []
@@ -662,16 +666,22 @@ class Headless:
 
     @classmethod
     def from_file(cls, params: SphParams, path: str, eos_res: int = 256, tree_mode: int = TREE_ASYNC,
-                  tree_threads: int = 8):
+                  tree_threads: int = 8, aneos: Optional[Dict[int, str]] = None, fixed_resampler: bool = False):
         """Headless::Headless(filepath) (Headless.cpp:19-23): initial
-        conditions from an HDF5 file."""
+        conditions from an HDF5 file; EOS tables from ANEOS files
+        ({400: path, 402: path}) or, without them, the Tillotson tables."""
         _, o = libs()
         obj = cls.__new__(cls)
         obj._o = o
         obj.params = params
         ptr = ctypes.c_void_p()
-        rc = o.sph_headless_create_from_file(ctypes.byref(params), os.fsencode(path), eos_res, tree_mode,
-                                             tree_threads, ctypes.byref(ptr))
+        if aneos:
+            rc = o.sph_headless_create_with_aneos(ctypes.byref(params), os.fsencode(path), os.fsencode(aneos[400]),
+                                                  os.fsencode(aneos[402]), eos_res, int(fixed_resampler),
+                                                  tree_mode, tree_threads, ctypes.byref(ptr))
+        else:
+            rc = o.sph_headless_create_from_file(ctypes.byref(params), os.fsencode(path), eos_res, tree_mode,
+                                                 tree_threads, ctypes.byref(ptr))
         if rc != SPH_OK:
             obj.ptr = None
             raise SphError(f"sph_headless_create_from_file failed ({rc}): {o.sph_hdf_last_error().decode()}")

@@ -225,6 +225,25 @@ Headless::Headless(const sph_params& params, const std::string& path, int eosRes
     compute = new Compute(params, data, tables, treeMode, treeThreads);
 }
 
+Headless::Headless(const sph_params& params, const std::string& path, const std::string& aneos400,
+                   const std::string& aneos402, int eosResolution, bool fixedResampler, int treeMode,
+                   int treeThreads) {
+    DataStruct data;
+    if (!readHDFFile(path, data, error_)) {
+        status_ = SPH_E_ARG;
+        return;
+    }
+    std::vector<EOSTable> tables(2);
+    tables[0].materialId = 400;
+    tables[1].materialId = 402;
+    if (!loadANEOSDataFromFile(aneos400, params, eosResolution, fixedResampler, tables[0], error_) ||
+        !loadANEOSDataFromFile(aneos402, params, eosResolution, fixedResampler, tables[1], error_)) {
+        status_ = SPH_E_ARG;
+        return;
+    }
+    compute = new Compute(params, data, tables, treeMode, treeThreads);
+}
+
 Headless::~Headless() { delete compute; }
 
 void Headless::enqueueStep() {

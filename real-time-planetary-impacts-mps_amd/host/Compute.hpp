@@ -111,6 +111,11 @@ public:
      * from an HDF5 file (readHDFFile); the EOS tables are the Tillotson ones
      * (the ANEOS files are not available). */
     Headless(const sph_params& params, const std::string& path, int eosResolution, int treeMode, int treeThreads);
+    /* The same with the two ANEOS tables read from their files (material 400
+     * and 402, loadANEOSDataFromFile at eosResolution), as the reference's
+     * constructor loads them. */
+    Headless(const sph_params& params, const std::string& path, const std::string& aneos400,
+             const std::string& aneos402, int eosResolution, bool fixedResampler, int treeMode, int treeThreads);
     ~Headless();
     /* updateOctreeBuffer, sort, gravity, density, acceleration, step, wait */
     void step();

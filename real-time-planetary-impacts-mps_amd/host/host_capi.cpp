@@ -54,6 +54,31 @@ int sph_headless_create_from_file(const sph_params* prm, const char* path, int e
     return SPH_OK;
 }
 
+int sph_headless_create_with_aneos(const sph_params* prm, const char* path, const char* aneos_400,
+                                   const char* aneos_402, int eos_res, int fixed_resampler, int tree_mode,
+                                   int tree_threads, sph_headless** out) {
+    if (!prm || !path || !aneos_400 || !aneos_402 || !out || eos_res < 2) return SPH_E_ARG;
+    if (tree_mode < SPH_TREE_ASYNC || tree_mode > SPH_TREE_GPU) return SPH_E_ARG;
+    *out = nullptr;
+    sph_headless* w = new sph_headless();
+    try {
+        w->h = new Headless(*prm, std::string(path), std::string(aneos_400), std::string(aneos_402), eos_res,
+                            fixed_resampler != 0, tree_mode, tree_threads);
+    } catch (const std::exception&) {
+        delete w;
+        return SPH_E_CAPACITY;
+    }
+    const int st = w->h->status();
+    if (st != 0) {
+        setHdfError(w->h->error());
+        delete w->h;
+        delete w;
+        return st;
+    }
+    *out = w;
+    return SPH_OK;
+}
+
 void sph_headless_destroy(sph_headless* w) {
     if (!w) return;
     delete w->h;
