@@ -67,6 +67,9 @@
 #define SORTING_BIT_MASK (SORTING_BUCKET_NUMBER - 1)
 #define SORTING_ITTERATIONS (32 / SORTING_MASK_LENGTH)
 
+/* Snapshot raster (Parameters.h:67) */
+#define SNAPSHOT_RESOLUTION 1024
+
 /* Headless timing loop (Parameters.h:69) */
 #define HEADLESS_ITTERATION_RATE_REFRESH 32
 

@@ -210,6 +210,19 @@ int sph_tree_info(sph_ctx* ctx, int64_t* n_ints, int32_t* node_count, int32_t* n
  * concatenated level lists `level_nodes` (node_count ints). */
 int sph_download_tree(sph_ctx* ctx, int32_t* octree, int32_t* level_nodes);
 
+/* ---------------------------------------------------------- snapshots */
+/* drawToSnapshot (render.metal:49-66) as Compute::drawSnapshot enqueues it
+ * (Compute.cpp:396-414): clears a res x res image of int32 material ids and
+ * marks each particle whose (x, y) lies in [250, 250 + 20 R_E)^2 with the
+ * largest material id landing on its pixel (row 0 = largest y).  Enqueued on
+ * the context's stream like a pass.  The reference's res is
+ * SNAPSHOT_RESOLUTION. */
+int sph_draw_snapshot(sph_ctx* ctx, int res);
+/* Copies the last drawn image (res * res int32) to the host; synchronous. */
+int sph_download_snapshot(sph_ctx* ctx, int32_t* out);
+/* res of the last drawn image (0 if none). */
+int sph_snapshot_resolution(const sph_ctx* ctx);
+
 /* ----------------------------------------------------------------- passes */
 int sph_sort(sph_ctx* ctx);          /* Compute::sort               Compute.cpp:265-326 */
 int sph_gravity(sph_ctx* ctx);       /* Compute::gravitationalPass  Compute.cpp:328-377 */

@@ -162,6 +162,16 @@ int sph_headless_create_from_file(const sph_params* prm, const char* path, int e
 int sph_headless_create_with_aneos(const sph_params* prm, const char* path, const char* aneos_400,
                                    const char* aneos_402, int eos_res, int fixed_resampler, int tree_mode,
                                    int tree_threads, sph_headless** out);
+/* Compute::drawSnapshot (Compute.cpp:396-448) in every step, before the
+ * passes, as Headless::step calls it (Headless.cpp:43): with `dir` non-NULL
+ * the positions are rasterised each step (sph_draw_snapshot at `res`) and,
+ * once the simulated time passes the next multiple of `interval` code time
+ * units (the reference: 1000), the image drawn at the previous step is
+ * written to dir/snapshot_<k>.png.  NULL turns snapshots off (the default). */
+int sph_headless_set_snapshots(sph_headless* h, const char* dir, int res, int interval);
+/* The snapshot PNG of a material-id image (Compute.cpp:419-446): 402 grey
+ * (160,160,160), 400 orange (255,100,0), anything else white; 8-bit RGBA. */
+int sph_write_snapshot_png(const int32_t* materials, int res, const char* path);
 void sph_headless_destroy(sph_headless* h);
 /* Headless::step (Headless.cpp:37-55): updateOctreeBuffer, then sort,
  * gravity, density, acceleration, step, then wait. */

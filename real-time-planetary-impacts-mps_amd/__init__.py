@@ -119,7 +119,8 @@ CAPI_SYMBOLS = [
     "sph_mlp_loss_grad", "sph_mlp_fit_norm", "sph_mlp_train", "sph_samples_create", "sph_samples_destroy",
     "sph_samples_count", "sph_samples_read", "sph_samples_append", "sph_samples_clear", "sph_set_sample_dump",
     "sph_set_learned_guess", "sph_features", "sph_conservation", "sph_time",
-    "sph_build_tree", "sph_tree_info", "sph_download_tree",
+    "sph_build_tree", "sph_tree_info", "sph_download_tree", "sph_draw_snapshot", "sph_download_snapshot",
+    "sph_snapshot_resolution",
 ]
 HOST_SYMBOLS = [
     "sph_tillotson_granite", "sph_tillotson_iron", "sph_make_eos_table", "sph_impact_config_default",
@@ -128,7 +129,7 @@ HOST_SYMBOLS = [
     "sph_headless_ctx", "sph_headless_tree_stats", "sph_headless_last_error",
     "sph_read_hdf", "sph_particles_free", "sph_write_hdf", "sph_checkpoint_write", "sph_checkpoint_read",
     "sph_hdf_last_error", "sph_headless_create_from_file", "sph_load_aneos", "sph_aneos_last_error",
-    "sph_headless_create_with_aneos",
+    "sph_headless_create_with_aneos", "sph_headless_set_snapshots", "sph_write_snapshot_png",
 ]
 
 _hip: Optional[ctypes.CDLL] = None
@@ -177,6 +178,9 @@ def libs():
         I32 = ctypes.POINTER(ctypes.c_int32)
         h.sph_tree_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), I32, I32, I32, ctypes.c_int32]
         h.sph_download_tree.argtypes = [ctypes.c_void_p, I32, I32]
+        h.sph_draw_snapshot.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        h.sph_download_snapshot.argtypes = [ctypes.c_void_p, I32]
+        h.sph_snapshot_resolution.argtypes = [ctypes.c_void_p]
         h.sph_snapshot_positions.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         h.sph_host_alloc.argtypes = [ctypes.c_size_t]
         h.sph_host_alloc.restype = ctypes.c_void_p
@@ -243,6 +247,8 @@ def libs():
         o.sph_headless_create_with_aneos.argtypes = [ctypes.POINTER(SphParams), ctypes.c_char_p, ctypes.c_char_p,
                                                      ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                      ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        o.sph_headless_set_snapshots.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        o.sph_write_snapshot_png.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.c_int, ctypes.c_char_p]
         o.sph_load_aneos.argtypes = [ctypes.c_char_p, ctypes.POINTER(SphParams), ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_float)]
         o.sph_aneos_last_error.argtypes = []
@@ -319,6 +325,16 @@ def load_aneos(path: str, params: SphParams, res: int, fixed: bool = False) -> n
     if o.sph_load_aneos(os.fsencode(path), ctypes.byref(params), res, int(fixed), _fp(out)) != SPH_OK:
         raise SphError(f"sph_load_aneos: {o.sph_aneos_last_error().decode()}")
     return out
+
+
+def write_snapshot_png(materials: np.ndarray, path: str):
+    """sph_write_snapshot_png: the reference's snapshot colouring, as PNG."""
+    _, o = libs()
+    m = np.ascontiguousarray(materials, np.int32)
+    if m.ndim != 2 or m.shape[0] != m.shape[1]:
+        raise SphError("materials must be a square image")
+    if o.sph_write_snapshot_png(_fp(m, ctypes.c_int32), m.shape[0], os.fsencode(path)) != SPH_OK:
+        raise SphError(f"sph_write_snapshot_png: {o.sph_hdf_last_error().decode()}")
 
 
 def read_hdf(path: str) -> Dict[str, np.ndarray]:
@@ -443,6 +459,13 @@ class Context:
         o = libs()[1]
         if o.sph_checkpoint_read(self.ptr, os.fsencode(path)) != SPH_OK:
             raise SphError(f"sph_checkpoint_read: {o.sph_hdf_last_error().decode()}")
+
+    def draw_snapshot(self, res: int = 1024) -> np.ndarray:
+        """sph_draw_snapshot + sph_download_snapshot: (res, res) int32 material ids."""
+        self._check(self._h.sph_draw_snapshot(self.ptr, res), "sph_draw_snapshot")
+        img = np.zeros((res, res), np.int32)
+        self._check(self._h.sph_download_snapshot(self.ptr, _fp(img, ctypes.c_int32)), "sph_download_snapshot")
+        return img
 
     def build_tree(self):
         """Octree built on the GPU from the current positions (sph_build_tree)."""
@@ -688,6 +711,12 @@ class Headless:
         obj.ptr = ptr
         obj.ctx = Context.from_handle(o.sph_headless_ctx(ptr), params)
         return obj
+
+    def set_snapshots(self, directory: Optional[str], res: int = 1024, interval: int = 1000):
+        """sph_headless_set_snapshots: PNG snapshots into `directory` (None: off)."""
+        d = None if directory is None else os.fsencode(directory)
+        if self._o.sph_headless_set_snapshots(self.ptr, d, res, interval) != SPH_OK:
+            raise SphError("sph_headless_set_snapshots failed")
 
     def step(self):
         rc = self._o.sph_headless_step(self.ptr)

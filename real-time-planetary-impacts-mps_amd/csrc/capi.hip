@@ -95,6 +95,7 @@ void destroy_ctx(sph_ctx* ctx) {
     dfree(ctx->eos[1]);
     free_tree(ctx->tree);
     sph_treebuild_free(ctx->tb);
+    dfree(ctx->snapshot_img);
     if (ctx->snap_done) (void)hipEventDestroy(ctx->snap_done);
     if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
     for (auto& t : ctx->ev_pending) {

@@ -164,6 +164,11 @@ struct sph_ctx {
 
     sph_treebuild tb; /* GPU octree build */
 
+    /* snapshot raster (render.hip): material id per pixel, res x res */
+    int32_t* snapshot_img = nullptr;
+    int64_t snapshot_cap = 0;
+    int snapshot_res = 0;
+
     /* octree handoff */
     float4* snap = nullptr;
     hipEvent_t snap_done = nullptr;

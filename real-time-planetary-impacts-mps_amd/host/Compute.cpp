@@ -166,6 +166,32 @@ void Compute::updateOctreeBuffer() {
     builder_ = std::thread(&Compute::updateOctreeData, this);
 }
 
+void Compute::setSnapshots(const std::string& dir, int resolution, int interval) {
+    snapshotDir_ = dir;
+    snapshotRes_ = resolution;
+    snapshotInterval_ = interval;
+}
+
+void Compute::drawSnapshot() {
+    if (!ctx || status_ != 0 || snapshotDir_.empty()) return;
+    /* Compute.cpp:416-418: the time of the last completed step */
+    int32_t ticks = 0;
+    check(sph_download(ctx, SPH_F_GLOBAL_TIME, &ticks), "sph_download");
+    const float floatTime = ticks * DT_MIN1;
+    if (floatTime > nextSnapshot_ && sph_snapshot_resolution(ctx) > 0) {
+        /* the image drawn at the previous call (Compute.cpp:419-446) */
+        const int res = sph_snapshot_resolution(ctx);
+        std::vector<int32_t> img((size_t)res * res);
+        check(sph_download_snapshot(ctx, img.data()), "sph_download_snapshot");
+        std::string err;
+        const std::string path =
+            snapshotDir_ + "/snapshot_" + std::to_string(nextSnapshot_ / snapshotInterval_) + ".png";
+        if (status_ == 0 && !writeSnapshotPNG(path, img.data(), res, err)) check(SPH_E_ARG, err.c_str());
+        nextSnapshot_ += snapshotInterval_;
+    }
+    check(sph_draw_snapshot(ctx, snapshotRes_), "sph_draw_snapshot");
+}
+
 void Compute::sort() { if (ctx) check(sph_sort(ctx), "sph_sort"); }
 void Compute::gravitationalPass() { if (ctx) check(sph_gravity(ctx), "sph_gravity"); }
 void Compute::densityPass() { if (ctx) check(sph_density(ctx), "sph_density"); }
@@ -250,6 +276,7 @@ void Headless::enqueueStep() {
     /* Headless.cpp:43-51 (drawSnapshot is visualisation and is not run) */
     if (!compute) return;
     compute->updateOctreeBuffer();
+    compute->drawSnapshot();
     for (int i = 0; i < STEPS_PER_FRAME; i++) {
         compute->sort();
         compute->gravitationalPass();

@@ -21,6 +21,7 @@
 #include <thread>
 #include <vector>
 
+#include "Parameters.h"
 #include "sph_capi.h"
 #include "sph_host.h"
 
@@ -62,6 +63,11 @@ public:
     void densityPass();        /* Compute.cpp:179-208 */
     void accelerationPass();   /* Compute.cpp:210-240 */
     void stepPass();           /* Compute.cpp:242-263 */
+    /* Compute.cpp:396-448: with snapshots on, writes snapshot_<k>.png from the
+     * image drawn at the previous call once the simulated time passes the
+     * next multiple of the interval, then draws the current positions. */
+    void drawSnapshot();
+    void setSnapshots(const std::string& dir, int resolution, int interval);
 
     int nParticles = 0;
     sph_ctx* ctx = nullptr;
@@ -100,6 +106,10 @@ private:
     std::atomic<double> lastBuildMs_{0.0};
     bool haveTree_ = false;
     int64_t treesSwapped_ = 0;
+    std::string snapshotDir_;   /* empty: snapshots off */
+    int snapshotRes_ = SNAPSHOT_RESOLUTION;
+    int snapshotInterval_ = 1000;
+    int nextSnapshot_ = 0;
 };
 
 /* Headless driver (Headless.cpp:19-55). */
@@ -145,6 +155,9 @@ bool writeHDFFile(const std::string& path, const DataStruct& data, std::string& 
 int writeCheckpoint(sph_ctx* ctx, const std::string& path, std::string& err);
 /* Restores a checkpoint into a context of the same particle count. */
 int readCheckpoint(sph_ctx* ctx, const std::string& path, std::string& err);
+/* The snapshot PNG (Compute.cpp:416-446): 402 grey, 400 orange, else white. */
+bool writeSnapshotPNG(const std::string& path, const int32_t* materials, int res, std::string& err);
+bool writePNG(const std::string& path, const uint8_t* rgba, int w, int h, std::string& err);
 /* sets what sph_hdf_last_error() reports on this thread */
 void setHdfError(const std::string& msg);
 

@@ -79,6 +79,13 @@ int sph_headless_create_with_aneos(const sph_params* prm, const char* path, cons
     return SPH_OK;
 }
 
+int sph_headless_set_snapshots(sph_headless* w, const char* dir, int res, int interval) {
+    if (!w || !w->h->compute) return SPH_E_ARG;
+    if (dir && (res < 1 || res > 16384 || interval < 1)) return SPH_E_ARG;
+    w->h->compute->setSnapshots(dir ? std::string(dir) : std::string(), res, interval);
+    return SPH_OK;
+}
+
 void sph_headless_destroy(sph_headless* w) {
     if (!w) return;
     delete w->h;
