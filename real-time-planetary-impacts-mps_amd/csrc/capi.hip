@@ -72,7 +72,7 @@ void destroy_ctx(sph_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
     float4** f4[] = {&ctx->pos,    &ctx->vel,    &ctx->acc,   &ctx->s_posm, &ctx->s_vel,
-                     &ctx->s_posh, &ctx->s_velb, &ctx->s_hyd, &ctx->snap};
+                     &ctx->s_posh, &ctx->s_velb, &ctx->s_hyd, &ctx->snap, &ctx->dv};
     for (auto p : f4) dfree(*p);
     float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c,
                     &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->phi, &ctx->dlnh};
@@ -269,6 +269,7 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->pos, n));
     TRY(dalloc(&ctx->vel, n));
     TRY(dalloc(&ctx->acc, n));
+    TRY(dalloc(&ctx->dv, n));
     TRY(dalloc(&ctx->snap, n));
     TRY(dalloc(&ctx->s_posm, n));
     TRY(dalloc(&ctx->s_vel, n));
@@ -528,6 +529,7 @@ int sph_gravity(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
     if (!ctx->have_tree) return ctx_fail(ctx, SPH_E_NOTREE, "gravity before any tree upload");
     SPH_TRY_RC(join_gravity(ctx));
+    SPH_TRY_RC(fold_dv(ctx));
     /* the near-field form checks the node numbering in the up pass */
     if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ctx->tree_check_pending = true;
     ctx->slot_list_valid = false; /* the up pass sets the activity */
@@ -575,7 +577,8 @@ int sph_density(sph_ctx* ctx) {
 
 int sph_acceleration(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
-    SPH_TRY_RC(join_gravity(ctx));
+    /* no join: the pass writes its pair sums to dv, not to the acc the
+     * gravity down pass may still be writing */
     if (!ctx->have_prev) return ctx_fail(ctx, SPH_E_STATE, "acceleration before any sort");
     int rc = ctx_begin_pass(ctx, SPH_PASS_ACCELERATION);
     if (rc == SPH_OK) rc = launch_acceleration(ctx);
@@ -622,6 +625,7 @@ int sph_download(sph_ctx* ctx, int field, void* dst) {
     const int64_t bytes = field_bytes(ctx, field);
     if (bytes < 0) return ctx_fail(ctx, SPH_E_ARG, "unknown field");
     SPH_TRY_RC(join_gravity(ctx));
+    if (field == SPH_F_ACC) SPH_TRY_RC(fold_dv(ctx));
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (field == SPH_F_CELL_START || field == SPH_F_CELL_END) {
         std::vector<uint2> se((size_t)ctx->ncells);
@@ -694,6 +698,10 @@ int sph_upload(sph_ctx* ctx, int field, const void* src) {
     void* dst = field_ptr(ctx, field);
     if (!dst) return ctx_fail(ctx, SPH_E_ARG, "field not uploadable");
     if (field == SPH_F_GRAVNORM) ctx->gravnorm_known = true;
+    /* acc replaced: the pending pair sums belonged to the old values; the
+     * activity replaced: fold with the flags the pass ran under first */
+    if (field == SPH_F_ACC) ctx->dv_pending = false;
+    if (field == SPH_F_ACTIVE) SPH_TRY_RC(fold_dv(ctx));
     ctx->slot_list_valid = false;
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     SPH_HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));

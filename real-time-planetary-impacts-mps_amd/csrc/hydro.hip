@@ -600,7 +600,7 @@ struct AccelArgs {
     const float4* s_velb;
     const float4* s_hyd;
     const uint8_t* active;
-    float4* acc;
+    float4* dv;       /* the pair sum; the step adds it to the gravity acceleration */
     float* du;
     float* dhdt;
     int32_t* next_active;
@@ -696,11 +696,7 @@ __global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
                     }
                 }
             }
-            float4 ac = a.acc[ind];
-            ac.x += dvx;
-            ac.y += dvy;
-            ac.z += dvz;
-            a.acc[ind] = ac;
+            a.dv[ind] = make_float4(dvx, dvy, dvz, 0.f);
             a.du[ind] = du_dt;
             a.dhdt[ind] = dhdt;
             const float goaldt = step_goal(a, h_i, v_sigi, dhdt);
@@ -802,11 +798,7 @@ __global__ __launch_bounds__(256) void k_acceleration_group(AccelArgs a) {
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1) v_sigi = fmaxf(v_sigi, __shfl_xor(v_sigi, off, 64));
         if (lane == 0) {
-            float4 ac = a.acc[ind];
-            ac.x += dvx;
-            ac.y += dvy;
-            ac.z += dvz;
-            a.acc[ind] = ac;
+            a.dv[ind] = make_float4(dvx, dvy, dvz, 0.f);
             a.du[ind] = du_dt;
             a.dhdt[ind] = dhdt;
             const float goaldt = step_goal(a, h_i, v_sigi, dhdt);
@@ -827,7 +819,8 @@ __global__ __launch_bounds__(256) void k_acceleration_group(AccelArgs a) {
 struct StepArgs {
     int n;
     float4 *pos, *vel;
-    const float4* acc;
+    float4* acc;
+    const float4* dv; /* pair sums of this step's acceleration pass, or null */
     float *rho, *u, *h;
     const float *du, *dhdt;
     const uint8_t* active;
@@ -846,7 +839,15 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     if (index >= a.n) return;
     const float dt = a.dtmin1 * (float)dtInt;
     const float4 velocity = a.vel[index];
-    const float4 acceleration = a.acc[index];
+    float4 acceleration = a.acc[index];
+    if (a.dv && a.active[index]) {
+        /* accelerations.metal:127: acc += dv, for the particles the pass ran on */
+        const float4 d = a.dv[index];
+        acceleration.x += d.x;
+        acceleration.y += d.y;
+        acceleration.z += d.z;
+        a.acc[index] = acceleration;
+    }
     const float h_i = a.h[index];
     const float h1 = 1.f / h_i;
     const float dhdt = a.dhdt[index];
@@ -876,6 +877,19 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 }
 
 __global__ void k_advance_time(int32_t* time) { time[0] += time[1]; }
+
+/* acc += dv for the particles the last acceleration pass ran on */
+__global__ __launch_bounds__(256) void k_fold_dv(int n, const uint8_t* __restrict__ active,
+                                                 const float4* __restrict__ dv, float4* __restrict__ acc) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n || !active[i]) return;
+    float4 ac = acc[i];
+    const float4 d = dv[i];
+    ac.x += d.x;
+    ac.y += d.y;
+    ac.z += d.z;
+    acc[i] = ac;
+}
 
 Cells make_cells(const sph_ctx* ctx) {
     Cells c;
@@ -966,6 +980,8 @@ int launch_density(sph_ctx* ctx) {
 }
 
 int launch_acceleration(sph_ctx* ctx) {
+    /* a second pass before the step: the first one's sums go in first */
+    SPH_TRY_RC(fold_dv(ctx));
     const sph_params& p = ctx->prm;
     AccelArgs a;
     a.n = ctx->n;
@@ -975,7 +991,7 @@ int launch_acceleration(sph_ctx* ctx) {
     a.s_velb = ctx->s_velb;
     a.s_hyd = ctx->s_hyd;
     a.active = ctx->active;
-    a.acc = ctx->acc;
+    a.dv = ctx->dv;
     a.du = ctx->du;
     a.dhdt = ctx->dhdt;
     a.next_active = ctx->next_active;
@@ -1009,6 +1025,16 @@ int launch_acceleration(sph_ctx* ctx) {
         default: k_acceleration<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a); break;
     }
     SPH_HIP_TRY(hipGetLastError());
+    ctx->dv_pending = true;
+    return SPH_OK;
+}
+
+int fold_dv(sph_ctx* ctx) {
+    if (!ctx->dv_pending) return SPH_OK;
+    SPH_TRY_RC(join_gravity(ctx));
+    k_fold_dv<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, ctx->active, ctx->dv, ctx->acc);
+    SPH_HIP_TRY(hipGetLastError());
+    ctx->dv_pending = false;
     return SPH_OK;
 }
 
@@ -1019,6 +1045,7 @@ int launch_step(sph_ctx* ctx) {
     a.pos = ctx->pos;
     a.vel = ctx->vel;
     a.acc = ctx->acc;
+    a.dv = ctx->dv_pending ? ctx->dv : nullptr;
     a.rho = ctx->rho;
     a.u = ctx->u;
     a.h = ctx->h;
@@ -1035,6 +1062,7 @@ int launch_step(sph_ctx* ctx) {
     const int threads = 256;
     k_step<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
     SPH_HIP_TRY(hipGetLastError());
+    ctx->dv_pending = false;
     if (p.max_rungs > 0) {
         k_advance_time<<<1, 1, 0, ctx->stream>>>(ctx->time);
         SPH_HIP_TRY(hipGetLastError());

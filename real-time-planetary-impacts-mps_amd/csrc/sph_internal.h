@@ -126,6 +126,11 @@ struct sph_ctx {
 
     /* particle state, original order */
     float4 *pos = nullptr, *vel = nullptr, *acc = nullptr;
+    /* the acceleration pass's pair sums: the step (or any call that reads acc
+     * first) adds them to acc, so that the pass need not wait for the gravity
+     * down pass, which writes acc on its own stream */
+    float4* dv = nullptr;
+    bool dv_pending = false;
     float *rho = nullptr, *u = nullptr, *m = nullptr, *P = nullptr, *h = nullptr;
     float *f = nullptr, *c = nullptr, *du = nullptr, *B = nullptr, *dhdt = nullptr, *gravnorm = nullptr;
     float* dlnh = nullptr; /* (dh/dt)/h of the last active step (SPH_FIX_INACTIVE_DRIFT) */
@@ -208,6 +213,8 @@ int launch_sort(sph_ctx* ctx);
 /* hydro.hip */
 int launch_density(sph_ctx* ctx);
 int launch_acceleration(sph_ctx* ctx);
+/* acc += dv for the last acceleration pass's particles, if not yet done */
+int fold_dv(sph_ctx* ctx);
 int launch_step(sph_ctx* ctx);
 /* learned.hip */
 int launch_density_prologue(sph_ctx* ctx);
