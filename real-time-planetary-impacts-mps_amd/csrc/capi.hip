@@ -521,6 +521,7 @@ int sph_sort(sph_ctx* ctx) {
     ctx->slot_list_valid = false;
     int rc = ctx_begin_pass(ctx, SPH_PASS_SORT);
     if (rc == SPH_OK) rc = launch_sort(ctx);
+    if (rc == SPH_OK) ctx->morton_fresh = true;
     if (rc == SPH_OK) rc = ctx_end_pass(ctx, SPH_PASS_SORT);
     return rc;
 }
@@ -701,6 +702,7 @@ int sph_upload(sph_ctx* ctx, int field, const void* src) {
     /* acc replaced: the pending pair sums belonged to the old values; the
      * activity replaced: fold with the flags the pass ran under first */
     if (field == SPH_F_ACC) ctx->dv_pending = false;
+    if (field == SPH_F_POS || field == SPH_F_VEL || field == SPH_F_MASS) ctx->morton_fresh = false;
     if (field == SPH_F_ACTIVE) SPH_TRY_RC(fold_dv(ctx));
     ctx->slot_list_valid = false;
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -575,9 +575,15 @@ __global__ void k_zero_i32(int32_t* p) { p[0] = 0; }
  * candidate, packed so that the range test needs one 16-byte load:
  * (x, y, z, h), (v, Balsara factor), (m, f P / rho^2, rho, c).  The product
  * f P / rho^2 is formed exactly as accelerations.metal:45,91 forms it. */
+/* The acceleration pass's Morton-ordered records.  MORTON: position, mass
+ * and velocity come from the sort's Morton-ordered copies (unchanged since
+ * the sort), read in order; only the density pass's outputs are gathered
+ * from original order.  Otherwise every field is gathered. */
+template <bool MORTON>
 __global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __restrict__ sorted,
                                                       const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                      const float* __restrict__ m, const float* __restrict__ h,
+                                                      const float* __restrict__ m, const float4* __restrict__ s_posm,
+                                                      const float4* __restrict__ s_vel, const float* __restrict__ h,
                                                       const float* __restrict__ f, const float* __restrict__ P,
                                                       const float* __restrict__ rho, const float* __restrict__ c,
                                                       const float* __restrict__ B, float4* __restrict__ s_posh,
@@ -586,10 +592,20 @@ __global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __rest
     if (k >= n) return;
     const uint32_t j = sorted[k].y;
     const float r = rho[j];
-    const float4 x = pos[j], v = vel[j];
+    float4 x, v;
+    float mj;
+    if (MORTON) {
+        x = s_posm[k];
+        v = s_vel[k];
+        mj = x.w;
+    } else {
+        x = pos[j];
+        v = vel[j];
+        mj = m[j];
+    }
     s_posh[k] = make_float4(x.x, x.y, x.z, h[j]);
     s_velb[k] = make_float4(v.x, v.y, v.z, B[j]);
-    s_hyd[k] = make_float4(m[j], f[j] * P[j] / (r * r), r, c[j]);
+    s_hyd[k] = make_float4(mj, f[j] * P[j] / (r * r), r, c[j]);
 }
 
 struct AccelArgs {
@@ -1006,9 +1022,14 @@ int launch_acceleration(sph_ctx* ctx) {
     a.max_h_len = p.max_smoothing_length;
     a.r_max = p.r_max;
     const int threads = 256;
-    k_gather_hydro<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(
-        ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m, ctx->h, ctx->f, ctx->P, ctx->rho, ctx->c, ctx->B,
-        ctx->s_posh, ctx->s_velb, ctx->s_hyd);
+    if (ctx->morton_fresh)
+        k_gather_hydro<true><<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(
+            ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m, ctx->s_posm, ctx->s_vel, ctx->h, ctx->f, ctx->P,
+            ctx->rho, ctx->c, ctx->B, ctx->s_posh, ctx->s_velb, ctx->s_hyd);
+    else
+        k_gather_hydro<false><<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(
+            ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m, ctx->s_posm, ctx->s_vel, ctx->h, ctx->f, ctx->P,
+            ctx->rho, ctx->c, ctx->B, ctx->s_posh, ctx->s_velb, ctx->s_hyd);
     SPH_HIP_TRY(hipGetLastError());
     const int asel = p.accel_lanes * 2 + (p.lane_split ? 1 : 0);
     switch (asel) {
@@ -1063,6 +1084,7 @@ int launch_step(sph_ctx* ctx) {
     k_step<<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
     SPH_HIP_TRY(hipGetLastError());
     ctx->dv_pending = false;
+    ctx->morton_fresh = false; /* positions and velocities moved */
     if (p.max_rungs > 0) {
         k_advance_time<<<1, 1, 0, ctx->stream>>>(ctx->time);
         SPH_HIP_TRY(hipGetLastError());

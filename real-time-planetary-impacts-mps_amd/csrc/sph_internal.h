@@ -149,7 +149,10 @@ struct sph_ctx {
     uint32_t* digit_total = nullptr; /* 256: pairs per digit of the current pass */
     int ntiles = 0;
 
-    /* Morton-ordered copies */
+    /* Morton-ordered copies; morton_fresh: s_posm / s_vel still equal the
+     * particles' positions, masses and velocities (set by a sort, cleared by a
+     * step or an upload of those fields) */
+    bool morton_fresh = false;
     float4 *s_posm = nullptr, *s_vel = nullptr;
     float4 *s_posh = nullptr, *s_velb = nullptr, *s_hyd = nullptr;
 
