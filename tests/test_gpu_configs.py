@@ -4,9 +4,9 @@
       ten steps against the oracle;
   the same with up to 8 rungs and the inactive-drift correction: twelve
       steps against the oracle;
-  precision form, 1e6 particles, 200 steps: gravity RMS against an FP64
-      direct sum on 8192 sampled particles (seed 8192), and energy /
-      angular-momentum drift over the run.
+  precision form, 1e6 particles, 200 steps with up to 8 rungs: gravity RMS
+      against an FP64 direct sum on 8192 sampled particles (seed 8192), and
+      energy / angular-momentum drift over the run.
 """
 import numpy as np
 import pytest
@@ -76,22 +76,18 @@ def test_inactive_drift_fix_1e5_eight_rungs_matches_oracle(sph, orc, tables256, 
 
 def test_precision_1e6_rms_and_drift(sph, orc):
     """Precision form at 1e6 particles, 200 steps through the host Headless
-    loop (octree rebuilt before every step), every particle active every step
-    (the reference's rule): energy and angular-momentum drift.  Its gravity
-    uses third-order expansions with the second-order acceptance estimate at
-    GRAVITY_ETA 1e-3.  On the final state: gravity RMS against an FP64 direct
-    sum at 8192 particles (seed 8192), for that form and, for comparison, for
-    the P = 2 expansions at the reference's GRAVITY_ETA.
-
-    Individual timesteps (the README rule) are not used here: the
-    reference's step drifts h and rho of inactive particles with their last
-    dh/dt (step.metal:12-46), and over many steps that pushes h to the
-    MAX_SMOOTHING_LENGTH cap, which makes their next density pass scan
-    thousands of cells."""
+    loop (octree rebuilt before every step), individual timesteps with up to
+    8 rungs (README activity rule, leaf/branch activation syncing, inactive
+    particles drifted with their kept log-rate): energy and angular-momentum
+    drift.  Its gravity uses third-order expansions with the second-order
+    acceptance estimate at GRAVITY_ETA 1e-3.  On the final state: gravity RMS
+    against an FP64 direct sum at 8192 particles (seed 8192), for that form
+    and, for comparison, for the P = 2 expansions at the reference's
+    GRAVITY_ETA."""
     import time
     n = 1_000_000
     form = dict(fixes=sph.SPH_FIX_ALL, precision=1, expansion_order=3, mac_order=2, gravity_eta=1e-3,
-                max_unchecked_pointers=8192)
+                max_unchecked_pointers=8192, activity_rule=sph.SPH_ACTIVE_README, max_rungs=8)
     prm = sph.default_params(**form)
     hl = sph.Headless(prm, sph.impact_config(n, 3), eos_res=256, tree_mode=sph.TREE_EVERY, tree_threads=8)
     ctx = hl.ctx
@@ -102,7 +98,8 @@ def test_precision_1e6_rms_and_drift(sph, orc):
     for s in range(199):
         hl.step()
         if s % 25 == 0 or s < 3:
-            print(f"[precision 1e6] step {s + 2} t {ctx.time():.2f} s wall {time.perf_counter() - t0:.1f} s", flush=True)
+            print(f"[precision 1e6] step {s + 2} t {ctx.time():.2f} s active {ctx.download('active').mean():.3f} "
+                  f"wall {time.perf_counter() - t0:.1f} s", flush=True)
     c1 = ctx.conservation()
     e1 = c1["kinetic"] + c1["internal"] + c1["potential"]
     de = abs(e1 - e0) / abs(e0)
