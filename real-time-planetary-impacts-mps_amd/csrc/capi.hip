@@ -63,6 +63,7 @@ void free_tree(sph_tree_dev& T) {
     dfree(T.mp_oct);
     dfree(T.loc3);
     dfree(T.loc);
+    dfree(T.loc_tot);
     T.octree_cap = T.level_cap = T.node_cap = 0;
 }
 
@@ -442,6 +443,8 @@ int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int 
                 SPH_HIP_TRY(dalloc(&T.loc3, (size_t)cap * 3));
             }
             SPH_HIP_TRY(dalloc(&T.loc, (size_t)cap * 4));
+            dfree(T.loc_tot);
+            if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) SPH_HIP_TRY(dalloc(&T.loc_tot, (size_t)cap * 7));
             T.node_cap = cap;
         }
         if (list_need > ctx->list_cap) {

@@ -46,6 +46,7 @@ struct Tree {
     float4* loc;             /* 4 per node */
     float4* oct;             /* 3 per node (P = 3): (XXX XXY XXZ XYY) (XYZ XZZ YYY YYZ) (YZZ ZZZ power3 -) */
     float4* loc3;            /* 3 per node (P = 3): third-order local terms, same order */
+    float4* loc_tot;         /* 7 per node: a near-field leaf's complete local expansion (loc, then loc3 layout) */
 };
 
 /* ------------------------------------------------------------ conversion */
@@ -931,6 +932,54 @@ __device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>&
     }
 }
 
+/* A near-field leaf's complete local expansion, between its two launches. */
+template <int P>
+__device__ __forceinline__ void store_local_total(const DownArgs& a, int id, const LocalExp& L) {
+    float4* r = a.t.loc_tot + 7 * id;
+    const float* e = L.e;
+    r[0] = make_float4(L.px, L.py, L.pz, e[eM]);
+    r[1] = make_float4(e[eX], e[eY], e[eZ], e[eXX]);
+    r[2] = make_float4(e[eXY], e[eXZ], e[eYY], e[eYZ]);
+    r[3] = make_float4(e[eZZ], 0.f, 0.f, 0.f);
+    if (P == 3) {
+        r[4] = make_float4(e[eXXX], e[eXXY], e[eXXZ], e[eXYY]);
+        r[5] = make_float4(e[eXYZ], e[eXZZ], e[eYYY], e[eYYZ]);
+        r[6] = make_float4(e[eYZZ], e[eZZZ], 0.f, 0.f);
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void load_local_total(const DownArgs& a, int id, LocalExp& L) {
+    const float4* r = a.t.loc_tot + 7 * id;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    L.px = r0.x;
+    L.py = r0.y;
+    L.pz = r0.z;
+    L.e[eM] = r0.w;
+    L.e[eX] = r1.x;
+    L.e[eY] = r1.y;
+    L.e[eZ] = r1.z;
+    L.e[eXX] = r1.w;
+    L.e[eXY] = r2.x;
+    L.e[eXZ] = r2.y;
+    L.e[eYY] = r2.z;
+    L.e[eYZ] = r2.w;
+    L.e[eZZ] = r3.x;
+    if (P == 3) {
+        const float4 r4 = r[4], r5 = r[5], r6 = r[6];
+        L.e[eXXX] = r4.x;
+        L.e[eXXY] = r4.y;
+        L.e[eXXZ] = r4.z;
+        L.e[eXYY] = r4.w;
+        L.e[eXYZ] = r5.x;
+        L.e[eXZZ] = r5.y;
+        L.e[eYYY] = r5.z;
+        L.e[eYYZ] = r5.w;
+        L.e[eYZZ] = r6.x;
+        L.e[eZZZ] = r6.y;
+    }
+}
+
 /* Near-field form at a leaf, with the work balanced over the GS lanes: every
  * round each lane takes one node -- list entries first, then nodes from the
  * group's LDS queue -- and either M2Ls it (MAC accepted), sums it pair by
@@ -942,9 +991,14 @@ __device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>&
  * leaf's particles, NEAR_K at a time in registers; the M2L part runs in the
  * first pass only.  Partial sums are reduced over the group, the incoming
  * local expansion is added, and lane k finishes particle k of the pass. */
-template <int GS, bool PREC, int P, int QCAP>
+template <int GS, bool PREC, int P, int QCAP, int PART>
 __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const NodeA& A, LocalExp& L, float4 l0,
                                  const int32_t* in, int lane, int wl, uint64_t gmask, int32_t* q) {
+    /* PART 1: the walk's M2L part only; the complete local expansion goes to
+     * loc_tot.  PART 2: the walk's pair sums only, with that expansion read
+     * back for L2P.  Both make the same walk (the same acceptance decisions,
+     * queue order and lane assignment), so together they sum what one walk
+     * does, with fewer registers live in each. */
     constexpr int NEAR_K = 4;
     constexpr int NTP = nterms<P>();
     const int np = inf.x;
@@ -954,18 +1008,19 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
 #pragma unroll
     for (int k = 0; k < NTP; ++k) Lc[k] = 0.f;
     for (int k0 = 0; k0 < np; k0 += NEAR_K) {
+        if (PART == 1 && k0 > 0) break;
         const int nk = min(NEAR_K, np - k0);
         NearTargets<NEAR_K> T;
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk) {
-            const float4 x = kk < nk ? a.pos[pl[k0 + kk]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 x = (PART != 1 && kk < nk) ? a.pos[pl[k0 + kk]] : make_float4(0.f, 0.f, 0.f, 0.f);
             T.tx[kk] = x.x;
             T.ty[kk] = x.y;
             T.tz[kk] = x.z;
             T.gx[kk] = T.gy[kk] = T.gz[kk] = 0.f;
             T.cx[kk] = T.cy[kk] = T.cz[kk] = T.gp[kk] = T.cp[kk] = 0.f;
         }
-        const bool do_m2l = k0 == 0;
+        const bool do_m2l = PART != 2 && k0 == 0;
         int lp = 0;            /* next list index (group-uniform, as are the rest) */
         bool list_open = true;
         int qh = 0, qt = 0;    /* queue head and tail (monotone; slot = value % QCAP) */
@@ -997,7 +1052,7 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
                 } else {
                     const int4 ci = a.t.info[item];
                     if (ci.x != 0) {
-                        near_p2p<PREC, NEAR_K>(a, T, nk, ci);
+                        if (PART != 1) near_p2p<PREC, NEAR_K>(a, T, nk, ci);
                     } else {
 #pragma unroll
                         for (int c = 0; c < 8; ++c) nch += a.t.child[8 * item + c] >= 0 ? 1 : 0;
@@ -1042,7 +1097,7 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
                     }
                     const int4 ci = a.t.info[cur];
                     ++cur;
-                    if (ci.x != 0) near_p2p<PREC, NEAR_K>(a, T, nk, ci);
+                    if (ci.x != 0 && PART != 1) near_p2p<PREC, NEAR_K>(a, T, nk, ci);
                 }
             }
             /* the queue writes of this round are read by other lanes next round */
@@ -1069,6 +1124,11 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             }
             add_incoming<P>(a, id, l0, L);
         }
+        if constexpr (PART == 1) {
+            if (lane == 0) store_local_total<P>(a, id, L);
+            return;
+        }
+        if constexpr (PART == 2) load_local_total<P>(a, id, L);
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk)
             if (lane == kk && kk < nk)
@@ -1123,18 +1183,20 @@ __device__ __forceinline__ void down_group_body(const DownArgs& a) {
 #pragma unroll
     for (int k = 0; k < NTP; ++k) L.e[k] = Lc[k] = 0.f;
     const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
-    /* MODE 1: branches only, MODE 2: leaves only (the near-field form runs
-     * the two as separate launches, so each has the registers of its own
-     * path); MODE 0: every node */
+    /* MODE 1: branches only; MODE 2 and 3: leaves only, their M2L part and
+     * then their pair sums (the near-field form runs the three as separate
+     * launches, so each has the registers of its own path); MODE 0: every
+     * node */
     if constexpr (MODE == 1) {
         if (np != 0) return;
     }
-    if constexpr (MODE == 2) {
+    if constexpr (MODE == 2 || MODE == 3) {
         static_assert(NEAR, "leaf launches are for the near-field form");
         constexpr int QCAP = 512;
         __shared__ int32_t near_q[GPB * QCAP];
         if (live && np != 0)
-            down_near_leaf_q<GS, PREC, P, QCAP>(a, id, inf, A, L, l0, in, lane, wl, gmask, near_q + (tid / GS) * QCAP);
+            down_near_leaf_q<GS, PREC, P, QCAP, MODE - 1>(a, id, inf, A, L, l0, in, lane, wl, gmask,
+                                                          near_q + (tid / GS) * QCAP);
         return;
     }
     const uint32_t outbase = (uint32_t)s * (uint32_t)a.maxu;
@@ -1222,12 +1284,18 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
     down_group_body<GS, NEAR, PREC, P, MODE>(a);
 }
 
-/* The near-field leaves of a level.  Two waves per SIMD are asked for: the
- * P = 3 precision walk otherwise needs more registers than that allows and
- * would run one wave per SIMD. */
+/* The near-field leaves of a level, M2L part.  Two waves per SIMD are asked
+ * for: the P = 3 precision expansions otherwise need more registers than
+ * that allows and would run one wave per SIMD. */
 template <int GS, bool PREC, int P>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_down_leaves(DownArgs a) {
     down_group_body<GS, true, PREC, P, 2>(a);
+}
+
+/* The near-field leaves of a level, pair sums and L2P (after k_down_leaves). */
+template <int GS, bool PREC, int P>
+__global__ __launch_bounds__(256) void k_down_leaves_p2p(DownArgs a) {
+    down_group_body<GS, true, PREC, P, 3>(a);
 }
 
 /* Level 0.  zeroFirstBranches (helpers.h:15-32); the root has node id 0.
@@ -1279,6 +1347,7 @@ Tree make_tree(sph_ctx* ctx) {
     t.loc = ctx->tree.loc;
     t.oct = ctx->tree.mp_oct;
     t.loc3 = ctx->tree.loc3;
+    t.loc_tot = ctx->tree.loc_tot;
     return t;
 }
 
@@ -1401,9 +1470,11 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
                 if (a.phi) {
                     k_down_group<GS, true, true, 3, 1><<<g, threads, 0, ctx->stream>>>(a);
                     k_down_leaves<GS, true, 3><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves_p2p<GS, true, 3><<<g, threads, 0, ctx->stream>>>(a);
                 } else if (a.near) {
                     k_down_group<GS, true, false, 3, 1><<<g, threads, 0, ctx->stream>>>(a);
                     k_down_leaves<GS, false, 3><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves_p2p<GS, false, 3><<<g, threads, 0, ctx->stream>>>(a);
                 } else {
                     k_down_group<GS, false, false, 3, 0><<<g, threads, 0, ctx->stream>>>(a);
                 }
@@ -1411,9 +1482,11 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
                 if (a.phi) {
                     k_down_group<GS, true, true, 2, 1><<<g, threads, 0, ctx->stream>>>(a);
                     k_down_leaves<GS, true, 2><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves_p2p<GS, true, 2><<<g, threads, 0, ctx->stream>>>(a);
                 } else if (a.near) {
                     k_down_group<GS, true, false, 2, 1><<<g, threads, 0, ctx->stream>>>(a);
                     k_down_leaves<GS, false, 2><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves_p2p<GS, false, 2><<<g, threads, 0, ctx->stream>>>(a);
                 } else {
                     k_down_group<GS, false, false, 2, 0><<<g, threads, 0, ctx->stream>>>(a);
                 }
