@@ -838,6 +838,19 @@ void down_pass_node(const GravCtx& c, uint32_t index, int treePointer, const int
     const int* tree = c.t->octree;
     const uint32_t MAXU = (uint32_t)c.prm->max_unchecked_pointers;
     if (treePointer == 0) {
+        if (tree[0] != 0) {
+            /* a root that is itself a leaf (n <= MAX_CHILDREN_IN_LEAF):
+             * zeroFirstBranches (helpers.h:15-32) would read its particle
+             * indices as child pointers.  Restated as the GPU path handles it:
+             * the intra-leaf sum with a zero local expansion at the root. */
+            Local root;
+            root.pos = c.mp_at(tree[1]).pos;
+            for (int k = 0; k < NT3; ++k) root.e[k] = 0;
+            std::vector<int> none;
+            local_to_leaf(c, root, 0, tree[0], none);
+            out[0] = -1;
+            return;
+        }
         zero_first_branches(c, out);
         return;
     }

@@ -379,3 +379,30 @@ def test_gpu_runs_are_deterministic(sph, impact, tables256):
         hl.close()
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 17])
+@pytest.mark.parametrize("fixes", [0, 0x7F])
+def test_tiny_particle_counts_match_oracle(sph, orc, tables256, impact, n, fixes):
+    """Whole steps with 1, 2, 5 and 17 particles (the root a leaf up to 4):
+    two steps against the oracle, as written and with every correction."""
+    prm = sph.default_params(fixes=fixes)
+    ic = {k: np.ascontiguousarray(v[:n]) for k, v in impact(2000, 5).items()}
+    ctx = sph.Context(prm, ic, tables256)
+    st = orc.State(prm, ic, tables256)
+    for s in range(2):
+        tree = orc.build_octree(st["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+        ctx.upload_tree(tree)
+        ctx.sort()
+        ctx.gravity()
+        ctx.density()
+        ctx.acceleration()
+        ctx.step()
+        ctx.sync()
+        st.full_step(tree)
+    d = np.abs(ctx.download("pos")[:, :3] - st["pos"][:, :3]).max()
+    dh = np.abs(ctx.download("h") / st["h"] - 1).max()
+    print(f"[tiny n={n} fixes={fixes:#x}] max |dx| {d:.3e} max |dh/h| {dh:.3e}", flush=True)
+    assert int(ctx.download("global_time")[0]) == int(st["global_time"][0])
+    assert np.all(np.isfinite(ctx.download("pos"))) and d < 1e-4 and dh < 1e-4
+    ctx.close()

@@ -999,3 +999,30 @@ def test_p3_fmm_is_more_accurate_than_p2(sph, orc, tables256):
             want = _mp3_record(st["pos"][:, :3], st["mass"])
             assert np.allclose(root[24:34], want[24:34], rtol=2e-3, atol=1e-5 * np.abs(want[24:34]).max())
     assert rms[3] < 0.5 * rms[2], rms
+
+
+@pytest.mark.parametrize("fixes", [0, 0x7F])
+def test_root_leaf_gravity_is_the_pair_sum(sph, orc, tables256, fixes):
+    """A tree whose root is a leaf (n <= MAX_CHILDREN_IN_LEAF): no local
+    expansion, only the intra-leaf sum.  Two particles 10 apart (beyond the
+    softening) attract with G m / r^2; a single particle feels nothing."""
+    prm = sph.default_params(fixes=fixes)
+    ic = sph.make_impact(prm, sph.impact_config(2000, 5), tables256)
+    ic = {k: np.ascontiguousarray(v[:2]) for k, v in ic.items()}
+    ic["pos"][0, :3] = [1.0, 2.0, 3.0]
+    ic["pos"][1, :3] = [11.0, 2.0, 3.0]
+    ic["mass"][:] = [0.5, 0.25]
+    st = orc.State(prm, ic, tables256)
+    tree = orc.build_octree(ic["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+    assert tree["node_count"] == 1 and tree["octree"][0] == 2
+    st.set_tree(tree)
+    st.gravity()
+    acc = st["acc"][:, :3].astype(np.float64)
+    G = prm.gravity_g
+    assert acc[0] == pytest.approx([G * 0.25 / 100.0, 0, 0], rel=1e-5, abs=1e-12)
+    assert acc[1] == pytest.approx([-G * 0.5 / 100.0, 0, 0], rel=1e-5, abs=1e-12)
+    one = {k: np.ascontiguousarray(v[:1]) for k, v in ic.items()}
+    st1 = orc.State(prm, one, tables256)
+    st1.set_tree(orc.build_octree(one["pos"], prm.max_children_in_leaf, prm.max_tree_depth))
+    st1.gravity()
+    assert np.all(st1["acc"][:, :3] == 0)
