@@ -131,3 +131,31 @@ def test_precision_1e6_rms_and_drift(sph, orc):
         g.close()
     assert out["P3 eta 1e-3"] < 1e-3
     assert de < 5e-2 and dl < 5e-3
+
+
+def test_headon_4e6_matches_oracle(sph, orc, tables256, impact):
+    """The largest configuration's size: the 4e6-particle head-on impact,
+    README activity rule, up to 8 rungs, inactive-drift correction.  Two
+    steps against the oracle: sorted pairs bit-exact, positions within 1e-4,
+    integer time exact."""
+    n = 4_000_000
+    prm = sph.default_params(activity_rule=sph.SPH_ACTIVE_README, max_rungs=8, fixes=sph.SPH_FIX_INACTIVE_DRIFT)
+    ic = impact(n, 5, headon=True)
+    ctx = sph.Context(prm, ic, tables256)
+    st = orc.State(prm, ic, tables256)
+    for s in range(2):
+        tree = orc.build_octree(st["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+        ctx.upload_tree(tree)
+        ctx.sort()
+        ctx.gravity()
+        ctx.density()
+        ctx.acceleration()
+        ctx.step()
+        ctx.sync()
+        st.full_step(tree)
+        assert np.array_equal(ctx.download("cell_array"), st["cell_array"].reshape(-1, 2))
+        d = np.abs(ctx.download("pos")[:, :3] - st["pos"][:, :3]).max()
+        print(f"[4e6 head-on] step {s} active {ctx.download('active').mean():.4f} max |dx| {d:.2e}", flush=True)
+        assert int(ctx.download("global_time")[0]) == int(st["global_time"][0])
+    assert d < 1e-4
+    ctx.close()
