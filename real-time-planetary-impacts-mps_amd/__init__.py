@@ -22,6 +22,7 @@ HOST_LIB = os.path.join(LIB_DIR, "libsphhost.so")
 
 # ---------------------------------------------------------------- constants
 SPH_OK = 0
+SPH_E_ARG, SPH_E_STATE, SPH_E_NOTREE, SPH_E_CAPACITY, SPH_E_NODEVICE = -1, -2, -3, -4, -5
 SPH_FIX_M2M_ZZ = 1 << 0
 SPH_FIX_SYNC_RANGE = 1 << 1
 SPH_FIX_SOFTENING = 1 << 2

@@ -165,3 +165,27 @@ def test_octree_with_coincident_particles_stops_at_max_depth(sph, orc):
     b = orc.build_octree(pos, 4, 7)
     assert np.array_equal(a["octree"], b["octree"])
     assert len(a["level_counts"]) == 8
+
+
+def test_create_rejects_empty_and_invalid_inputs_before_touching_a_device(sph):
+    """sph_create checks its arguments before it looks for a GPU: an empty
+    particle set and out-of-range parameters are SPH_E_ARG on any machine."""
+    h, _ = sph.libs()
+    prm = sph.default_params()
+    tabs = sph.make_eos_tables(prm, 8)
+    t400 = np.ascontiguousarray(tabs[400])
+    arr = (sph.SphEosTable * 2)(sph.SphEosTable(400, 8, sph._fp(t400)), sph.SphEosTable(402, 8, sph._fp(t400)))
+    one = np.zeros((1, 4), np.float32)
+    s1 = np.ones(1, np.float32)
+    mat = np.full(1, 400, np.int32)
+    f = sph._fp
+    out = ctypes.c_void_p()
+    for n in (0, -3):
+        view = sph.SphIcView(n, f(one), f(one), f(s1), f(s1), f(s1), f(mat, ctypes.c_int32), f(s1), f(s1))
+        assert h.sph_create(ctypes.byref(prm), ctypes.byref(view), arr, 2, ctypes.byref(out)) == sph.SPH_E_ARG
+        assert not out.value
+    view = sph.SphIcView(1, f(one), f(one), f(s1), f(s1), f(s1), f(mat, ctypes.c_int32), f(s1), f(s1))
+    for field, bad in (("cell_power", 0), ("max_unchecked_pointers", 1), ("expansion_order", 4), ("density_lanes", 3)):
+        p = sph.default_params(**{field: bad})
+        assert h.sph_create(ctypes.byref(p), ctypes.byref(view), arr, 2, ctypes.byref(out)) == sph.SPH_E_ARG, field
+    assert h.sph_create(ctypes.byref(prm), ctypes.byref(view), arr, 0, ctypes.byref(out)) == sph.SPH_E_ARG
