@@ -39,6 +39,7 @@
  */
 #include <algorithm>
 #include <climits>
+#include <cstring>
 
 #include "sph_internal.h"
 
@@ -46,9 +47,11 @@ namespace {
 
 constexpr int kMaxDepth = 10; /* 30-bit keys */
 /* device scalars: [0] non-finite flag, [1] flat-array ints, [2] nodes,
- * [3] positions (scan length), [kLevelCounts + L] nodes in level L */
+ * [3] positions (scan length), [kLevelCounts + L] nodes in level L, then
+ * the flat-array ints as a 64-bit sum (two ints, 8-byte aligned) */
 constexpr int kLevelCounts = 4;
-constexpr int kScalars = kLevelCounts + kMaxDepth + 2;
+constexpr int kTotal64 = kLevelCounts + kMaxDepth + 2;
+constexpr int kScalars = kTotal64 + 2;
 constexpr int kGridCap = 2048; /* grid-stride launches: at most 2048 workgroups */
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -73,6 +76,8 @@ __global__ void k_tb_init(int32_t* scalars, int n) {
     const int t = threadIdx.x;
     if (t < kScalars) scalars[t] = t == 3 ? n : 0;
 }
+
+static_assert(kTotal64 % 2 == 0, "the 64-bit total needs 8-byte alignment");
 
 /* Per-workgroup bounds of the positions; any non-finite coordinate sets
  * flags[0]. */
@@ -151,10 +156,14 @@ __global__ __launch_bounds__(256) void k_tb_keys(const float4* __restrict__ pos,
  * nodes per level. */
 __global__ __launch_bounds__(256) void k_tb_chain(const uint2* __restrict__ sorted, int n, int max_leaf, int max_depth,
                                                   int4* __restrict__ chain, int32_t* __restrict__ psize,
-                                                  int32_t* __restrict__ pcount, int32_t* __restrict__ level_counts) {
+                                                  int32_t* __restrict__ pcount, int32_t* __restrict__ level_counts,
+                                                  unsigned long long* __restrict__ total64) {
     __shared__ int hist[kMaxDepth + 1];
+    __shared__ unsigned long long bsize;
     if (threadIdx.x <= kMaxDepth) hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) bsize = 0ull;
     __syncthreads();
+    unsigned long long mysize = 0ull;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const uint32_t key = sorted[i].x;
         int lmin = 0;
@@ -199,10 +208,13 @@ __global__ __launch_bounds__(256) void k_tb_chain(const uint2* __restrict__ sort
         chain[i] = make_int4(lmin, lleaf, cnt, 0);
         psize[i] = 10 * (lleaf - lmin) + 2 + cnt;
         pcount[i] = lleaf - lmin + 1;
+        mysize += (unsigned long long)psize[i];
         for (int L = lmin; L <= lleaf; ++L) atomicAdd(&hist[L], 1);
     }
+    if (mysize) atomicAdd(&bsize, mysize);
     __syncthreads();
     if (threadIdx.x <= kMaxDepth && hist[threadIdx.x]) atomicAdd(&level_counts[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x == 0 && bsize) atomicAdd(total64, bsize);
 }
 
 /* The records of every chain, and its (level, offset) pairs at the nodes'
@@ -445,7 +457,7 @@ int sph_build_tree(sph_ctx* ctx) {
     SPH_TRY_RC(launch_sort_pairs(ctx, B.pairs[0], B.pairs[1], n, 4, B.tile_hist, B.tile_cap, B.digit_total, &sorted));
     /* chains, then offsets and ids */
     k_tb_chain<<<g, 256, 0, s>>>(sorted, n, max_leaf, max_depth, B.chain, B.pbase, B.pnbase,
-                                 B.scalars + kLevelCounts);
+                                 B.scalars + kLevelCounts, (unsigned long long*)(B.scalars + kTotal64));
     SPH_HIP_TRY(hipGetLastError());
     SPH_TRY_RC(exclusive_scan(ctx, B.pbase, B.scalars + 3, n, B.scalars + 1));
     SPH_TRY_RC(exclusive_scan(ctx, B.pnbase, B.scalars + 3, n, B.scalars + 2));
@@ -454,6 +466,10 @@ int sph_build_tree(sph_ctx* ctx) {
     SPH_HIP_TRY(hipMemcpyAsync(h, B.scalars, sizeof(h), hipMemcpyDeviceToHost, s));
     SPH_HIP_TRY(hipStreamSynchronize(s));
     if (h[0]) return ctx_fail(ctx, SPH_E_ARG, "non-finite particle position");
+    uint64_t total64 = 0;
+    std::memcpy(&total64, h + kTotal64, sizeof(total64));
+    if (total64 > (uint64_t)INT32_MAX)
+        return ctx_fail(ctx, SPH_E_CAPACITY, "octree larger than the int32 flat layout allows");
     int n_levels = 0;
     while (n_levels <= max_depth && h[kLevelCounts + n_levels] > 0) ++n_levels;
     const int32_t* counts = h + kLevelCounts;
