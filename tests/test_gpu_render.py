@@ -20,7 +20,8 @@ def raster(pos, mat, res):
     ux = (pos[:, 0].astype(f) - f(250.0)) * scale
     uy = (pos[:, 1].astype(f) - f(250.0)) * scale
     ok = (ux >= 0) & (uy >= 0) & (ux <= f(res)) & (uy <= f(res))
-    px, py = ux.astype(np.int64), uy.astype(np.int64)
+    # out-of-window values (NaN, -1e9) are masked before the cast
+    px, py = np.where(ok, ux, 0).astype(np.int64), np.where(ok, uy, 0).astype(np.int64)
     ok &= (px < res) & (py < res)
     img = np.zeros(res * res, np.int64)
     np.maximum.at(img, (px + (res - 1 - py) * res)[ok], mat[ok].astype(np.int64))
