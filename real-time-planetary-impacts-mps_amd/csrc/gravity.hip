@@ -1184,18 +1184,18 @@ __device__ __forceinline__ void down_group_body(const DownArgs& a) {
     for (int k = 0; k < NTP; ++k) L.e[k] = Lc[k] = 0.f;
     const int32_t* in = a.in + (live ? (int64_t)a.t.parent_slot[id] * a.maxu : 0);
     /* MODE 1: branches only; MODE 2 and 3: leaves only, their M2L part and
-     * then their pair sums (the near-field form runs the three as separate
-     * launches, so each has the registers of its own path); MODE 0: every
-     * node */
+     * then their pair sums (separate launches, so each has the registers of
+     * its own path); MODE 4: leaves only, M2L and pair sums in one walk;
+     * MODE 0: every node */
     if constexpr (MODE == 1) {
         if (np != 0) return;
     }
-    if constexpr (MODE == 2 || MODE == 3) {
+    if constexpr (MODE == 2 || MODE == 3 || MODE == 4) {
         static_assert(NEAR, "leaf launches are for the near-field form");
         constexpr int QCAP = 512;
         __shared__ int32_t near_q[GPB * QCAP];
         if (live && np != 0)
-            down_near_leaf_q<GS, PREC, P, QCAP, MODE - 1>(a, id, inf, A, L, l0, in, lane, wl, gmask,
+            down_near_leaf_q<GS, PREC, P, QCAP, MODE == 4 ? 0 : MODE - 1>(a, id, inf, A, L, l0, in, lane, wl, gmask,
                                                           near_q + (tid / GS) * QCAP);
         return;
     }
@@ -1296,6 +1296,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 template <int GS, bool PREC, int P>
 __global__ __launch_bounds__(256) void k_down_leaves_p2p(DownArgs a) {
     down_group_body<GS, true, PREC, P, 3>(a);
+}
+
+/* The near-field leaves of a level in one walk (M2L, pair sums, L2P), for
+ * the second-order FP32 form: there the whole walk fits in as few registers
+ * as either half (4 waves per SIMD), so splitting it would only walk each
+ * subtree twice.  The other forms split it (k_down_leaves, then
+ * k_down_leaves_p2p); both give the same results, bit for bit. */
+template <int GS, bool PREC, int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_down_leaves_walk(DownArgs a) {
+    down_group_body<GS, true, PREC, P, 4>(a);
 }
 
 /* Level 0.  zeroFirstBranches (helpers.h:15-32); the root has node id 0.
@@ -1464,8 +1474,8 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
             constexpr int GS = 32;
             const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
             const dim3 g(xcd_grid(blocks));
-            /* near-field form: branches (list processing, L2L) and leaves
-             * (balanced walk, L2P) as two launches over the level */
+            /* near-field form: branches (list processing, L2L), then the
+             * leaves (balanced walk, pair sums, L2P) in one or two launches */
             if (order3) {
                 if (a.phi) {
                     k_down_group<GS, true, true, 3, 1><<<g, threads, 0, ctx->stream>>>(a);
@@ -1485,8 +1495,7 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
                     k_down_leaves_p2p<GS, true, 2><<<g, threads, 0, ctx->stream>>>(a);
                 } else if (a.near) {
                     k_down_group<GS, true, false, 2, 1><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves<GS, false, 2><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves_p2p<GS, false, 2><<<g, threads, 0, ctx->stream>>>(a);
+                    k_down_leaves_walk<GS, false, 2><<<g, threads, 0, ctx->stream>>>(a);
                 } else {
                     k_down_group<GS, false, false, 2, 0><<<g, threads, 0, ctx->stream>>>(a);
                 }
