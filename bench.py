@@ -46,7 +46,7 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=None, help="particles (default 1e6; 4e6 with --learned-precision)")
     ap.add_argument("--seed", type=int, default=3)
