@@ -306,7 +306,9 @@ def test_acceleration_brute_force_and_momentum(small_density, prm):
     np.fill_diagonal(r, np.inf)
 
     def gradw(hh):
-        q = r / hh
+        # q clipped at 4 (the gradient is zero from 2 on) so that the diagonal's
+        # r = inf does not evaluate inf - inf in the unused branch
+        q = np.minimum(r / hh, 4.0)
         dwdq = np.where(q < 1, -3 * q + 2.25 * q * q, np.where(q < 2, -0.75 * (2 - q) ** 2, 0.0))
         return (dwdq / (math.pi * hh ** 4))[..., None] * xij / r[..., None]
 
