@@ -474,8 +474,11 @@ __device__ __forceinline__ void m2l_emit(const LocalExp& L, float4 b0, float4 b1
                                          float eps, Emit emit) {
     const float vx = L.px - b0.x, vy = L.py - b0.y, vz = L.pz - b0.z;
     const float r2 = vx * vx + vy * vy + vz * vz;
-    const float r = sqrtf(r2);
-    const float r_inv = r > 1e-6f ? 1.f / r : 0.f;
+    /* the hardware square root and reciprocal (about 1 ulp) instead of the
+     * correctly rounded library sequences: this is the down pass's
+     * per-entry inner loop, and the expansion's own error is orders larger */
+    const float r = __builtin_amdgcn_sqrtf(r2);
+    const float r_inv = r > 1e-6f ? __builtin_amdgcn_rcpf(r) : 0.f;
     float Dt_1, Dt_2, Dt_3, Dt_4 = 0.f;
     if (r2 < eps * eps) {
         const float eps_inv = 1.f / eps;
@@ -915,18 +918,25 @@ __device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>&
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk) {
             const float dx = T.tx[kk] - xj.x, dy = T.ty[kk] - xj.y, dz = T.tz[kk] - xj.z;
-            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            if (kk >= np || r == 0.f) continue;
-            const float w = mj * dphi_dr_m2l(r, a.eps);
             if constexpr (PREC) {
+                const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+                if (kk >= np || r == 0.f) continue;
+                const float w = mj * dphi_dr_m2l(r, a.eps);
                 kadd(T.gx[kk], T.cx[kk], -(w * (dx / r)));
                 kadd(T.gy[kk], T.cy[kk], -(w * (dy / r)));
                 kadd(T.gz[kk], T.cz[kk], -(w * (dz / r)));
                 kadd(T.gp[kk], T.cp[kk], mj * pot_m2l(r, a.eps));
             } else {
-                T.gx[kk] -= w * (dx / r);
-                T.gy[kk] -= w * (dy / r);
-                T.gz[kk] -= w * (dz / r);
+                /* FP32 form: the hardware square root and reciprocal (about
+                 * 1 ulp), as in M2L; the precision form keeps the correctly
+                 * rounded operations its compensated sums are built on */
+                const float r = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+                if (kk >= np || r == 0.f) continue;
+                const float ri = __builtin_amdgcn_rcpf(r);
+                const float w = mj * (r >= a.eps ? ri * ri : dphi_dr_m2l(r, a.eps));
+                T.gx[kk] -= w * (dx * ri);
+                T.gy[kk] -= w * (dy * ri);
+                T.gz[kk] -= w * (dz * ri);
             }
         }
     }
