@@ -335,7 +335,9 @@ __device__ __forceinline__ int gsum_i(int v) {
  * Newton-Raphson control flow stays uniform over the group.
  * The terms are the reference's; only the order of the additions differs
  * (per-cell partial sums, then a tree), which changes results at the
- * rounding level. */
+ * rounding level, and the distance's square root and the second pass's 1/r
+ * are the hardware's (about 1 ulp, as under the reference's Metal fast-math
+ * defaults) rather than the correctly rounded library sequences. */
 /* Calls body(key) for the cells first, first + STEP, ... of the box
  * [mn, mx] in x -> y -> z order (z fastest), with the Morton key of the
  * periodic cell.  The position in the box advances by carries instead of a
@@ -432,7 +434,7 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
                 const float r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 > hb2) continue;
                 w_in++;
-                const float r = sqrtf(r2);
+                const float r = __builtin_amdgcn_sqrtf(r2);
                 if (use_cache) {
                     if (r > hsum) continue;
                 } else {
@@ -493,13 +495,13 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
                 const float r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 > hb2) continue;
                 f_in++;
-                const float r = sqrtf(r2);
+                const float r = __builtin_amdgcn_sqrtf(r2);
                 if (cached && r > hsum) continue;
                 const float mass = q.w;
                 if (!cached) dpart += kW(r, h1) * mass;
                 density_h += mass * kdWdh(r, h1);
                 if (r > 1e-12f) {
-                    const float r1 = 1.f / r;
+                    const float r1 = __builtin_amdgcn_rcpf(r);
                     const float4 vj = a.s_vel[k];
                     const float vx = vj.x - vi4.x, vy = vj.y - vi4.y, vz = vj.z - vi4.z;
                     const float g = kGradWFactor(r, h1);
