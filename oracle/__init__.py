@@ -90,6 +90,10 @@ def lib() -> ctypes.CDLL:
         for name in ("orc_gravity_up", "orc_gravity_down"):
             getattr(L, name).argtypes = [ctypes.c_void_p, _P(OrcState), _P(OrcTree), _P(OrcGravityBuffers)]
             getattr(L, name).restype = None
+        L.orc_gravity_account.argtypes = [ctypes.c_void_p, _P(OrcState), _P(OrcTree), _P(OrcGravityBuffers),
+                                          ctypes.c_int, _P(ctypes.c_int64), _P(ctypes.c_double),
+                                          _P(ctypes.c_int64), _P(ctypes.c_double)]
+        L.orc_gravity_account.restype = ctypes.c_int
         L.orc_direct_sum_fp64.argtypes = [_P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_int32, _P(ctypes.c_int32),
                                           ctypes.c_int32, ctypes.c_double, ctypes.c_double, _P(ctypes.c_double),
                                           _P(ctypes.c_double)]
@@ -381,6 +385,32 @@ class State:
         ot, gb = self._tree_structs()
         s = self._s()
         lib().orc_gravity_down(_pp(self.params), ctypes.byref(s), ctypes.byref(ot), ctypes.byref(gb))
+
+    def gravity_account(self, plant: int = 0) -> Dict[str, object]:
+        """The down pass with the interaction account (orc_gravity_account),
+        after gravity_up: per particle the particles and FP64 mass it received
+        through M2L, its leaf and its P2P source leaves, and the part counted
+        twice through accepted ancestors.  `plant` 1/2 plants a dropped /
+        doubled list entry (test use)."""
+        ot, gb = self._tree_structs()
+        s = self._s()
+        n = self.n
+        out = {"count": np.zeros(n, np.int64), "mass": np.zeros(n, np.float64),
+               "excess_count": np.zeros(n, np.int64), "excess_mass": np.zeros(n, np.float64)}
+        unexpected = lib().orc_gravity_account(
+            _pp(self.params), ctypes.byref(s), ctypes.byref(ot), ctypes.byref(gb), int(plant),
+            _p(out["count"], ctypes.c_int64), _p(out["mass"], ctypes.c_double),
+            _p(out["excess_count"], ctypes.c_int64), _p(out["excess_mass"], ctypes.c_double))
+        out["unexpected"] = int(unexpected)
+        return out
+
+    def gravity_down_on(self, multipoles24: np.ndarray):
+        """The down pass alone on given multipole records (node_count x 24,
+        the SPH_F_MULTIPOLES layout, order 2), after a gravity_up of the same
+        tree (which supplies the parent indexes and activity flags)."""
+        assert self.params.expansion_order != 3
+        self.grav["multipoles"][:, :24] = multipoles24
+        self.gravity_down()
 
     def gravity(self):
         """gravity.metal upPass + downPass.  The near-field form first

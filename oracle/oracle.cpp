@@ -13,6 +13,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -199,6 +200,15 @@ struct Multipole { /* poles.h:46-55 */
 struct Local { /* poles.h:58-61 */
     f3 pos;
     float e[NT3];
+    /* interaction account (orc_gravity_account only; not in the reference):
+     * particles and mass of every source node M2L'd into this expansion along
+     * the path from the root, and the part of that count that comes from
+     * accepting a node which contains the expansion's own node */
+    int64_t acct_n = 0;
+    double acct_m = 0.0;
+    int64_t acct_excess_n = 0;
+    double acct_excess_m = 0.0;
+    int self_node = -1;
 };
 
 Multipole load_mp(const float* rec, int order = 2) {
@@ -527,6 +537,27 @@ struct GravCtx {
           mac_order(p->mac_order == 2 ? 2 : (p->expansion_order == 3 ? 3 : 2)),
           fix_p3((p->fixes & SPH_FIX_ORDER3) != 0) {}
     Multipole mp_at(int dataPointer) const { return load_mp(g->multipoles + MP_REC * (int64_t)dataPointer, order); }
+    struct Account* acct = nullptr;   /* test-only interaction account, see orc_gravity_account */
+};
+
+/* Test-only interaction account of the down pass.  Per node (by data
+ * pointer): the particles and mass of its subtree (FP64) and its parent; per
+ * node the account carried into its local expansion; per particle the final
+ * account.  `plant` puts a deliberate mistake into the traversal so that a
+ * test can show the account detects it: 1 = skip the last entry of every
+ * non-empty input list, 2 = M2L the first accepted node of every leaf twice. */
+struct Account {
+    std::vector<int64_t> node_n;
+    std::vector<double> node_m;
+    std::vector<int> parent;
+    std::vector<int64_t> local_n, local_excess_n;
+    std::vector<double> local_m, local_excess_m;
+    int64_t* out_n;
+    double* out_m;
+    int64_t* out_excess_n;
+    double* out_excess_m;
+    std::atomic<int64_t> unexpected{0};
+    int plant = 0;
 };
 
 /* P2M.metal:12-107 */
@@ -657,6 +688,30 @@ Multipole M2M(const GravCtx& c, uint32_t index, int treePointer) {
     return mp;
 }
 
+/* Books an accepted source node into the account of `local`.  A source that
+ * contains the expansion's own node (the reference's acceptance test can
+ * accept an ancestor, accept.metal:17-25) counts that node's particles a
+ * second time: that part is kept apart as the excess.  Only the direct
+ * parent can be such a source (a list holds its node itself and nodes
+ * disjoint from it); any other containing source is counted as unexpected. */
+inline void book_accept(const GravCtx& c, Local& local, int src) {
+    Account& a = *c.acct;
+    local.acct_n += a.node_n[src];
+    local.acct_m += a.node_m[src];
+    const int self = local.self_node;
+    if (self < 0) return;
+    for (int q = self; q >= 0; q = a.parent[q]) {
+        if (q != src) continue;
+        if (a.parent[self] == src) {
+            local.acct_excess_n += a.node_n[self];
+            local.acct_excess_m += a.node_m[self];
+        } else {
+            a.unexpected++;
+        }
+        break;
+    }
+}
+
 inline bool check_and_add(const GravCtx& c, const Multipole& mp, Local& local, int toScan, bool forceAccept) {
     /* helpers.h:34-45 */
     int nodeDataPointer = c.t->octree[toScan + 1];
@@ -664,6 +719,7 @@ inline bool check_and_add(const GravCtx& c, const Multipole& mp, Local& local, i
     if (forceAccept || m2l_accept(mp, nodeMp, c.prm->gravity_eta, c.mac_order, c.fix_p3)) {
         Local nl = M2L(local.pos, nodeMp, c.prm->gravity_softening, c.order);
         for (int j = 0; j < NT3; ++j) local.e[j] += nl.e[j];
+        if (c.acct) book_accept(c, local, nodeDataPointer);
         return true;
     }
     return false;
@@ -798,6 +854,21 @@ void local_to_leaf(const GravCtx& c, const Local& local, int treePointer, int nP
         }
         st3(s->acc, pp, c.prm->gravity_g * a);
         s->gravnorm[pp] = len(a);
+        if (c.acct) {
+            Account& ac = *c.acct;
+            int64_t n_acc = local.acct_n + (nParticles - 1);
+            double m_acc = local.acct_m;
+            for (int j = start; j < end; j++)
+                if (j != i) m_acc += (double)s->m[tree[j]];
+            for (int leaf : p2p_leaves) {
+                n_acc += tree[leaf];
+                for (int j = leaf + 2; j < leaf + 2 + tree[leaf]; j++) m_acc += (double)s->m[tree[j]];
+            }
+            ac.out_n[pp] = n_acc;
+            ac.out_m[pp] = m_acc;
+            ac.out_excess_n[pp] = local.acct_excess_n;
+            ac.out_excess_m[pp] = local.acct_excess_m;
+        }
     }
 }
 
@@ -814,6 +885,13 @@ void L2L(const GravCtx& c, const Local& local, int treePointer) {
         if (!nodeMp.active) continue;
         f3 r = nodeMp.pos - local.pos;
         store_lc(c.g->locals + LC_REC * (int64_t)ndp, transform_local(local, r, c.order), c.order);
+        if (c.acct) {
+            Account& ac = *c.acct;
+            ac.local_n[ndp] = local.acct_n;
+            ac.local_m[ndp] = local.acct_m;
+            ac.local_excess_n[ndp] = local.acct_excess_n;
+            ac.local_excess_m[ndp] = local.acct_excess_m;
+        }
     }
 }
 
@@ -845,6 +923,7 @@ void down_pass_node(const GravCtx& c, uint32_t index, int treePointer, const int
              * the intra-leaf sum with a zero local expansion at the root. */
             Local root;
             root.pos = c.mp_at(tree[1]).pos;
+            root.self_node = tree[1];
             for (int k = 0; k < NT3; ++k) root.e[k] = 0;
             std::vector<int> none;
             local_to_leaf(c, root, 0, tree[0], none);
@@ -868,16 +947,34 @@ void down_pass_node(const GravCtx& c, uint32_t index, int treePointer, const int
     uint32_t startScan = parentIndex * MAXU;
     uint32_t endScan = (parentIndex + 1) * MAXU;
     Local local = load_lc(c.g->locals + LC_REC * (int64_t)dataPointer, c.order);
+    local.self_node = dataPointer;
+    uint32_t scanEnd = endScan;
+    int repeatFirst = 0;
+    if (c.acct) {
+        Account& ac = *c.acct;
+        local.acct_n = ac.local_n[dataPointer];
+        local.acct_m = ac.local_m[dataPointer];
+        local.acct_excess_n = ac.local_excess_n[dataPointer];
+        local.acct_excess_m = ac.local_excess_m[dataPointer];
+        if (ac.plant == 1) { /* planted mistake: drop the last list entry */
+            uint32_t e = startScan;
+            while (e < endScan && in[e] != -1) ++e;
+            if (e > startScan) scanEnd = e - 1;
+        }
+        if (ac.plant == 2 && nParticles != 0) repeatFirst = 1; /* planted mistake: a leaf scans its first entry twice */
+    }
     const bool near = c.prm->fixes & SPH_FIX_NEAR_FIELD;
     std::vector<int> p2p_leaves;
-    for (uint32_t i = startScan; i < endScan; i++) {
+    for (uint32_t i = startScan; i < scanEnd; i++) {
         int toScan = in[i];
         if (toScan == -1) break;
-        if (nParticles != 0) {
-            if (near) near_scan(c, mp, local, toScan, treePointer, p2p_leaves);
-            else recursive_scan(c, mp, local, toScan, treePointer);
-        } else {
-            non_recursive_scan(c, out, mp, local, outputPointer, maxOutputPointer, toScan, treePointer);
+        for (int rep = 0; rep <= (i == startScan ? repeatFirst : 0); ++rep) {
+            if (nParticles != 0) {
+                if (near) near_scan(c, mp, local, toScan, treePointer, p2p_leaves);
+                else recursive_scan(c, mp, local, toScan, treePointer);
+            } else {
+                non_recursive_scan(c, out, mp, local, outputPointer, maxOutputPointer, toScan, treePointer);
+            }
         }
     }
     if (outputPointer < maxOutputPointer && nParticles == 0) out[outputPointer] = -1;
@@ -1473,8 +1570,10 @@ void orc_gravity_up(const sph_params* prm, const orc_state* s, const orc_tree* t
 
 /* gravity.metal:64-143 over levels 0..deepest with the list ping-pong of
  * Compute.cpp:353-376 */
-void orc_gravity_down(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g) {
+static void gravity_down(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g,
+                         Account* acct) {
     GravCtx c{prm, s, t, g};
+    c.acct = acct;
     std::vector<int64_t> off(t->n_levels + 1, 0);
     for (int l = 0; l < t->n_levels; ++l) off[l + 1] = off[l] + t->level_counts[l];
     for (int l = 0; l < t->n_levels; ++l) {
@@ -1485,6 +1584,65 @@ void orc_gravity_down(const sph_params* prm, const orc_state* s, const orc_tree*
         for (int index = 0; index < t->level_counts[l]; ++index)
             down_pass_node(c, (uint32_t)index, pointers[index], in, out);
     }
+}
+
+void orc_gravity_down(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g) {
+    gravity_down(prm, s, t, g, nullptr);
+}
+
+/* Subtree particle counts and FP64 masses and the parent of every node, by a
+ * walk of the flat tree (octree.cpp:14-141 layout). */
+static void account_nodes(const orc_state* s, const orc_tree* t, int treePointer, int parent, Account& a) {
+    const int* tree = t->octree;
+    const int dp = tree[treePointer + 1];
+    a.parent[dp] = parent;
+    const int np = tree[treePointer];
+    if (np != 0) {
+        a.node_n[dp] = np;
+        double m = 0.0;
+        for (int j = treePointer + 2; j < treePointer + 2 + np; ++j) m += (double)s->m[tree[j]];
+        a.node_m[dp] = m;
+        return;
+    }
+    int64_t n = 0;
+    double m = 0.0;
+    for (int j = treePointer + 2; j < treePointer + 10; ++j) {
+        const int child = tree[j];
+        if (child == -1) continue;
+        account_nodes(s, t, child, dp, a);
+        n += a.node_n[tree[child + 1]];
+        m += a.node_m[tree[child + 1]];
+    }
+    a.node_n[dp] = n;
+    a.node_m[dp] = m;
+}
+
+int orc_gravity_account(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g,
+                        int plant, int64_t* count_out, double* mass_out, int64_t* excess_count_out,
+                        double* excess_mass_out) {
+    Account a;
+    const size_t K = (size_t)t->node_count;
+    a.node_n.assign(K, 0);
+    a.node_m.assign(K, 0.0);
+    a.parent.assign(K, -1);
+    a.local_n.assign(K, 0);
+    a.local_excess_n.assign(K, 0);
+    a.local_m.assign(K, 0.0);
+    a.local_excess_m.assign(K, 0.0);
+    a.out_n = count_out;
+    a.out_m = mass_out;
+    a.out_excess_n = excess_count_out;
+    a.out_excess_m = excess_mass_out;
+    a.plant = plant;
+    for (int32_t i = 0; i < s->n; ++i) {
+        count_out[i] = -1;
+        mass_out[i] = 0.0;
+        excess_count_out[i] = 0;
+        excess_mass_out[i] = 0.0;
+    }
+    account_nodes(s, t, 0, -1, a);
+    gravity_down(prm, s, t, g, &a);
+    return (int)a.unexpected.load();
 }
 
 void orc_pole_m2l(float* local16, const float* mp24, float eps) {

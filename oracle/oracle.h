@@ -110,6 +110,23 @@ typedef struct orc_gravity_buffers {
 } orc_gravity_buffers;
 void orc_gravity_up(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g);
 void orc_gravity_down(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g);
+/* The down pass (after orc_gravity_up) with an interaction account, for the
+ * pins of the list logic (gravity.metal:64-143, helpers.h:15-147), which has
+ * no closed form of its own: per active particle, the number of particles and
+ * the FP64 mass of every source node M2L'd into the expansions on its path,
+ * plus its leaf's other members and, in the near-field form, the members of
+ * its P2P source leaves.  Each particle should receive every other particle
+ * exactly once: count = n - 1 and mass = M_total - m_p, except for accepts of
+ * a node that contains the receiving node (accept.metal:17-25 can accept an
+ * ancestor), whose double-counted part is returned apart as the excess.
+ * Returns the number of containing accepts that the list structure does not
+ * explain (any source other than the receiving node's parent; 0 expected).
+ * `plant` (test use) 0 = as restated, 1 = drop the last entry of every list,
+ * 2 = a leaf scans its first list entry twice.  Inactive particles get count
+ * -1.  Also writes acc and gravnorm as orc_gravity_down does. */
+int orc_gravity_account(const sph_params* prm, const orc_state* s, const orc_tree* t, orc_gravity_buffers* g,
+                        int plant, int64_t* count_out, double* mass_out, int64_t* excess_count_out,
+                        double* excess_mass_out);
 
 /* Single FMM operators on 24-float multipole / 16-float local records (for
  * unit pins): M2L (M2L.metal:12-108) added into `local`, transformLocal

@@ -826,6 +826,75 @@ def test_as_written_near_field_is_inaccurate(sph, orc, tables256):
     assert math.sqrt(np.mean(rel ** 2)) > 0.05
 
 
+# ------------------------- the down pass's list logic: interaction account
+_ACCOUNT_CASES = [
+    # (n, seed, tree builder, sph_params overrides)
+    (20000, 41, "mean", {}),                                   # as written, MAX_UNCHECKED_POINTERS 1024
+    (100000, 42, "mean", {}),
+    (20000, 43, "mean", {"max_unchecked_pointers": 16}),       # list-full force-accepts
+    (20000, 44, "mean", {"expansion_order": 3}),               # P = 3 as written (order-3 MAC)
+    (20000, 45, "mean", {"expansion_order": 3, "mac_order": 2, "max_unchecked_pointers": 16}),
+    (20000, 46, "mean", {"fixes": 0x15}),                      # near-field form (P2P source leaves)
+    (20000, 47, "mean", {"fixes": 0x15, "max_unchecked_pointers": 16}),
+    (20000, 48, "morton", {}),                                 # the GPU-built tree's shape
+]
+
+
+def _account_state(sph, orc, tables256, n, seed, builder, kw):
+    prm = sph.default_params(**kw)
+    ic = sph.make_impact(prm, sph.impact_config(n, seed), tables256)
+    st = orc.State(prm, ic, tables256)
+    build = orc.build_octree if builder == "mean" else orc.build_octree_morton
+    st.set_tree(build(ic["pos"], prm.max_children_in_leaf, prm.max_tree_depth))
+    return st, ic["mass"].astype(np.float64)
+
+
+@pytest.mark.parametrize("n,seed,builder,kw", _ACCOUNT_CASES)
+def test_down_pass_reaches_every_particle_exactly_once(sph, orc, tables256, n, seed, builder, kw):
+    """Pins the list logic of the down pass (gravity.metal:64-143,
+    helpers.h:15-147: lists seeded with the node itself, MAC-rejected
+    branches pushed, leaves and full lists force-accepted, recursion cut at
+    GRAVITY_MAX_RECURSION), which has no closed form of its own.  Whatever
+    the acceptance decisions, the nodes M2L'd along a particle's path, its
+    leaf's other members and its P2P source leaves must partition all other
+    particles: every particle receives n - 1 particles and M - m_p of mass,
+    apart from ancestor accepts (accept.metal:17-25), which count their own
+    node's particles twice and are booked separately.  Cold (gravNorm = 0,
+    every MAC test rejects, the deepest lists) and warm."""
+    st, m = _account_state(sph, orc, tables256, n, seed, builder, kw)
+    mtot = float(m.sum())
+    for phase in ("cold", "warm"):
+        if phase == "warm":
+            st.gravity()
+        st.gravity_up()
+        a = st.gravity_account()
+        acc_account = st["acc"].copy()
+        st.gravity_down()
+        assert np.array_equal(acc_account, st["acc"]), "the account changed the field"
+        got_n = a["count"] - a["excess_count"]
+        got_m = a["mass"] - a["excess_mass"]
+        n_anc = int((a["excess_count"] > 0).sum())
+        print(f"[account {n} {builder} {kw} {phase}] ancestor accepts on {n_anc} particles, "
+              f"max |mass residual| {np.abs(got_m - (mtot - m)).max() / mtot:.2e}", flush=True)
+        assert a["unexpected"] == 0
+        assert np.array_equal(got_n, np.full(n, n - 1, np.int64))
+        assert np.abs(got_m - (mtot - m)).max() < 1e-12 * mtot
+        assert n_anc <= n // 1000
+
+
+@pytest.mark.parametrize("plant", [1, 2])
+def test_account_detects_a_dropped_or_doubled_list_entry(sph, orc, tables256, plant):
+    """The account has teeth: a list entry dropped from every list (plant 1)
+    or a leaf that scans its first entry twice (plant 2), planted in the
+    oracle, breaks the per-particle count."""
+    st, _ = _account_state(sph, orc, tables256, 20000, 41, "mean", {})
+    st.gravity()
+    st.gravity_up()
+    a = st.gravity_account(plant)
+    wrong = (a["count"] - a["excess_count"]) != 20000 - 1
+    assert wrong.mean() > 0.5
+
+
 # ------------------------------------------- expansion order 3 (P = 3)
 def _mp3_record(pos, m):
     """40-float P = 3 record: the 24-float record plus the octupole in the
