@@ -38,6 +38,9 @@ Compute::Compute(const sph_params& params, const DataStruct& data, const std::ve
         ctx = nullptr;
         return;
     }
+    check(sph_device_field(ctx, SPH_F_POS, &positionBuffer, nullptr), "sph_device_field");
+    check(sph_device_field(ctx, SPH_F_MAT, &materialIdBuffer, nullptr), "sph_device_field");
+    check(sph_device_field(ctx, SPH_F_RHO, &densityBuffer, nullptr), "sph_device_field");
     snapshot_ = (float*)sph_host_alloc(16 * (size_t)nParticles);
     if (!snapshot_) {
         status_ = SPH_E_CAPACITY;
@@ -63,21 +66,23 @@ Compute::~Compute() {
     if (builder_.joinable()) builder_.join();
     if (ctx) (void)sph_sync(ctx);
     sph_host_free(snapshot_);
-    sph_host_free(stageOctree_);
-    sph_host_free(stageLevels_);
+    for (int k = 0; k < 2; ++k) {
+        sph_host_free(stageOctree_[k]);
+        sph_host_free(stageLevels_[k]);
+    }
     sph_destroy(ctx);
 }
 
-void Compute::ensureStaging(int64_t nInts, int64_t nNodes) {
-    if (nInts > stageOctreeCap_) {
-        sph_host_free(stageOctree_);
-        stageOctreeCap_ = (int64_t)(nInts * 1.2) + 64;
-        stageOctree_ = (int32_t*)sph_host_alloc(4 * (size_t)stageOctreeCap_);
+void Compute::ensureStaging(int set, int64_t nInts, int64_t nNodes) {
+    if (nInts > stageOctreeCap_[set]) {
+        sph_host_free(stageOctree_[set]);
+        stageOctreeCap_[set] = (int64_t)(nInts * 1.2) + 64;
+        stageOctree_[set] = (int32_t*)sph_host_alloc(4 * (size_t)stageOctreeCap_[set]);
     }
-    if (nNodes > stageLevelsCap_) {
-        sph_host_free(stageLevels_);
-        stageLevelsCap_ = (int64_t)(nNodes * 1.2) + 64;
-        stageLevels_ = (int32_t*)sph_host_alloc(4 * (size_t)stageLevelsCap_);
+    if (nNodes > stageLevelsCap_[set]) {
+        sph_host_free(stageLevels_[set]);
+        stageLevelsCap_[set] = (int64_t)(nNodes * 1.2) + 64;
+        stageLevels_[set] = (int32_t*)sph_host_alloc(4 * (size_t)stageLevelsCap_[set]);
     }
 }
 
@@ -98,9 +103,12 @@ void Compute::buildAndUpload(const float* pos4) {
     treesSwapped_++;
 }
 
-void Compute::updateOctreeData() {
+void Compute::updateOctreeData(int set) {
     /* runs on the builder thread: touches only host memory and waits for the
-     * snapshot event */
+     * snapshot event.  The snapshot's device-to-host copy is queued on the
+     * copy stream after every tree upload issued so far, so once it has
+     * landed the last upload from staging set `set` has landed too and the
+     * set may be overwritten. */
     int rc = sph_wait_snapshot(ctx);
     sph_octree t;
     const auto t0 = std::chrono::steady_clock::now();
@@ -109,15 +117,15 @@ void Compute::updateOctreeData() {
                               treeThreads_, &t);
     lastBuildMs_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rc == SPH_OK) {
-        ensureStaging(t.n_ints, t.node_count);
-        if (!stageOctree_ || !stageLevels_) {
+        ensureStaging(set, t.n_ints, t.node_count);
+        if (!stageOctree_[set] || !stageLevels_[set]) {
             rc = SPH_E_CAPACITY;
         } else {
-            std::memcpy(stageOctree_, t.octree, 4 * (size_t)t.n_ints);
-            std::memcpy(stageLevels_, t.level_nodes, 4 * (size_t)t.node_count);
-            stageCounts_.assign(t.level_counts, t.level_counts + t.n_levels);
-            stageInts_ = t.n_ints;
-            stageNodes_ = t.node_count;
+            std::memcpy(stageOctree_[set], t.octree, 4 * (size_t)t.n_ints);
+            std::memcpy(stageLevels_[set], t.level_nodes, 4 * (size_t)t.node_count);
+            stageCounts_[set].assign(t.level_counts, t.level_counts + t.n_levels);
+            stageInts_[set] = t.n_ints;
+            stageNodes_[set] = t.node_count;
         }
         sph_octree_free(&t);
     }
@@ -143,7 +151,10 @@ void Compute::updateOctreeBuffer() {
         buildAndUpload(snapshot_);
         return;
     }
-    /* SPH_TREE_ASYNC (Compute.cpp:456-508) */
+    /* SPH_TREE_ASYNC (Compute.cpp:456-508): while a build runs, keep the
+     * current tree; once it has finished, swap its tree in and start the next
+     * build in the same call, from the positions at this point of the stream
+     * (those of the last completed step). */
     if (updating_.load(std::memory_order_acquire)) return;
     if (builder_.joinable()) builder_.join();
     if (stageBuildRc_ != 0) {
@@ -152,18 +163,21 @@ void Compute::updateOctreeBuffer() {
     }
     if (treeReady_.load(std::memory_order_acquire)) {
         treeReady_.store(false);
-        check(sph_upload_tree(ctx, stageOctree_, stageInts_, stageLevels_, stageCounts_.data(),
-                              (int)stageCounts_.size(), stageNodes_),
+        const int set = buildSet_;
+        check(sph_upload_tree(ctx, stageOctree_[set], stageInts_[set], stageLevels_[set], stageCounts_[set].data(),
+                              (int)stageCounts_[set].size(), stageNodes_[set]),
               "sph_upload_tree");
         treesSwapped_++;
-        return; /* the next rebuild starts once this upload has landed */
+        buildSet_ = 1 - set;
     }
-    const int done = sph_tree_upload_done(ctx);
-    if (done < 0) check(done, "sph_tree_upload_done");
-    if (done != 1) return;
+    if (status_ == 0) startBuild();
+}
+
+void Compute::startBuild() {
     check(sph_snapshot_positions(ctx, snapshot_), "sph_snapshot_positions");
+    if (status_ != 0) return;
     updating_.store(true, std::memory_order_release);
-    builder_ = std::thread(&Compute::updateOctreeData, this);
+    builder_ = std::thread(&Compute::updateOctreeData, this, buildSet_);
 }
 
 void Compute::setSnapshots(const std::string& dir, int resolution, int interval) {

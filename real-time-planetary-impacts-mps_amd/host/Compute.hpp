@@ -10,7 +10,10 @@
  * The octree is still built on a CPU thread (Compute.cpp:456-516), with two
  * differences: positions reach the thread through an explicit pinned
  * snapshot (the reference reads the shared Metal buffer while the GPU may be
- * writing it), and the hand-off uses atomics instead of a plain bool.
+ * writing it), and the hand-off uses atomics instead of a plain bool.  As in
+ * the reference, the call that swaps in a finished tree also starts the next
+ * build: the thread writes into one of two pinned staging sets while the
+ * other one's upload may still be in flight.
  */
 #ifndef SPH_HOST_COMPUTE_HPP
 #define SPH_HOST_COMPUTE_HPP
@@ -71,6 +74,13 @@ public:
 
     int nParticles = 0;
     sph_ctx* ctx = nullptr;
+    /* Compute.hpp:55-57: the buffers the renderer binds (Render.cpp:26), as
+     * device addresses (sph_device_field): positions (4 floats per
+     * particle), material ids (int32), densities (float); original-ID order,
+     * read-only, current after each Headless::step. */
+    void* positionBuffer = nullptr;
+    void* materialIdBuffer = nullptr;
+    void* densityBuffer = nullptr;
 
     /* first failing call (0 = none) and its message */
     int status() const { return status_; }
@@ -82,8 +92,9 @@ private:
     void check(int rc, const char* what);
     void buildAndUpload(const float* pos4);
     void buildOnDevice(); /* sph_build_tree (SPH_TREE_GPU) */
-    void updateOctreeData(); /* body of the octree thread (Compute.cpp:510-516) */
-    void ensureStaging(int64_t nInts, int64_t nNodes);
+    void updateOctreeData(int set); /* body of the octree thread (Compute.cpp:510-516) */
+    void startBuild();               /* snapshot + thread writing into the free staging set */
+    void ensureStaging(int set, int64_t nInts, int64_t nNodes);
 
     sph_params params_{};
     int treeMode_ = SPH_TREE_ASYNC;
@@ -93,12 +104,15 @@ private:
 
     /* octree hand-off */
     float* snapshot_ = nullptr;       /* pinned, 4 floats per particle */
-    int32_t* stageOctree_ = nullptr;  /* pinned staging of octreeData */
-    int32_t* stageLevels_ = nullptr;  /* pinned staging of the level lists */
-    int64_t stageOctreeCap_ = 0, stageLevelsCap_ = 0;
-    std::vector<int32_t> stageCounts_;
-    int64_t stageInts_ = 0;
-    int32_t stageNodes_ = 0;
+    /* two pinned staging sets of octreeData and the level lists: the thread
+     * fills set `buildSet_` while the other one may still be uploading */
+    int32_t* stageOctree_[2] = {nullptr, nullptr};
+    int32_t* stageLevels_[2] = {nullptr, nullptr};
+    int64_t stageOctreeCap_[2] = {0, 0}, stageLevelsCap_[2] = {0, 0};
+    std::vector<int32_t> stageCounts_[2];
+    int64_t stageInts_[2] = {0, 0};
+    int32_t stageNodes_[2] = {0, 0};
+    int buildSet_ = 0;
     int stageBuildRc_ = 0;
     std::thread builder_;
     std::atomic<bool> updating_{false};
