@@ -126,7 +126,12 @@ typedef struct sph_params {
      * beyond the geometric limit although its neglected order-4 moment is
      * not small. */
     int32_t mac_order;
-    int32_t reserved[2];
+    /* Density pass with density_lanes > 0 and lane_split 0: 1 = each lane
+     * walks the candidates of its cells as one flat loop (the same
+     * candidates in the same order as 0, so the same sums bit for bit);
+     * 0 = one loop per cell. */
+    int32_t density_flat;
+    int32_t reserved[1];
 } sph_params;
 
 /* One EOS table: res x res float2 (pressure, sound speed) in code units.
@@ -279,6 +284,19 @@ int64_t sph_field_bytes(const sph_ctx* ctx, int field);
 /* Synchronous copies (they drain the compute stream first). */
 int sph_download(sph_ctx* ctx, int field, void* dst);
 int sph_upload(sph_ctx* ctx, int field, const void* src);
+/* Device address and size of a per-particle field, for zero-copy readers
+ * such as a renderer: replaces the reference's public positionBuffer,
+ * materialIdBuffer and densityBuffer (Compute.hpp:55-57, bound by
+ * Render.cpp:26), and the other per-particle buffers alike.  The layout is
+ * sph_download's (original-ID order, float4 stride for 3-vectors).  The
+ * address stays valid for the context's life; the contents are those of
+ * the passes enqueued so far once sph_sync has returned (the reference reads
+ * its buffers after waitUntilCompleted).  Read-only for the caller: writes
+ * go through sph_upload, which keeps the context's derived copies in step.
+ * SPH_E_ARG for SPH_F_ACC (until the step pass adds them, the acceleration
+ * pass's pair sums sit in a buffer of their own: use sph_download) and for
+ * fields that are not per-particle arrays. */
+int sph_device_field(sph_ctx* ctx, int field, void** dev_ptr, int64_t* bytes);
 
 /* ------------------------------------------------------------------ timing */
 enum sph_pass {

@@ -181,6 +181,10 @@ int sph_headless_enqueue_step(sph_headless* h);
 sph_ctx* sph_headless_ctx(sph_headless* h);
 /* Tree statistics: number of trees swapped in so far and the last build ms. */
 int sph_headless_tree_stats(sph_headless* h, int64_t* trees_swapped, double* last_build_ms);
+/* Compute's public positionBuffer, materialIdBuffer and densityBuffer
+ * (Compute.hpp:55-57, bound by Render.cpp:26) as device addresses; any
+ * pointer may be NULL. */
+int sph_headless_render_buffers(sph_headless* h, void** positions, void** material_ids, void** densities);
 const char* sph_headless_last_error(const sph_headless* h);
 
 #ifdef __cplusplus

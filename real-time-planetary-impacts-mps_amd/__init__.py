@@ -63,7 +63,8 @@ class SphParams(ctypes.Structure):
         ("cell_width", ctypes.c_float), ("cell_power", ctypes.c_int32),
         ("fixes", ctypes.c_uint32), ("precision", ctypes.c_int32), ("density_lanes", ctypes.c_int32),
         ("accel_lanes", ctypes.c_int32), ("lane_split", ctypes.c_int32),
-        ("expansion_order", ctypes.c_int32), ("mac_order", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
+        ("expansion_order", ctypes.c_int32), ("mac_order", ctypes.c_int32), ("density_flat", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -121,7 +122,7 @@ CAPI_SYMBOLS = [
     "sph_samples_count", "sph_samples_read", "sph_samples_append", "sph_samples_clear", "sph_set_sample_dump",
     "sph_set_learned_guess", "sph_features", "sph_conservation", "sph_time",
     "sph_build_tree", "sph_tree_info", "sph_download_tree", "sph_draw_snapshot", "sph_download_snapshot",
-    "sph_snapshot_resolution",
+    "sph_snapshot_resolution", "sph_device_field",
 ]
 HOST_SYMBOLS = [
     "sph_tillotson_granite", "sph_tillotson_iron", "sph_make_eos_table", "sph_impact_config_default",
@@ -131,6 +132,7 @@ HOST_SYMBOLS = [
     "sph_read_hdf", "sph_particles_free", "sph_write_hdf", "sph_checkpoint_write", "sph_checkpoint_read",
     "sph_hdf_last_error", "sph_headless_create_from_file", "sph_load_aneos", "sph_aneos_last_error",
     "sph_headless_create_with_aneos", "sph_headless_set_snapshots", "sph_write_snapshot_png",
+    "sph_headless_render_buffers",
 ]
 
 _hip: Optional[ctypes.CDLL] = None
@@ -173,6 +175,8 @@ def libs():
         h.sph_field_bytes.restype = ctypes.c_int64
         h.sph_download.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         h.sph_upload.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        h.sph_device_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_int64)]
         h.sph_upload_tree.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int,
                                       ctypes.c_int32]
@@ -237,6 +241,7 @@ def libs():
         o.sph_headless_ctx.restype = ctypes.c_void_p
         o.sph_headless_tree_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                               ctypes.POINTER(ctypes.c_double)]
+        o.sph_headless_render_buffers.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_void_p)] * 3
         o.sph_headless_last_error.argtypes = [ctypes.c_void_p]
         o.sph_headless_last_error.restype = ctypes.c_char_p
         o.sph_read_hdf.argtypes = [ctypes.c_char_p, ctypes.POINTER(SphParticles)]
@@ -515,6 +520,13 @@ class Context:
         self._check(self._h.sph_download(self.ptr, fid, out.ctypes.data_as(ctypes.c_void_p)), f"download {field}")
         return out
 
+    def device_field(self, field: str):
+        """(device address, bytes) of a per-particle field (sph_device_field)."""
+        ptr, nb = ctypes.c_void_p(), ctypes.c_int64()
+        self._check(self._h.sph_device_field(self.ptr, FIELD_ID[field], ctypes.byref(ptr), ctypes.byref(nb)),
+                    "sph_device_field")
+        return int(ptr.value or 0), int(nb.value)
+
     def upload(self, field: str, arr: np.ndarray):
         fid = FIELD_ID[field]
         nbytes = self._h.sph_field_bytes(self.ptr, fid)
@@ -728,6 +740,13 @@ class Headless:
         rc = self._o.sph_headless_enqueue_step(self.ptr)
         if rc != SPH_OK:
             raise SphError(f"Headless::step failed ({rc}): {self._o.sph_headless_last_error(self.ptr).decode()}")
+
+    def render_buffers(self):
+        """Compute's positionBuffer, materialIdBuffer, densityBuffer (device addresses)."""
+        p = [ctypes.c_void_p() for _ in range(3)]
+        if self._o.sph_headless_render_buffers(self.ptr, *[ctypes.byref(x) for x in p]) != SPH_OK:
+            raise SphError("sph_headless_render_buffers failed")
+        return tuple(int(x.value or 0) for x in p)
 
     def tree_stats(self):
         n = ctypes.c_int64()

@@ -33,6 +33,7 @@ bool valid_params(const sph_params* p, std::string& why) {
     if ((p->density_lanes == 2 || p->density_lanes == 4 || p->accel_lanes == 2 || p->accel_lanes == 4) &&
         p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }
+    if (p->density_flat != 0 && p->density_flat != 1) { why = "density_flat must be 0 or 1"; return false; }
     if (p->expansion_order != 2 && p->expansion_order != 3) { why = "expansion_order must be 2 or 3"; return false; }
     if (p->mac_order != 0 && p->mac_order != 2) { why = "mac_order must be 0 or 2"; return false; }
     if (p->expansion_order == 3 && p->gravity_max_recursion != 1 && !(p->fixes & SPH_FIX_NEAR_FIELD)) {
@@ -572,6 +573,7 @@ int sph_density(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
     if (!ctx->have_prev) return ctx_fail(ctx, SPH_E_STATE, "density before any sort");
     int rc = ctx_begin_pass(ctx, SPH_PASS_DENSITY);
+    if (rc == SPH_OK) rc = refresh_morton(ctx);
     if (rc == SPH_OK) rc = launch_density_prologue(ctx);
     if (rc == SPH_OK) rc = launch_density(ctx);
     if (rc == SPH_OK) rc = launch_density_epilogue(ctx);
@@ -687,6 +689,25 @@ int sph_download(sph_ctx* ctx, int field, void* dst) {
     void* src = field_ptr(ctx, field);
     if (!src) return ctx_fail(ctx, SPH_E_ARG, "field not downloadable");
     SPH_HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int sph_device_field(sph_ctx* ctx, int field, void** dev_ptr, int64_t* bytes) {
+    if (!ctx || !dev_ptr) return SPH_E_ARG;
+    switch (field) {
+        case SPH_F_POS: case SPH_F_VEL: case SPH_F_RHO: case SPH_F_U: case SPH_F_MASS: case SPH_F_PRESSURE:
+        case SPH_F_H: case SPH_F_MAT: case SPH_F_GRADTERM: case SPH_F_SOUNDSPEED: case SPH_F_DU:
+        case SPH_F_BALSARA: case SPH_F_DHDT: case SPH_F_ACTIVE: case SPH_F_NEXT_ACTIVE: case SPH_F_GRAVNORM:
+        case SPH_F_NR_ITERS: case SPH_F_POTENTIAL: case SPH_F_DLNH:
+            break;
+        case SPH_F_ACC:
+            return ctx_fail(ctx, SPH_E_ARG, "acc holds the gravity part until the step adds the pair sums: "
+                                            "use sph_download");
+        default:
+            return ctx_fail(ctx, SPH_E_ARG, "not a per-particle field");
+    }
+    *dev_ptr = field_ptr(ctx, field);
+    if (bytes) *bytes = field_bytes(ctx, field);
     return SPH_OK;
 }
 

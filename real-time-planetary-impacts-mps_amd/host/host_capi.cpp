@@ -113,6 +113,15 @@ int sph_headless_tree_stats(sph_headless* w, int64_t* trees_swapped, double* las
     return SPH_OK;
 }
 
+int sph_headless_render_buffers(sph_headless* w, void** positions, void** material_ids, void** densities) {
+    if (!w || !w->h->compute) return SPH_E_ARG;
+    const Compute* c = w->h->compute;
+    if (positions) *positions = c->positionBuffer;
+    if (material_ids) *material_ids = c->materialIdBuffer;
+    if (densities) *densities = c->densityBuffer;
+    return SPH_OK;
+}
+
 const char* sph_headless_last_error(const sph_headless* w) {
     if (!w) return "null handle";
     const_cast<sph_headless*>(w)->err = w->h->error();
