@@ -19,6 +19,11 @@ import os
 import sys
 import time
 
+# OpenMP placement of the CPU baseline (the oracle): one thread per core,
+# packed, fixed before any OpenMP runtime starts
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -54,7 +59,9 @@ def parse():
                     help="octree: CPU thread (async, as the reference), CPU before every step, the first one kept, "
                          "or built on the GPU before every step")
     ap.add_argument("--tree-threads", type=int, default=8)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--c1-steps", type=int, default=100,
+                    help="steps of the 2e4-particle run on the CPU path and on the GPU (0 = skip)")
     ap.add_argument("--no-timing", action="store_true", help="do not record per-pass HIP events")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes 5 times (calibrates the "
@@ -115,6 +122,55 @@ def density_flops(work):
     return 9.0 * w[0] + 9.0 * w[1] + 9.0 * w[2] + 54.0 * w[3]
 
 
+def host_cpu():
+    """CPU model, physical cores and logical CPUs of this host (lscpu's
+    sources: /proc/cpuinfo and the process's CPU affinity).  Called before
+    the OpenMP runtime starts: with OMP_PROC_BIND set it binds this thread
+    to one place, and the affinity would then read 1."""
+    model, phys = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            cur = {}
+            for line in f:
+                if ":" in line:
+                    k, v = [x.strip() for x in line.split(":", 1)]
+                    cur[k] = v
+                    if k == "model name":
+                        model = v
+                elif cur:
+                    phys.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+                    cur = {}
+            if cur:
+                phys.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"cpu_model": model, "physical_cores_host": len(phys) or None, "logical_cpus_usable": usable}
+
+
+CPU_PASSES = ("sort", "gravity", "density", "acceleration", "step")
+
+
+def oracle_steps(oracle, st, prm, steps, rebuild_tree):
+    """Runs the oracle for `steps` steps with per-pass wall time; the octree
+    (serial CPU build, as in the reference) is timed apart."""
+    ms = {k: 0.0 for k in CPU_PASSES + ("octree",)}
+    t_all = time.perf_counter()
+    for i in range(steps):
+        if rebuild_tree or i == 0:
+            t0 = time.perf_counter()
+            st.set_tree(oracle.build_octree(st["pos"], prm.max_children_in_leaf, prm.max_tree_depth))
+            ms["octree"] += 1000 * (time.perf_counter() - t0)
+        for name in CPU_PASSES:
+            t0 = time.perf_counter()
+            getattr(st, name)()
+            ms[name] += 1000 * (time.perf_counter() - t0)
+    wall = time.perf_counter() - t_all
+    step_ms = sum(ms[k] for k in CPU_PASSES)
+    return {"steps": steps, "steps_per_s": steps / (step_ms / 1000), "steps_per_s_with_octree": steps / wall,
+            "pass_ms": {k: round(v / steps, 3) for k, v in ms.items()}}
+
+
 def cpu_baseline(sph, oracle, hl, prm, tables, steps):
     """Times the oracle on the warmed GPU state (same particles, same tree rule)."""
     ctx = hl.ctx
@@ -124,27 +180,53 @@ def cpu_baseline(sph, oracle, hl, prm, tables, steps):
               "global_time", "dt"):
         st[f] = ctx.download(f)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    tree = oracle.build_octree(st["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
-    t_tree = time.perf_counter() - t0
-    st.set_tree(tree)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        st.sort()
-        st.gravity()
-        st.density()
-        st.acceleration()
-        st.step()
-    dt = time.perf_counter() - t0
+    r = oracle_steps(oracle, st, prm, steps, rebuild_tree=False)
+    cpu = HOST_CPU
     return {
-        "value": steps / dt,
+        "value": r["steps_per_s"],
         "unit": "steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{steps} step(s) of the same {st.n}-particle impact from the warmed GPU state "
-                   f"(sort, gravity, density, acceleration, step), OpenMP over {threads} threads; the serial "
-                   f"octree build ({t_tree:.2f} s) is excluded as it runs on a separate thread in the GPU path"),
+        "sample": (f"{steps} steps of the same {st.n}-particle impact from the warmed GPU state "
+                   f"(sort, gravity, density, acceleration, step), OpenMP over {threads} threads "
+                   f"(OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, OMP_PLACES={os.environ.get('OMP_PLACES')}) "
+                   f"on {cpu['cpu_model']}; the serial octree build ({r['pass_ms']['octree']:.0f} ms) is excluded "
+                   "as it runs on a separate thread in the GPU path"),
+        "pass_ms": r["pass_ms"],
+        **cpu,
     }
+
+
+def c1_run(sph, oracle, args):
+    """The 2e4-particle impact (89% / 11%, seed 1), 100 global steps, on the
+    CPU path (the oracle, octree rebuilt serially before every step) and on
+    the GPU (Headless, octree rebuilt before every step): steps/s and
+    per-pass ms of both."""
+    prm = sph.default_params()
+    tables = sph.make_eos_tables(prm, 256)
+    cfg = sph.impact_config(20000, 1)
+    ic = sph.make_impact(prm, cfg, tables)
+    st = oracle.State(prm, ic, tables)
+    cpu = oracle_steps(oracle, st, prm, args.c1_steps, rebuild_tree=True)
+    cpu["threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    hl = sph.Headless(prm, cfg, eos_res=256, tree_mode=sph.TREE_EVERY, tree_threads=args.tree_threads)
+    ctx = hl.ctx
+    ctx.reset_pass_times()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.c1_steps):
+        hl.step()
+    ctx.sync()
+    el = time.perf_counter() - t0
+    passes = ctx.pass_times()
+    ctx.set_timing(False)
+    gpu = {"steps": args.c1_steps, "steps_per_s_with_octree": args.c1_steps / el,
+           "pass_ms": {k: round(v["ms"] / v["calls"], 4) for k, v in passes.items() if v["calls"]}}
+    d = float(np.abs(ctx.download("pos")[:, :3] - st["pos"][:, :3]).max())
+    hl.close()
+    return {"workload": f"two-body 45-degree impact, 20000 particles, {args.c1_steps} steps, reference step as "
+                        "written, octree rebuilt before every step", "cpu": cpu, "gpu": gpu,
+            "final_positions_max_abs_diff": d}
 
 
 def learned_comparison(sph, args):
@@ -332,7 +414,11 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": src.numel() * 4, "launches": 5}))
 
 
+HOST_CPU = {}
+
+
 def main():
+    HOST_CPU.update(host_cpu())
     args = parse()
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
@@ -461,6 +547,11 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sph, oracle, hl, prm, tables, args.cpu_steps)
         log("cpu baseline done")
     hl.close()
+    if args.c1_steps > 0:
+        import oracle
+        log("2e4-particle run, CPU path and GPU")
+        out["c1_2e4"] = c1_run(sph, oracle, args)
+        log("2e4-particle run done")
     print(json.dumps(out))
 
 

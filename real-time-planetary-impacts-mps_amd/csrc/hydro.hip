@@ -137,6 +137,28 @@ struct DensityArgs {
 };
 
 constexpr float kGuessTrust = 0.05f;
+/* Successive changes below this are left to the plain iteration's stopping
+ * rule: near the fixed point they are rounding noise of the density sum and
+ * may grow from one scan to the next without the guess being wrong. */
+constexpr float kGuessNoise = 1e-5f;
+
+/* The learned starting guess is judged on its first two scans only: the
+ * first residual must be below kGuessTrust and the second must not exceed
+ * the first (unless both are at the noise level).  A guess that fails either
+ * test is dropped and the iteration restarts from the step's h; one that
+ * passes both is then iterated exactly as the plain method would iterate
+ * from that point.  Returns true when the iteration must restart. */
+__device__ __forceinline__ bool guess_rejected(bool& guided, float eps, float& eps_last) {
+    if (!guided) return false;
+    const bool first = eps_last > 1.0e38f;
+    if (first ? eps > kGuessTrust : (eps > eps_last && eps > kGuessNoise)) {
+        guided = false;
+        return true;
+    }
+    if (first) eps_last = eps;
+    else guided = false;
+    return false;
+}
 
 /* density.metal:12-239 */
 __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
@@ -166,10 +188,8 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     const int mask = a.cells.mask;
 
     /* With a learned starting guess the iteration starts there and restarts
-     * from the step's h once the residual grows, or when the first residual
-     * exceeds kGuessTrust (a guess that far off is not worth iterating
-     * from).  No counterpart in the reference, whose iteration always
-     * starts from h. */
+     * from the step's h when guess_rejected() says so.  No counterpart in the
+     * reference, whose iteration always starts from h. */
     const float h_start = a.h[ind];
     float h_i = a.guess ? a.guess[index] : h_start;
     bool guided = a.guess != nullptr;
@@ -227,18 +247,14 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
         float newH = fminf(a.res_eta * powf(mass_i / density, 1.f / 3.f), a.max_h_len);
         newH = 0.5f * (h_i + newH);
         eps = fabsf(newH - h_i) / h_i;
-        if (guided) {
-            if (eps > eps_last || eps > kGuessTrust) {
-                guided = false;
-                h_i = h_start;
-                h1 = 1.f / h_i;
-                h2x4 = 4.f * h_i * h_i;
-                max_h = 0;
-                eps = 1.f;
-                count++;
-                continue;
-            }
-            eps_last = eps;
+        if (guess_rejected(guided, eps, eps_last)) {
+            h_i = h_start;
+            h1 = 1.f / h_i;
+            h2x4 = 4.f * h_i * h_i;
+            max_h = 0;
+            eps = 1.f;
+            count++;
+            continue;
         }
         if (eps > a.eta) {
             max_h = (int)fmaxf((float)max_h, h_i);
@@ -370,7 +386,86 @@ __device__ __forceinline__ void box_cells(const int mn[3], const int mx[3], int 
     }
 }
 
-template <int G, int SPLIT>
+/* Calls on_cell(count) for each non-empty cell and body(k) for every
+ * candidate slot k of the cells first, first + STEP, ... of the box
+ * [mn, mx] in x -> y -> z order, like box_cells with a loop over each
+ * cell's slots, but as one flat loop: a lane moves on to its next cell as
+ * soon as it has used up the current one, instead of every lane of the wave
+ * waiting at each cell for the lane with the most candidates.  The slot range
+ * of a lane's next cell is loaded one cell ahead.  The slots and their order
+ * are those of box_cells, so the sums are bit for bit the same. */
+template <int STEP, class CellFn, class Body>
+__device__ __forceinline__ void box_slots_flat(const int mn[3], const int mx[3], int first, int mask,
+                                               const uint2* __restrict__ cse, CellFn on_cell, Body body) {
+    const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
+    const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
+    int c = first;
+    if (c >= ncell) return;
+    int iz = c % nz;
+    const int t = c / nz;
+    int iy = t % ny, ix = t / ny;
+    uint32_t exy = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
+                   (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1);
+    /* the cell after the current one: its slot range, loaded ahead */
+    uint2 nxt = cse[exy | (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2)];
+    bool more = true;
+    uint32_t k = 1, end = 0; /* current range, empty */
+    for (;;) {
+        if (k > end) {
+            if (!more) break;
+            k = nxt.x;
+            end = nxt.y;
+            if (k == 0xFFFFFFFFu) {
+                k = 1;
+                end = 0;
+            } else {
+                on_cell(end - k + 1);
+            }
+            c += STEP;
+            if (c < ncell) {
+                iz += STEP;
+                if (iz >= nz) {
+                    do {
+                        iz -= nz;
+                        if (++iy >= ny) {
+                            iy = 0;
+                            ++ix;
+                        }
+                    } while (iz >= nz);
+                    exy = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
+                          (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1);
+                }
+                nxt = cse[exy | (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2)];
+            } else {
+                more = false;
+            }
+            if (k > end) continue;
+        }
+        body(k);
+        ++k;
+    }
+}
+
+/* The candidate slots of a particle's box for lane `lane` of its G lanes:
+ * SPLIT 0 whole cells per lane (per-cell loops, or FLAT: one flat loop over
+ * the lane's cells), SPLIT 1 the candidates of each cell split over the
+ * lanes.  on_cell(count) per non-empty cell (on lane 0 only with SPLIT). */
+template <int G, int SPLIT, int FLAT, class CellFn, class Body>
+__device__ __forceinline__ void box_slots(const int mn[3], const int mx[3], int lane, int mask,
+                                          const uint2* __restrict__ cse, CellFn on_cell, Body body) {
+    if (FLAT && !SPLIT) {
+        box_slots_flat<G>(mn, mx, lane, mask, cse, on_cell, body);
+        return;
+    }
+    box_cells<SPLIT ? 1 : G>(mn, mx, SPLIT ? 0 : lane, mask, [&](uint32_t key) {
+        const uint2 se = cse[key];
+        if (se.x == 0xFFFFFFFFu) return;
+        if (!SPLIT || lane == 0) on_cell(se.y - se.x + 1);
+        for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) body(k);
+    });
+}
+
+template <int G, int SPLIT, int FLAT>
 __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     constexpr int PPB = 256 / G;
     const int blk = xcd_block((a.n + PPB - 1) / PPB);
@@ -424,24 +519,20 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
         cell_box(position, cs, hb, mask + 1, mn, mx);
         float part = 0.f;
         int cnt = 0;
-        box_cells<SPLIT ? 1 : G>(mn, mx, SPLIT ? 0 : lane, mask, [&](uint32_t key) {
-            const uint2 se = cse[key];
-            if (se.x == 0xFFFFFFFFu) return;
-            if (!SPLIT || lane == 0) w_cand += (int)(se.y - se.x + 1);
-            for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) {
-                const float4 q = spm[k];
-                const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > hb2) continue;
-                w_in++;
-                const float r = __builtin_amdgcn_sqrtf(r2);
-                if (use_cache) {
-                    if (r > hsum) continue;
-                } else {
-                    cnt++;
-                }
-                part += kW(r, h1) * q.w;
+        box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, [&](uint32_t nc) { w_cand += (int)nc; },
+                                  [&](uint32_t k) {
+            const float4 q = spm[k];
+            const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > hb2) return;
+            w_in++;
+            const float r = __builtin_amdgcn_sqrtf(r2);
+            if (use_cache) {
+                if (r > hsum) return;
+            } else {
+                cnt++;
             }
+            part += kW(r, h1) * q.w;
         });
         density = gsum<G>(part);
         if (!use_cache) {
@@ -452,18 +543,14 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
         float newH = fminf(a.res_eta * powf(mass_i / density, 1.f / 3.f), a.max_h_len);
         newH = 0.5f * (h_i + newH);
         eps = fabsf(newH - h_i) / h_i;
-        if (guided) {
-            if (eps > eps_last || eps > kGuessTrust) {
-                guided = false;
-                h_i = h_start;
-                h1 = 1.f / h_i;
-                h2x4 = 4.f * h_i * h_i;
-                max_h = 0;
-                eps = 1.f;
-                count++;
-                continue;
-            }
-            eps_last = eps;
+        if (guess_rejected(guided, eps, eps_last)) {
+            h_i = h_start;
+            h1 = 1.f / h_i;
+            h2x4 = 4.f * h_i * h_i;
+            max_h = 0;
+            eps = 1.f;
+            count++;
+            continue;
         }
         if (eps > a.eta) {
             max_h = (int)fmaxf((float)max_h, h_i);
@@ -485,32 +572,28 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
         const float hsum = h_i + h_i;
         int mn[3], mx[3];
         cell_box(position, cs, hb, mask + 1, mn, mx);
-        box_cells<SPLIT ? 1 : G>(mn, mx, SPLIT ? 0 : lane, mask, [&](uint32_t key) {
-            const uint2 se = cse[key];
-            if (se.x == 0xFFFFFFFFu) return;
-            if (!SPLIT || lane == 0) f_cand += (int)(se.y - se.x + 1);
-            for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) {
-                const float4 q = spm[k];
-                const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > hb2) continue;
-                f_in++;
-                const float r = __builtin_amdgcn_sqrtf(r2);
-                if (cached && r > hsum) continue;
-                const float mass = q.w;
-                if (!cached) dpart += kW(r, h1) * mass;
-                density_h += mass * kdWdh(r, h1);
-                if (r > 1e-12f) {
-                    const float r1 = __builtin_amdgcn_rcpf(r);
-                    const float4 vj = a.s_vel[k];
-                    const float vx = vj.x - vi4.x, vy = vj.y - vi4.y, vz = vj.z - vi4.z;
-                    const float g = kGradWFactor(r, h1);
-                    const float gx = g * (dx * r1), gy = g * (dy * r1), gz = g * (dz * r1);
-                    velDiv += mass * (vx * gx + vy * gy + vz * gz);
-                    cx += mass * (vy * gz - vz * gy);
-                    cy += mass * (vz * gx - vx * gz);
-                    cz += mass * (vx * gy - vy * gx);
-                }
+        box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, [&](uint32_t nc) { f_cand += (int)nc; },
+                                  [&](uint32_t k) {
+            const float4 q = spm[k];
+            const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > hb2) return;
+            f_in++;
+            const float r = __builtin_amdgcn_sqrtf(r2);
+            if (cached && r > hsum) return;
+            const float mass = q.w;
+            if (!cached) dpart += kW(r, h1) * mass;
+            density_h += mass * kdWdh(r, h1);
+            if (r > 1e-12f) {
+                const float r1 = __builtin_amdgcn_rcpf(r);
+                const float4 vj = a.s_vel[k];
+                const float vx = vj.x - vi4.x, vy = vj.y - vi4.y, vz = vj.z - vi4.z;
+                const float g = kGradWFactor(r, h1);
+                const float gx = g * (dx * r1), gy = g * (dy * r1), gz = g * (dz * r1);
+                velDiv += mass * (vx * gx + vy * gy + vz * gz);
+                cx += mass * (vy * gz - vz * gy);
+                cy += mass * (vz * gx - vx * gz);
+                cz += mass * (vx * gy - vy * gx);
             }
         });
     }
@@ -608,6 +691,22 @@ __global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __rest
     s_posh[k] = make_float4(x.x, x.y, x.z, h[j]);
     s_velb[k] = make_float4(v.x, v.y, v.z, B[j]);
     s_hyd[k] = make_float4(mj, f[j] * P[j] / (r * r), r, c[j]);
+}
+
+/* The sort's Morton-ordered copies (position + mass, velocity) gathered
+ * again from original order, for a density pass that follows a change of
+ * positions, velocities or masses without a new sort (the reference's
+ * density pass reads the position buffer itself). */
+__global__ __launch_bounds__(256) void k_refresh_morton(int n, const uint2* __restrict__ sorted,
+                                                        const float4* __restrict__ pos,
+                                                        const float4* __restrict__ vel, const float* __restrict__ m,
+                                                        float4* __restrict__ s_posm, float4* __restrict__ s_vel) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = sorted[k].y;
+    const float4 x = pos[j];
+    s_posm[k] = make_float4(x.x, x.y, x.z, m[j]);
+    s_vel[k] = vel[j];
 }
 
 struct AccelArgs {
@@ -933,6 +1032,15 @@ int launch_active_list(sph_ctx* ctx) {
     return SPH_OK;
 }
 
+int refresh_morton(sph_ctx* ctx) {
+    if (ctx->morton_fresh) return SPH_OK;
+    k_refresh_morton<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m,
+                                                                     ctx->s_posm, ctx->s_vel);
+    SPH_HIP_TRY(hipGetLastError());
+    ctx->morton_fresh = true;
+    return SPH_OK;
+}
+
 int launch_density(sph_ctx* ctx) {
     const sph_params& p = ctx->prm;
     DensityArgs a;
@@ -979,20 +1087,30 @@ int launch_density(sph_ctx* ctx) {
         SPH_HIP_TRY(hipGetLastError());
     }
     const int threads = 256;
-    const int dsel = p.density_lanes * 2 + (p.lane_split ? 1 : 0);
+    const int flat = p.density_flat ? 1 : 0;
+    const int dsel = (p.density_lanes * 2 + (p.lane_split ? 1 : 0)) * 2 + flat;
+#define SPH_DENS(G, S, F, PPB) \
+    k_density_group<G, S, F><<<xcd_grid((ctx->n + (PPB) - 1) / (PPB)), threads, 0, ctx->stream>>>(a)
     switch (dsel) {
-        case 4: k_density_group<2, 0><<<xcd_grid((ctx->n + 127) / 128), threads, 0, ctx->stream>>>(a); break;
-        case 8: k_density_group<4, 0><<<xcd_grid((ctx->n + 63) / 64), threads, 0, ctx->stream>>>(a); break;
-        case 16: k_density_group<8, 0><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
-        case 17: k_density_group<8, 1><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
-        case 32: k_density_group<16, 0><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
-        case 33: k_density_group<16, 1><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
-        case 64: k_density_group<32, 0><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
-        case 65: k_density_group<32, 1><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
-        case 128: k_density_group<64, 0><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
-        case 129: k_density_group<64, 1><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
+        case 8: SPH_DENS(2, 0, 0, 128); break;
+        case 9: SPH_DENS(2, 0, 1, 128); break;
+        case 16: SPH_DENS(4, 0, 0, 64); break;
+        case 17: SPH_DENS(4, 0, 1, 64); break;
+        case 32: SPH_DENS(8, 0, 0, 32); break;
+        case 33: SPH_DENS(8, 0, 1, 32); break;
+        case 34: case 35: SPH_DENS(8, 1, 0, 32); break;
+        case 64: SPH_DENS(16, 0, 0, 16); break;
+        case 65: SPH_DENS(16, 0, 1, 16); break;
+        case 66: case 67: SPH_DENS(16, 1, 0, 16); break;
+        case 128: SPH_DENS(32, 0, 0, 8); break;
+        case 129: SPH_DENS(32, 0, 1, 8); break;
+        case 130: case 131: SPH_DENS(32, 1, 0, 8); break;
+        case 256: SPH_DENS(64, 0, 0, 4); break;
+        case 257: SPH_DENS(64, 0, 1, 4); break;
+        case 258: case 259: SPH_DENS(64, 1, 0, 4); break;
         default: k_density<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a); break;
     }
+#undef SPH_DENS
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }

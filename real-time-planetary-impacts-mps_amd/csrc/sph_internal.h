@@ -215,6 +215,8 @@ int ctx_fold_times(sph_ctx* ctx);
 /* sort.hip */
 int launch_sort(sph_ctx* ctx);
 /* hydro.hip */
+/* regathers s_posm / s_vel from original order unless morton_fresh */
+int refresh_morton(sph_ctx* ctx);
 int launch_density(sph_ctx* ctx);
 int launch_acceleration(sph_ctx* ctx);
 /* acc += dv for the last acceleration pass's particles, if not yet done */

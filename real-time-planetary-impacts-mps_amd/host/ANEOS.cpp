@@ -56,7 +56,7 @@ bool read_list(std::ifstream& in, int count, std::vector<float>& out) {
 bool parse(const std::string& path, RawTable& t, std::string& err) {
     std::ifstream in(path);
     if (!in.is_open()) {
-        err = "Could not open ANEOS data file: " + path;
+        err = path + ": cannot open the table file";
         return false;
     }
     std::string line;
@@ -108,7 +108,8 @@ bool parse(const std::string& path, RawTable& t, std::string& err) {
         }
     }
     if (ti < t.numT) {
-        err = "Not enough pressure data found in file";
+        err = path + ": the table ends after " + std::to_string(ti) + " of " + std::to_string(t.numT) +
+              " temperature rows";
         return false;
     }
     return true;
@@ -137,30 +138,31 @@ void resample(const RawTable& t, const sph_params& prm, int res, bool fixed, flo
     for (int ri = 0; ri < res; ++ri) {
         const float rho = rhoAxis[ri];
         while (rho > t.rho[rT + 1] && rT < nR - 2) ++rT;
-        const float rhoLFact = (rho - t.rho[rT]) / (t.rho[rT + 1] - t.rho[rT]);
-        const float rhoHFact = 1 - rhoLFact;
-        int ulT = 0, uhT = 0;
+        /* fr: position of rho inside its density bracket [rT, rT + 1] */
+        const float fr = (rho - t.rho[rT]) / (t.rho[rT + 1] - t.rho[rT]);
+        int lo = 0, hi = 0; /* energy brackets in density rows rT and rT + 1 */
         for (int ui = 0; ui < res; ++ui) {
             const float u = uAxis[ui];
-            while (u > U(rT, ulT + 1) && ulT < nT - 2) ++ulT;
+            while (u > U(rT, lo + 1) && lo < nT - 2) ++lo;
             if (fixed) {
-                while (u > U(rT + 1, uhT + 1) && uhT < nT - 2) ++uhT;
+                while (u > U(rT + 1, hi + 1) && hi < nT - 2) ++hi;
             } else {
-                while (u > U(rT + 1, uhT + 1) && ulT < nT - 2 && (int64_t)(rT + 1) * nT + uhT + 1 < N - 1) ++uhT;
+                /* as written the upper row's search is bounded by the lower
+                 * row's index (it may run into the next row) */
+                while (u > U(rT + 1, hi + 1) && lo < nT - 2 && (int64_t)(rT + 1) * nT + hi + 1 < N - 1) ++hi;
             }
-            const float uLLFact = (u - U(rT, ulT)) / (U(rT, ulT + 1) - U(rT, ulT));
-            const float uLHFact = (u - U(rT + 1, uhT)) / (U(rT + 1, uhT + 1) - U(rT + 1, uhT));
-            const float uHLFact = 1 - uLLFact;
-            const float uHHFact = 1 - uLHFact;
+            /* f0, f1: position of u inside the bracket of row rT, row rT + 1 */
+            const float f0 = (u - U(rT, lo)) / (U(rT, lo + 1) - U(rT, lo));
+            const float f1 = (u - U(rT + 1, hi)) / (U(rT + 1, hi + 1) - U(rT + 1, hi));
             for (int comp = 0; comp < 2; ++comp) {
                 const std::vector<float>& D = comp == 0 ? t.p : t.c;
-                const float a = flat(D, rT, ulT), b = flat(D, rT, ulT + 1);
-                const float c = flat(D, rT + 1, uhT), d = flat(D, rT + 1, uhT + 1);
+                const float v00 = flat(D, rT, lo), v01 = flat(D, rT, lo + 1);
+                const float v10 = flat(D, rT + 1, hi), v11 = flat(D, rT + 1, hi + 1);
                 float v;
-                if (fixed)
-                    v = rhoHFact * (a * uHLFact + b * uLLFact) + rhoLFact * (c * uHHFact + d * uLHFact);
-                else
-                    v = rhoLFact * (a * uLLFact + b * uHLFact) + rhoHFact * (c * uLHFact + d * uHHFact);
+                if (fixed) /* bilinear */
+                    v = (1 - fr) * (v00 * (1 - f0) + v01 * f0) + fr * (v10 * (1 - f1) + v11 * f1);
+                else /* as written: every node takes the weight of the opposite node */
+                    v = fr * (v00 * f0 + v01 * (1 - f0)) + (1 - fr) * (v10 * f1 + v11 * (1 - f1));
                 pc[2 * ((int64_t)ri * res + ui) + comp] = v;
             }
         }

@@ -33,7 +33,7 @@ bool read_vec3(h5::Reader& r, const std::string& name, std::vector<float>& out4,
     }
     /* hdfHandler.cpp:41-50 */
     if (dims.size() != 2 || dims[1] != 3) {
-        err = "Dataset " + name + " does not have 3 columns as expected.";
+        err = std::string(kGroup) + "/" + name + ": a vector dataset needs shape (n, 3)";
         return false;
     }
     n = (size_t)dims[0];

@@ -162,9 +162,9 @@ def test_parse_layout_and_errors(sph, tmp_path):
     assert np.allclose(tab[..., 0], 1.0, rtol=1e-5) and np.allclose(tab[..., 1], 1.0, rtol=1e-5)
     short = tmp_path / "short.txt"
     short.write_text("\n".join(lines[:-3]) + "\n")
-    with pytest.raises(sph.SphError, match="Not enough pressure data"):
+    with pytest.raises(sph.SphError, match="the table ends after"):
         sph.load_aneos(str(short), prm, 8)
-    with pytest.raises(sph.SphError, match="Could not open ANEOS data file"):
+    with pytest.raises(sph.SphError, match="cannot open the table file"):
         sph.load_aneos(str(tmp_path / "absent.txt"), prm, 8)
     bad = tmp_path / "bad.txt"
     bad.write_text("\n".join(HEADER + ["1 5"]) + "\n")

@@ -130,7 +130,9 @@ def test_precision_1e6_rms_and_drift(sph, orc):
               f"median {np.median(rel):.3e} max {rel.max():.3e}", flush=True)
         g.close()
     assert out["P3 eta 1e-3"] < 1e-3
-    assert de < 5e-2 and dl < 5e-3
+    # twice the measured drifts (|dE/E| 1.4e-2, |dL|/|L| 5.8e-5): a change
+    # that doubles either fails
+    assert de < 2.8e-2 and dl < 1.2e-4
 
 
 def test_headon_4e6_matches_oracle(sph, orc, tables256, impact):

@@ -120,4 +120,5 @@ def test_conservation_drift_with_rungs(sph, orc, tables256, impact):
     print(f"[drift] t = {ctx.time():.4f} s: |dE/E| {de:.3e}, |dL|/|L| {dl:.3e} "
           f"(E0 {e0:.6e}: K {c['kinetic']:.4e} U {c['internal']:.4e} W {c['potential']:.4e})", flush=True)
     assert np.isfinite(de) and np.isfinite(dl)
-    assert de < 5e-2 and dl < 5e-3
+    # twice the measured drifts (|dE/E| 1.6e-2, |dL|/|L| 4.0e-4)
+    assert de < 3.2e-2 and dl < 8e-4
