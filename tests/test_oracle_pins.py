@@ -355,8 +355,16 @@ def test_momentum_conserved_with_uniform_h(sph, orc, prm, tables256):
     assert np.linalg.norm(p) < 1e-5 * (m[:, None] * np.abs(a)).sum()
 
 
-def test_timestep_is_power_of_two_rung(small_density, prm):
-    st = small_density
+def test_timestep_is_power_of_two_rung(sph, orc, prm, tables256):
+    """The acceleration pass's timestep (accelerations.metal:130-138): every
+    particle's tick step is a power of two and the global step is their
+    minimum.  Runs on a state of its own so it does not depend on the order
+    in which the other tests of the module run."""
+    st, _ = _state(sph, orc, prm, tables256, 2000, 22)
+    st.sort()
+    st.density()
+    st["acc"] = 0.0
+    st.acceleration()
     dt = int(st["dt"][0])
     assert dt > 0 and (dt & (dt - 1)) == 0 and dt <= 65536
     steps = st["next_active"] - st["global_time"][0]
