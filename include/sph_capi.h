@@ -9,7 +9,8 @@
  *
  * Conventions
  *  - Return codes: 0 = OK, negative = SPH_E_* error, positive = hipError_t.
- *    sph_last_error() gives the message of the last failure on a context.
+ *    sph_last_error() gives the message of the last failure on a context
+ *    (with NULL: of the last failed context-free call on this thread).
  *  - Particle arrays are in original-ID order (the order of the initial
  *    conditions), as in the reference.  3-vectors use a 16-byte stride
  *    (x, y, z, unused), matching simd::float3 (Compute.cpp:95-97).
@@ -236,6 +237,29 @@ int sph_acceleration(sph_ctx* ctx);  /* Compute::accelerationPass   Compute.cpp:
 int sph_step(sph_ctx* ctx);          /* Compute::stepPass           Compute.cpp:242-263 */
 /* Headless::step's commit + waitUntilCompleted (Headless.cpp:53-54). */
 int sph_sync(sph_ctx* ctx);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the hash, radix sort and cell start/end of sph_sort,
+ * without a context, calling no hipMalloc / hipFree and no allocating
+ * library routine.  Inputs and outputs are the caller's device buffers:
+ *   pos4          float x 4n, positions with a 16-byte stride (w unused)
+ *   sorted_pairs  uint32 x 2n, out: (cell key, particle index) pairs in
+ *                 (key, index) order, as SPH_F_CELL_ARRAY
+ *   cell_se       uint32 x 2 x 2^(3 cell_power), out: per cell (first slot,
+ *                 last slot), (UINT_MAX, 0) for an empty cell, as
+ *                 SPH_F_CELL_START / SPH_F_CELL_END interleaved
+ * cell_width and cell_power are sph_params' (CELL_WIDTH, CELL_POWER).  The
+ * sort's working memory is a fixed device array of the library (about
+ * 34 MiB, held by each GPU it runs on until the process ends), which limits
+ * n to SPH_SORT_NOALLOC_MAX_N: larger n returns SPH_E_CAPACITY.  Work is
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream) on the
+ * current device and is not waited for.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (other streams or threads) race.
+ * Failures return a code; sph_last_error(NULL) gives the message. */
+#define SPH_SORT_NOALLOC_MAX_N 4194304
+int sph_sort_noalloc(const float* pos4, int32_t n, float cell_width, int32_t cell_power,
+                     uint32_t* sorted_pairs, uint32_t* cell_se, void* stream);
 
 /* ------------------------------------------------------------- state access */
 enum sph_field {

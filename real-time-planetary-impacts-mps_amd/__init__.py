@@ -122,7 +122,7 @@ CAPI_SYMBOLS = [
     "sph_samples_count", "sph_samples_read", "sph_samples_append", "sph_samples_clear", "sph_set_sample_dump",
     "sph_set_learned_guess", "sph_features", "sph_conservation", "sph_time",
     "sph_build_tree", "sph_tree_info", "sph_download_tree", "sph_draw_snapshot", "sph_download_snapshot",
-    "sph_snapshot_resolution", "sph_device_field",
+    "sph_snapshot_resolution", "sph_device_field", "sph_sort_noalloc",
 ]
 HOST_SYMBOLS = [
     "sph_tillotson_granite", "sph_tillotson_iron", "sph_make_eos_table", "sph_impact_config_default",
@@ -187,6 +187,8 @@ def libs():
         h.sph_download_snapshot.argtypes = [ctypes.c_void_p, I32]
         h.sph_snapshot_resolution.argtypes = [ctypes.c_void_p]
         h.sph_snapshot_positions.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        h.sph_sort_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_int32,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         h.sph_host_alloc.argtypes = [ctypes.c_size_t]
         h.sph_host_alloc.restype = ctypes.c_void_p
         h.sph_host_free.argtypes = [ctypes.c_void_p]
@@ -392,6 +394,22 @@ def octree_build(pos4: np.ndarray, max_leaf: int = 4, max_depth: int = 7, nthrea
 
 
 # ----------------------------------------------------------------- context
+SORT_NOALLOC_MAX_N = 4194304  # SPH_SORT_NOALLOC_MAX_N
+
+
+def sort_noalloc(pos4_ptr: int, n: int, cell_width: float, cell_power: int, pairs_ptr: int, cell_se_ptr: int,
+                 stream: int = 0) -> None:
+    """sph_sort_noalloc on device buffers the caller owns: positions (16-byte
+    stride), the sorted (key, index) pairs (2n uint32) and the interleaved
+    (first, last) cell table (2 x 2^(3 cell_power) uint32).  Enqueues on
+    `stream` (0 = the null stream) and does not wait."""
+    rc = libs()[0].sph_sort_noalloc(ctypes.c_void_p(pos4_ptr), int(n), float(cell_width), int(cell_power),
+                                    ctypes.c_void_p(pairs_ptr), ctypes.c_void_p(cell_se_ptr),
+                                    ctypes.c_void_p(stream or None))
+    if rc != 0:
+        raise SphError(f"sph_sort_noalloc failed ({rc}): {libs()[0].sph_last_error(None).decode()}")
+
+
 class Context:
     """One sph_ctx: device state of a simulation, driven pass by pass."""
 

@@ -344,7 +344,7 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
 
 void sph_destroy(sph_ctx* ctx) { destroy_ctx(ctx); }
 
-const char* sph_last_error(const sph_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* sph_last_error(const sph_ctx* ctx) { return ctx ? ctx->err.c_str() : noalloc_last_error(); }
 
 int sph_num_particles(const sph_ctx* ctx) { return ctx ? ctx->n : 0; }
 

@@ -28,6 +28,9 @@
  * 2^24 cells (128 MiB) every step, only the cells occupied after the previous
  * sort are reset -- the table content is the same.
  */
+#include <cstdio>
+#include <mutex>
+
 #include "sph_internal.h"
 
 namespace {
@@ -296,5 +299,112 @@ int launch_sort(sph_ctx* ctx) {
     if (cur == 1) std::swap(ctx->pairs_a, ctx->pairs_b);
     ctx->prev_keys_src = ctx->pairs_a;
     ctx->have_prev = true;
+    return SPH_OK;
+}
+
+/* ------------------------------------------------------------------------
+ * sph_sort_noalloc: the hash, radix sort and cell start/end of sph_sort
+ * without a context and without any allocation.  The sort's working memory
+ * (the second pair buffer of the ping-pong, the per-tile digit table and the
+ * digit totals) lives in `scratch`, a file-scope device array of fixed size
+ * that every GPU the library runs on holds until the process ends.  Host
+ * code never passes `scratch` itself to a kernel (on the host it is only a
+ * stand-in for the symbol): each device's address of it comes from
+ * hipGetSymbolAddress, once per device on its first call, and is kept in
+ * noalloc_scratch_ptr[].  Every call on a GPU uses that GPU's one array, so
+ * calls that overlap on one GPU (two streams, two threads) race.  Nothing in
+ * the array needs clearing: each call writes every entry it reads.
+ * ---------------------------------------------------------------------- */
+namespace {
+
+constexpr int64_t kNoAllocMaxN = SPH_SORT_NOALLOC_MAX_N;
+constexpr int64_t kNoAllocMaxTiles = (kNoAllocMaxN + kTile - 1) / kTile;
+constexpr size_t kNoAllocPairBytes = (size_t)kNoAllocMaxN * sizeof(uint2);
+constexpr size_t kNoAllocHistBytes = (size_t)256 * kNoAllocMaxTiles * sizeof(uint32_t);
+constexpr size_t kNoAllocBytes = kNoAllocPairBytes + kNoAllocHistBytes + 256 * sizeof(uint32_t);
+static_assert(kNoAllocPairBytes % 256 == 0 && kNoAllocHistBytes % 256 == 0, "scratch partitions stay 256-B aligned");
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mutex;
+void* noalloc_scratch_ptr[kMaxDevices] = {};
+
+thread_local char noalloc_err[256] = "null context";
+
+int noalloc_fail(int code, const char* fmt, long long a = 0, long long b = 0) {
+    snprintf(noalloc_err, sizeof(noalloc_err), fmt, a, b);
+    return code;
+}
+
+/* The run boundaries of the sorted keys give each occupied cell's first and
+ * last slot (sort.metal:140-148); the other cells keep the fill value. */
+__global__ void k_cells_bounds(const uint2* __restrict__ sorted, int n, uint2* __restrict__ cell_se) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = sorted[i].x;
+    if (i == 0 || sorted[i - 1].x != key) cell_se[key].x = (uint32_t)i;
+    if (i == n - 1 || sorted[i + 1].x != key) cell_se[key].y = (uint32_t)i;
+}
+
+}  // namespace
+
+const char* noalloc_last_error() { return noalloc_err; }
+
+extern "C" int sph_sort_noalloc(const float* pos4, int32_t n, float cell_width, int32_t cell_power,
+                                uint32_t* sorted_pairs, uint32_t* cell_se, void* stream) {
+    if (!pos4 || !sorted_pairs || !cell_se) return noalloc_fail(SPH_E_ARG, "sph_sort_noalloc: null pointer");
+    if (n < 1) return noalloc_fail(SPH_E_ARG, "sph_sort_noalloc: n = %lld, needs at least 1 particle", n);
+    if (n > kNoAllocMaxN)
+        return noalloc_fail(SPH_E_CAPACITY, "sph_sort_noalloc: n = %lld exceeds the scratch maximum of %lld particles",
+                            n, kNoAllocMaxN);
+    if (cell_power < 1 || cell_power > 10)
+        return noalloc_fail(SPH_E_ARG, "sph_sort_noalloc: cell_power = %lld, must be 1..10", cell_power);
+    if (!(cell_width > 0.f) || !(cell_width < 3.0e38f))
+        return noalloc_fail(SPH_E_ARG, "sph_sort_noalloc: cell_width must be positive and finite");
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail((int)e, "sph_sort_noalloc: hipGetDevice failed (%lld)", (long long)e);
+    if (dev < 0 || dev >= kMaxDevices)
+        return noalloc_fail(SPH_E_ARG, "sph_sort_noalloc: device %lld beyond the %lld supported", dev, kMaxDevices);
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(noalloc_mutex);
+        if (!noalloc_scratch_ptr[dev]) {
+            void* p = nullptr;
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return noalloc_fail((int)(e != hipSuccess ? e : hipErrorInvalidSymbol),
+                                    "sph_sort_noalloc: hipGetSymbolAddress(scratch) failed (%lld)", (long long)e);
+            noalloc_scratch_ptr[dev] = p;
+        }
+        base = noalloc_scratch_ptr[dev];
+    }
+    uint2* spare = reinterpret_cast<uint2*>(base);
+    uint32_t* tile_hist = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(base) + kNoAllocPairBytes);
+    uint32_t* digit_total = tile_hist + 256 * kNoAllocMaxTiles;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int ntiles = (n + kTile - 1) / kTile;
+    const int passes = (3 * cell_power + 7) / 8;
+    /* the hash writes into the buffer that makes the last pass land in the
+     * caller's array: an odd number of passes starts in the spare one */
+    uint2* out = reinterpret_cast<uint2*>(sorted_pairs);
+    uint2* bufs[2] = {(passes & 1) ? spare : out, (passes & 1) ? out : spare};
+    const int cells_per_dim = 1 << cell_power;
+    const int ncells = 1 << (3 * cell_power);
+    k_fill_cells<<<(ncells + 255) / 256, 256, 0, s>>>(reinterpret_cast<uint2*>(cell_se), ncells);
+    k_hash_hist<<<ntiles, kThreads, 0, s>>>(reinterpret_cast<const float4*>(pos4), n, cell_width, cells_per_dim,
+                                            bufs[0], tile_hist, ntiles);
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        if (p > 0) k_hist<<<ntiles, kThreads, 0, s>>>(bufs[cur], n, shift, tile_hist, ntiles);
+        k_scan_rows<<<256, kThreads, 0, s>>>(tile_hist, ntiles, digit_total);
+        k_scatter<<<ntiles, kThreads, 0, s>>>(bufs[cur], bufs[cur ^ 1], n, shift, tile_hist, ntiles, digit_total);
+        cur ^= 1;
+    }
+    k_cells_bounds<<<(n + 255) / 256, 256, 0, s>>>(bufs[cur], n, reinterpret_cast<uint2*>(cell_se));
+    e = hipGetLastError();
+    if (e != hipSuccess) return noalloc_fail((int)e, "sph_sort_noalloc: launch failed (%lld)", (long long)e);
     return SPH_OK;
 }

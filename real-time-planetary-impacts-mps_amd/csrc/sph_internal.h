@@ -234,6 +234,8 @@ int join_gravity(sph_ctx* ctx);
 int launch_sort_pairs(sph_ctx* ctx, uint2* a, uint2* b, int n, int passes, uint32_t* tile_hist, int tile_cap,
                       uint32_t* digit_total, uint2** out);
 int sort_tile_size(); /* pairs per tile of the radix sort */
+/* Message of the last failed context-free call on this thread. */
+const char* noalloc_last_error();
 /* device tree buffers for a tree of the given size: checks, grows, and picks
  * the upload slot; tree_commit makes that slot current and converts it */
 int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int n_levels, int32_t node_count,
