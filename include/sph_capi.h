@@ -132,8 +132,25 @@ typedef struct sph_params {
      * candidates in the same order as 0, so the same sums bit for bit);
      * 0 = one loop per cell. */
     int32_t density_flat;
-    int32_t reserved[1];
+    /* Stopping rule of the density pass's Newton-Raphson iteration
+     * (density.metal:73,131-136).  0 = the reference's: stop once successive
+     * iterates differ by at most density_eta.  1 (no counterpart in the
+     * reference) = stop once the estimated distance of h from the fixed point
+     * is at most density_eta: with eps the relative change of the last scan
+     * and k = eps / (previous scan's eps) the observed contraction, stop when
+     * eps / (1 - k) <= density_eta (two scans at least, never while k >=
+     * SPH_NR_MAX_CONTRACTION, and with one FP32 ulp, SPH_NR_EPS_ULP, added
+     * to eps and taken from the previous one), or when eps is at the FP32
+     * rounding floor (SPH_NR_NOISE_FLOOR).
+     * Two iterations started from different guesses then end within about
+     * 2 density_eta of each other; under rule 0 they can end density_eta /
+     * (1 - k) apart, 20x that for weakly contracting particles. */
+    int32_t density_stop;
 } sph_params;
+/* Relative change of h (2 ulp) at which density_stop = 1 stops whatever k. */
+#define SPH_NR_NOISE_FLOOR 2.5e-7f
+#define SPH_NR_EPS_ULP 1.2e-7f
+#define SPH_NR_MAX_CONTRACTION 0.995f
 
 /* One EOS table: res x res float2 (pressure, sound speed) in code units.
  * Row index = density sample, column index = internal-energy sample, sample i

@@ -1144,6 +1144,23 @@ void orc_eos_sample(const sph_params* prm, const float* table, int32_t res, floa
 }
 
 /* density.metal:12-239, one thread per sorted slot. */
+/* The stopping test of the iteration after a scan with relative change eps
+ * (eps_prev: the previous scan's, 3e38 before the second).  Rule 0:
+ * density.metal:73,131-136, successive iterates within eta.  Rule 1 (not in
+ * the reference; sph_params.density_stop): the estimated distance from the
+ * fixed point, eps / (1 - k) with k = eps / eps_prev (one ulp of slack on
+ * each), within eta, or eps at the rounding floor; never stops on a first
+ * scan or while k >= SPH_NR_MAX_CONTRACTION. */
+static bool nr_goes_on(int rule, float eps, float eps_prev, float eta) {
+    if (rule == 0) return eps > eta;
+    if (eps <= SPH_NR_NOISE_FLOOR) return false;
+    if (eps_prev > 1.0e38f) return true;
+    const float e = eps + SPH_NR_EPS_ULP; /* one ulp of slack on each eps */
+    const float k = e / (eps_prev - SPH_NR_EPS_ULP);
+    if (k >= SPH_NR_MAX_CONTRACTION) return true;
+    return e > eta * (1.f - k);
+}
+
 void orc_density(const sph_params* prm, const orc_state* s) {
     const int n = s->n;
     const float cellSize = prm->cell_width;
@@ -1176,7 +1193,9 @@ void orc_density(const sph_params* prm, const orc_state* s) {
             float eps = 1;
             int count = 0;
             int nNeighbours = 0;
-            while (eps > eta && count < maxIt) {
+            float eps_prev = 3.0e38f;
+            bool goOn = prm->density_stop ? true : eps > eta;
+            while (goOn && count < maxIt) {
                 density = 0;
                 if (h_i < max_h && nNeighbours < NE) {
                     for (int i = 0; i < nNeighbours; i++) {
@@ -1215,7 +1234,9 @@ void orc_density(const sph_params* prm, const orc_state* s) {
                 float newH = std::min(prm->resolution_eta * powf(mass_i / density, 1.f / 3), prm->max_smoothing_length);
                 newH = 0.5f * (h_i + newH);
                 eps = fabsf(newH - h_i) / h_i;
-                if (eps > eta) {
+                goOn = nr_goes_on(prm->density_stop, eps, eps_prev, eta);
+                eps_prev = eps;
+                if (goOn) {
                     max_h = (int)std::fmax((float)max_h, h_i);
                     h_i = newH;
                     h1 = 1 / h_i;

@@ -64,7 +64,7 @@ class SphParams(ctypes.Structure):
         ("fixes", ctypes.c_uint32), ("precision", ctypes.c_int32), ("density_lanes", ctypes.c_int32),
         ("accel_lanes", ctypes.c_int32), ("lane_split", ctypes.c_int32),
         ("expansion_order", ctypes.c_int32), ("mac_order", ctypes.c_int32), ("density_flat", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("density_stop", ctypes.c_int32),
     ]
 
 

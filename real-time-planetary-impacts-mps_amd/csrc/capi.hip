@@ -34,6 +34,7 @@ bool valid_params(const sph_params* p, std::string& why) {
         p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }
     if (p->density_flat != 0 && p->density_flat != 1) { why = "density_flat must be 0 or 1"; return false; }
+    if (p->density_stop != 0 && p->density_stop != 1) { why = "density_stop must be 0 or 1"; return false; }
     if (p->expansion_order != 2 && p->expansion_order != 3) { why = "expansion_order must be 2 or 3"; return false; }
     if (p->mac_order != 0 && p->mac_order != 2) { why = "mac_order must be 0 or 2"; return false; }
     if (p->expansion_order == 3 && p->gravity_max_recursion != 1 && !(p->fixes & SPH_FIX_NEAR_FIELD)) {

@@ -67,6 +67,14 @@ __device__ __forceinline__ float kW(float r, float h1) {
     }
     return fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
 }
+/* kW with its normalisation fac = h^-3 / pi given, as selects instead of
+ * branches: the same operations in the same order, so the same value. */
+__device__ __forceinline__ float kW_sel(float q, float fac) {
+    const float tmp = 2.f - q;
+    const float outer = fac * 0.25f * tmp * tmp * tmp;
+    const float inner = fac * (1.f - 1.5f * q * q * (1.f - 0.5f * q));
+    return q > 2.0f ? 0.f : (q > 1.0f ? outer : inner);
+}
 /* kernels.h:52-69: scalar factor g with gradW = g * x_ij / r */
 __device__ __forceinline__ float kGradWFactor(float r, float h1) {
     const float q = r * h1;
@@ -134,7 +142,25 @@ struct DensityArgs {
     /* README rule: the inactive particles' EOS runs in k_inactive_eos (in
      * original order, coalesced) and their threads here return at once */
     int skip_inactive;
+    int stop_rule; /* sph_params.density_stop */
 };
+
+/* Whether the Newton-Raphson iteration goes on after a scan whose relative
+ * change of h is eps; eps_prev is the previous scan's (3e38 before the
+ * second scan).  Rule 0 is the reference's successive-iterate test
+ * (density.metal:73,131-136).  Rule 1 goes on until the estimated distance
+ * of h from the fixed point, eps / (1 - k) with the observed contraction
+ * k = eps / eps_prev, is within eta, or eps is at the rounding floor. */
+__device__ __forceinline__ bool nr_goes_on(int rule, float eps, float eps_prev, float eta) {
+    if (rule == 0) return eps > eta;
+    if (eps <= SPH_NR_NOISE_FLOOR) return false;
+    if (eps_prev > 1.0e38f) return true;
+    /* each eps is a difference of FP32 values: allow one ulp either way */
+    const float e = eps + SPH_NR_EPS_ULP;
+    const float k = e / (eps_prev - SPH_NR_EPS_ULP);
+    if (k >= SPH_NR_MAX_CONTRACTION) return true;
+    return e > eta * (1.f - k);
+}
 
 constexpr float kGuessTrust = 0.05f;
 /* Successive changes below this are left to the plain iteration's stopping
@@ -204,7 +230,9 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     float h_scan = h_i;     /* h of the last scan */
     float hs2x4 = h2x4;     /* its 4 h^2 */
     int w_cand = 0, w_in = 0, f_cand = 0, f_in = 0;
-    while (eps > a.eta && count < a.max_it) {
+    float eps_prev = 3.0e38f;
+    bool go_on = a.stop_rule ? true : eps > a.eta;
+    while (go_on && count < a.max_it) {
         density = 0.f;
         const bool use_cache = (h_i < (float)max_h) && (nN < a.ne);
         const float hb = use_cache ? h_scan : h_i;
@@ -253,10 +281,14 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
             h2x4 = 4.f * h_i * h_i;
             max_h = 0;
             eps = 1.f;
+            eps_prev = 3.0e38f;
+            go_on = a.stop_rule ? true : eps > a.eta;
             count++;
             continue;
         }
-        if (eps > a.eta) {
+        go_on = nr_goes_on(a.stop_rule, eps, eps_prev, a.eta);
+        eps_prev = eps;
+        if (go_on) {
             max_h = (int)fmaxf((float)max_h, h_i);
             h_i = newH;
             h1 = 1.f / h_i;
@@ -510,7 +542,9 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     float h_scan = h_i;
     float hs2x4 = h2x4;
     int w_cand = 0, w_in = 0, f_cand = 0, f_in = 0;
-    while (eps > a.eta && count < a.max_it) {
+    float eps_prev = 3.0e38f;
+    bool go_on = a.stop_rule ? true : eps > a.eta;
+    while (go_on && count < a.max_it) {
         const bool use_cache = (h_i < (float)max_h) && (nN < a.ne);
         const float hb = use_cache ? h_scan : h_i;
         const float hb2 = use_cache ? hs2x4 : h2x4;
@@ -519,20 +553,23 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
         cell_box(position, cs, hb, mask + 1, mn, mx);
         float part = 0.f;
         int cnt = 0;
+        /* the candidate body without branches: a wave of 64 lanes almost
+         * always holds a lane whose candidate is inside 2h, so the kernel
+         * term costs the same either way, and selects leave out the exec-mask
+         * bookkeeping of the nested tests (same terms, same order) */
+        const float fac = kPi1 * h1 * h1 * h1;
         box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, [&](uint32_t nc) { w_cand += (int)nc; },
                                   [&](uint32_t k) {
             const float4 q = spm[k];
             const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 > hb2) return;
-            w_in++;
+            const bool in = !(r2 > hb2);
             const float r = __builtin_amdgcn_sqrtf(r2);
-            if (use_cache) {
-                if (r > hsum) return;
-            } else {
-                cnt++;
-            }
-            part += kW(r, h1) * q.w;
+            const bool take = in && !(use_cache && r > hsum);
+            w_in += in ? 1 : 0;
+            cnt += (in && !use_cache) ? 1 : 0;
+            const float w = kW_sel(r * h1, fac);
+            part += take ? w * q.w : 0.f;
         });
         density = gsum<G>(part);
         if (!use_cache) {
@@ -549,10 +586,14 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
             h2x4 = 4.f * h_i * h_i;
             max_h = 0;
             eps = 1.f;
+            eps_prev = 3.0e38f;
+            go_on = a.stop_rule ? true : eps > a.eta;
             count++;
             continue;
         }
-        if (eps > a.eta) {
+        go_on = nr_goes_on(a.stop_rule, eps, eps_prev, a.eta);
+        eps_prev = eps;
+        if (go_on) {
             max_h = (int)fmaxf((float)max_h, h_i);
             h_i = newH;
             h1 = 1.f / h_i;
@@ -1082,6 +1123,7 @@ int launch_density(sph_ctx* ctx) {
      * list of the active slots would leave too few waves to hide the
      * neighbour loops' latency.) */
     a.skip_inactive = p.activity_rule == SPH_ACTIVE_README ? 1 : 0;
+    a.stop_rule = p.density_stop;
     if (a.skip_inactive) {
         k_inactive_eos<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(a);
         SPH_HIP_TRY(hipGetLastError());

@@ -1,13 +1,22 @@
 """Learned smoothing-length guess on the GPU: BF16 MFMA forward and backward
 against the FP64 restatement (oracle/mlp.py), features against their FP64
 restatement, the sample dump, training, and what the trained network gains
-over plain Newton-Raphson at a tightened tolerance (|dh/h| <= 1e-5)."""
+over plain Newton-Raphson at identical converged h (|dh/h| <= 1e-5).
+
+The comparisons stop the iteration with density_stop = 1 at density_eta =
+2.5e-6: each run ends within 2.5e-6 of the fixed point of the update, so
+the plain and the learned run end within 1e-5 of each other for every
+particle.  The reference's successive-iterate rule cannot give that: it
+stops density_eta / (1 - k) from the fixed point, and weakly contracting
+particles (k near 1) end far apart when started from different guesses."""
 import numpy as np
 import pytest
 
 from oracle import mlp as omlp
 
 pytestmark = pytest.mark.gpu
+
+TIGHT = dict(density_eta=2.5e-6, density_stop=1, max_nr_iterations=400)
 
 
 def _random_norm(rng):
@@ -128,10 +137,10 @@ def _density_ab(ctx, mlp):
 
 
 def test_training_and_learned_guess_gain(sph, orc, tables256, impact):
-    """Dump (features, converged h) from a plain run at |dh/h| <= 1e-5, train
-    on the GPU, then start the iteration from the network's guess: the same
-    converged h, fewer iterations."""
-    prm = sph.default_params(density_eta=1e-5)
+    """Dump (features, converged h) from a plain run at the tight stopping
+    rule, train on the GPU, then start the iteration from the network's
+    guess: the same converged h (|dh/h| <= 1e-5), fewer iterations."""
+    prm = sph.default_params(**TIGHT)
     n = 100000
     ctx = sph.Context(prm, impact(n, 11), tables256)
     _run_steps(ctx, orc, prm, 1)
@@ -154,9 +163,7 @@ def test_training_and_learned_guess_gain(sph, orc, tables256, impact):
     dh = np.abs(learned["h"].astype(np.float64) / plain["h"] - 1)
     print(f"[learned] NR iterations plain {plain['it'].mean():.3f} learned {learned['it'].mean():.3f}; "
           f"|dh/h| max {dh.max():.2e} q99.9 {np.quantile(dh, 0.999):.2e}", flush=True)
-    # both stop at |dh/h| <= 1e-5 between iterates; approached from
-    # different sides the two fixed-point estimates differ by a few 1e-5
-    assert np.quantile(dh, 0.999) < 2e-4 and dh.max() < 1e-3
+    assert dh.max() <= 1e-5
     assert learned["it"].mean() < 0.85 * plain["it"].mean()
 
 
@@ -164,7 +171,7 @@ def test_bad_guess_still_converges(sph, orc, tables256, impact):
     """A network that always predicts the clamp (h x e^0.5) costs at most one
     extra iteration and ends at the plain iteration's h: its first residual
     is far above the trust bound, so the iteration restarts from h."""
-    prm = sph.default_params(density_eta=1e-5)
+    prm = sph.default_params(**TIGHT)
     ctx = sph.Context(prm, impact(20000, 12), tables256)
     _run_steps(ctx, orc, prm, 1)
     net = sph.Mlp(seed=1)
@@ -179,7 +186,7 @@ def test_bad_guess_still_converges(sph, orc, tables256, impact):
     dh = np.abs(learned["h"].astype(np.float64) / plain["h"] - 1)
     print(f"[bad guess] iterations plain {plain['it'].mean():.2f} learned {learned['it'].mean():.2f} "
           f"|dh/h| max {dh.max():.2e}", flush=True)
-    assert np.quantile(dh, 0.999) < 2e-4
+    assert dh.max() <= 1e-5
     assert learned["it"].mean() <= plain["it"].mean() + 1.0 + 1e-9
 
 
@@ -187,8 +194,8 @@ def test_learned_guess_with_individual_timesteps(sph, orc, tables256, impact):
     """README rule (up to 8 rungs): only the active particles' features and
     guesses are computed (compacted sorted slots).  Active particles converge
     to the plain iteration's h; inactive ones keep theirs in both runs."""
-    prm = sph.default_params(density_eta=1e-5, activity_rule=sph.SPH_ACTIVE_README, max_rungs=8,
-                             fixes=sph.SPH_FIX_INACTIVE_DRIFT)
+    prm = sph.default_params(activity_rule=sph.SPH_ACTIVE_README, max_rungs=8, fixes=sph.SPH_FIX_INACTIVE_DRIFT,
+                             **TIGHT)
     ctx = sph.Context(prm, impact(20000, 13), tables256)
     _run_steps(ctx, orc, prm, 3)
     net = sph.Mlp(seed=3)
@@ -203,4 +210,59 @@ def test_learned_guess_with_individual_timesteps(sph, orc, tables256, impact):
     print(f"[learned README] active {act.mean():.4f}; |dh/h| active max {dh[act].max():.2e}", flush=True)
     assert 0 < act.sum() < len(act)
     assert np.array_equal(learned["h"][~act], h0[~act]) and np.array_equal(plain["h"][~act], h0[~act])
-    assert np.quantile(dh[act], 0.999) < 2e-4 and dh[act].max() < 1e-3
+    assert dh[act].max() <= 1e-5
+
+
+def test_learned_guess_1e6_c4_recipe(sph):
+    """The learned-h form at 1e6 particles: (features, converged h) dumped by
+    8 plain steps each of seeded 1e5 (seed 11) and 1e6 (seed 12) impacts,
+    2000 Adam steps at batch 65536 on the GPU, then one density pass of a
+    third 1e6 impact (seed 3, after 3 steps) run plain and from the network's
+    guess: every particle's converged h the same to 1e-5, fewer iterations,
+    a shorter pass."""
+    prm = sph.default_params(**TIGHT)
+    smp = sph.Samples(8 * 1_100_000)
+    for n, seed in ((100_000, 11), (1_000_000, 12)):
+        hl = sph.Headless(prm, sph.impact_config(n, seed), eos_res=256, tree_mode=sph.TREE_EVERY, tree_threads=8)
+        hl.step()
+        hl.ctx.set_sample_dump(smp)
+        for _ in range(8):
+            hl.step()
+        hl.ctx.set_sample_dump(None)
+        hl.close()
+    assert len(smp) == 8 * 1_100_000
+    net = sph.Mlp(seed=13)
+    net.fit_norm(smp)
+    losses = net.train(smp, steps=2000, batch=65536, lr=2e-3, seed=14)
+    smp.close()
+    print(f"[C4] loss first {losses[0]:.4f} last10 {np.mean(losses[-10:]):.5f}", flush=True)
+    assert np.all(np.isfinite(losses)) and np.mean(losses[-10:]) < 0.05 * losses[0]
+    hl = sph.Headless(prm, sph.impact_config(1_000_000, 3), eos_res=256, tree_mode=sph.TREE_EVERY, tree_threads=8)
+    for _ in range(3):
+        hl.step()
+    ctx = hl.ctx
+    ctx.sort()
+    ctx.sync()
+    keep = {k: ctx.download(k) for k in ("h", "rho", "global_time", "dt")}
+    out = {}
+    for mode in ("plain", "learned"):
+        ctx.set_learned_guess(net if mode == "learned" else None)
+        for k, v in keep.items():
+            ctx.upload(k, v)
+        ctx.reset_pass_times()
+        ctx.set_timing(True)
+        ctx.density()
+        t = ctx.pass_times()["density"]
+        ctx.set_timing(False)
+        out[mode] = {"ms": t["ms"], "h": ctx.download("h").astype(np.float64), "it": ctx.download("nr_iters")}
+    ctx.set_learned_guess(None)
+    hl.close()
+    net.close()
+    dh = np.abs(out["learned"]["h"] / out["plain"]["h"] - 1)
+    print(f"[C4] NR iterations plain {out['plain']['it'].mean():.3f} learned {out['learned']['it'].mean():.3f}; "
+          f"density ms plain {out['plain']['ms']:.3f} learned {out['learned']['ms']:.3f}; "
+          f"|dh/h| max {dh.max():.2e} q99.9 {np.quantile(dh, 0.999):.2e}", flush=True)
+    assert out["plain"]["it"].max() < 400 and out["learned"]["it"].max() < 400
+    assert dh.max() <= 1e-5
+    assert out["learned"]["it"].mean() < 0.8 * out["plain"]["it"].mean()
+    assert out["learned"]["ms"] < out["plain"]["ms"]

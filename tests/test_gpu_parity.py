@@ -164,6 +164,33 @@ def test_density_lanes_parity(sph, orc, tables256, impact, lanes, split):
     assert s["balsara_abs"][0] < 1e-4
 
 
+@pytest.mark.parametrize("lanes", [0, 8])
+def test_density_stop_rule_parity(sph, orc, tables256, impact, lanes):
+    """density_stop = 1 (stop once the estimated distance from the fixed
+    point is within density_eta) against the oracle's same rule.  The two
+    sides sum in different orders, so their iteration counts may differ, but
+    both end within density_eta of the same fixed point: every particle is
+    compared, with no flips set aside."""
+    n = 100000
+    eta = 2.5e-6
+    ic = _stir(impact(n, 2), 2)
+    prm, ctx, st = _pair(sph, orc, ic, tables256, density_eta=eta, density_stop=1, max_nr_iterations=400,
+                         density_lanes=lanes)
+    ctx.sort()
+    st.sort()
+    ctx.density()
+    st.density()
+    nr_g, nr_c = ctx.download("nr_iters"), st["nr_iters"]
+    dh = _rel(ctx.download("h"), st["h"])
+    drho = _rel(ctx.download("rho"), st["rho"])
+    print(f"[density stop rule lanes={lanes}] iterations GPU {nr_g.mean():.3f} oracle {nr_c.mean():.3f}, "
+          f"counts differ {int((nr_g != nr_c).sum())}; |dh/h| max {dh.max():.2e}, |drho/rho| max {drho.max():.2e}",
+          flush=True)
+    assert nr_g.max() < 400 and nr_c.max() < 400
+    assert dh.max() <= 4 * eta and drho.max() <= 3e-5
+    assert abs(nr_g.mean() - nr_c.mean()) < 0.05 * nr_c.mean()
+
+
 def test_density_after_position_upload(sph, orc, tables256, impact):
     """Positions and velocities replaced after the sort: the density pass
     reads the new ones (the reference's density pass reads the position

@@ -61,7 +61,7 @@ def test_default_params_are_the_reference_values(sph):
 
 def test_params_struct_layout_matches_c(sph):
     # 26 four-byte fields, precision, density_lanes, accel_lanes, lane_split, expansion_order, mac_order,
-    # 2 reserved ints
+    # density_flat, density_stop
     assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 8)
 
 

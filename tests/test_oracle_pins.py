@@ -237,6 +237,63 @@ def test_density_fixed_point_invariant(small_density, prm):
     assert st["nr_iters"].max() > 0
 
 
+def _fp64_fixed_point(pos, m, h0, prm, iters=400):
+    """The fixed point of the damped update h <- (h + min(RESOLUTION_ETA
+    (m / rho(h))^(1/3), MAX_SMOOTHING_LENGTH)) / 2 (density.metal:131-133),
+    rho(h) the FP64 kernel sum over every particle within 2h, iterated from
+    h0 for all particles at once (pairs from a k-d tree within 2.2 max h0)."""
+    from scipy.spatial import cKDTree
+    pairs = cKDTree(pos).query_pairs(2.2 * h0.max(), output_type="ndarray")
+    i = np.concatenate([pairs[:, 0], pairs[:, 1], np.arange(len(m))])
+    j = np.concatenate([pairs[:, 1], pairs[:, 0], np.arange(len(m))])
+    r = np.linalg.norm(pos[i] - pos[j], axis=1)
+    h = h0.copy()
+    for _ in range(iters):
+        q = r / h[i]
+        w = np.where(q < 1, 1 - 1.5 * q ** 2 + 0.75 * q ** 3, np.where(q < 2, 0.25 * (2 - q) ** 3, 0.0))
+        rho = np.bincount(i, w * m[j], minlength=len(m)) / (math.pi * h ** 3)
+        new = 0.5 * (h + np.minimum(prm.resolution_eta * (m / rho) ** (1 / 3), prm.max_smoothing_length))
+        done = np.abs(new / h - 1).max() < 1e-14
+        h = new
+        if done:
+            break
+    assert h.max() < 1.1 * h0.max()
+    return h
+
+
+def test_density_stop_rule_bounds_distance_to_fixed_point(sph, orc, tables256):
+    """density_stop = 1 stops once eps / (1 - k) <= eta, k the observed
+    contraction.  Every particle's h then lies within eta of where the same
+    FP32 iteration ends when it never stops early (eta = 0, 400 iterations),
+    which the reference's successive-iterate rule at the same eta does not
+    give: it can stop eta / (1 - k) away.  The end point itself is the FP64
+    fixed point of a brute-force density sum, up to the FP32 rounding of the
+    update amplified by 1 / (1 - k)."""
+    eta = 2.5e-6
+    h = {}
+    for name, rule, e in (("rule0", 0, eta), ("rule1", 1, eta), ("limit", 0, 0.0)):
+        prm = sph.default_params(density_eta=e, density_stop=rule, max_nr_iterations=400)
+        st, _ = _state(sph, orc, prm, tables256, 4000, 26)
+        st.sort()
+        st.density()
+        h[name] = st["h"].astype(np.float64)
+        if name != "limit":
+            assert st["nr_iters"].max() < 400, name
+    d0 = np.abs(h["rule0"] / h["limit"] - 1)
+    d1 = np.abs(h["rule1"] / h["limit"] - 1)
+    pos = st["pos"][:, :3].astype(np.float64)
+    hstar = _fp64_fixed_point(pos, st["mass"].astype(np.float64), h["limit"], prm)
+    free = h["limit"] < prm.max_smoothing_length  # the others sit at the clamp in every run
+    d64 = np.abs(h["limit"] / hstar - 1)[free]
+    print(f"distance to the iteration's end: rule 0 max {d0.max():.2e}, rule 1 max {d1.max():.2e}; "
+          f"FP32 end vs FP64 fixed point ({free.sum()} unclamped) max {d64.max():.2e} median {np.median(d64):.2e}")
+    # the iterated end point may still move by an ulp or two (2.4e-7)
+    assert d1.max() <= eta + 2.4e-7
+    assert d0.max() > 2 * eta
+    assert free.sum() > len(free) // 2
+    assert np.median(d64) < 2e-7 and d64.max() < 2e-5
+
+
 def test_gradient_term_and_balsara_by_brute_force(small_density):
     st = small_density
     pos = st["pos"][:, :3].astype(np.float64)
