@@ -349,7 +349,10 @@ def learned_precision_run(sph, args):
     }
     for prm in forms.values():
         apply_overrides(prm, args)
-    net, train = train_guess(sph, args, forms["plain_fp32"], ((100_000, 11), (1_000_000, 12)), True)
+    # dumps with every particle active and the iteration stopped within
+    # 2.5e-6 of its fixed point: the targets are the fixed points
+    dump_prm = apply_overrides(sph.default_params(density_eta=2.5e-6, density_stop=1, max_nr_iterations=400), args)
+    net, train = train_guess(sph, args, dump_prm, ((100_000, 11), (1_000_000, 12)), True)
     res = {}
     for name, prm in forms.items():
         hl = sph.Headless(prm, sph.impact_config(args.n, 5, headon=True), eos_res=256, tree_mode=sph.TREE_ASYNC,
@@ -512,10 +515,18 @@ def main():
                                "that overlap, so the passes do not add up to ms_per_step")
         out["density_ms"] = round(per_pass["density"], 4)
         # dominant kernel: the density pass is one kernel
-        kname = (f"k_density_group<{int(prm.density_lanes)}, {int(prm.lane_split)}>" if prm.density_lanes > 0
-                 else "k_density")
-        work = ctx.download("density_work")
+        kname = (f"k_density_group<{int(prm.density_lanes)}, {int(prm.lane_split)}, {int(prm.density_flat)}>"
+                 if prm.density_lanes > 0 else "k_density")
+        # the reference algorithm's work, counted by one more sort + density
+        # pass of the state after the timed steps (untimed: counting is a
+        # measurement aid)
         nr = ctx.download("nr_iters")
+        ctx.set_work_counters(True)
+        ctx.sort()
+        ctx.density()
+        ctx.sync()
+        ctx.set_work_counters(False)
+        work = ctx.download("density_work")
         out["nr_iterations_mean"] = round(float(nr.mean()), 4)
         out["density_work_per_particle"] = {
             "nr_scan_candidates": round(float(work[:, 0].mean()), 1),
@@ -526,7 +537,7 @@ def main():
         flops = density_flops(work)
         # the recorded counters cover the default workload and launch shape only
         default_shape = (args.n == 1_000_000 and args.fixes == 0 and not args.param and not args.headon
-                         and prm.density_lanes == 8 and prm.lane_split == 0)
+                         and prm.density_lanes == 8 and prm.lane_split == 0 and prm.density_flat == 0)
         traffic_bytes = profiled_traffic() if default_shape else None
         t = per_pass["density"] / 1000.0
         achieved = flops / t / 1e12

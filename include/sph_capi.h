@@ -350,6 +350,12 @@ int sph_set_timing(sph_ctx* ctx, int enabled);
  * `ms` has SPH_PASS_COUNT entries; `calls` (optional) counts the passes. */
 int sph_pass_times(sph_ctx* ctx, double* ms, int64_t* calls);
 int sph_reset_pass_times(sph_ctx* ctx);
+/* Measurement aid (no counterpart in the reference): while enabled, density
+ * passes record per particle the reference algorithm's work in
+ * SPH_F_DENSITY_WORK (candidates of the cell boxes and those inside 2h, Newton-
+ * Raphson scans and final pass).  Off by default: SPH_F_DENSITY_WORK then
+ * keeps its last content. */
+int sph_set_work_counters(sph_ctx* ctx, int enabled);
 
 /* ------------------------------------------------------------ diagnostics */
 /* FP64 sums over all particles, reduced on the device (no counterpart in

@@ -122,7 +122,7 @@ CAPI_SYMBOLS = [
     "sph_samples_count", "sph_samples_read", "sph_samples_append", "sph_samples_clear", "sph_set_sample_dump",
     "sph_set_learned_guess", "sph_features", "sph_conservation", "sph_time",
     "sph_build_tree", "sph_tree_info", "sph_download_tree", "sph_draw_snapshot", "sph_download_snapshot",
-    "sph_snapshot_resolution", "sph_device_field", "sph_sort_noalloc",
+    "sph_snapshot_resolution", "sph_device_field", "sph_sort_noalloc", "sph_set_work_counters",
 ]
 HOST_SYMBOLS = [
     "sph_tillotson_granite", "sph_tillotson_iron", "sph_make_eos_table", "sph_impact_config_default",
@@ -170,6 +170,7 @@ def libs():
                      "sph_wait_snapshot", "sph_tree_upload_done", "sph_reset_pass_times", "sph_build_tree"):
             getattr(h, name).argtypes = [ctypes.c_void_p]
         h.sph_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        h.sph_set_work_counters.argtypes = [ctypes.c_void_p, ctypes.c_int]
         h.sph_pass_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
         h.sph_field_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int]
         h.sph_field_bytes.restype = ctypes.c_int64
@@ -560,6 +561,11 @@ class Context:
         calls = (ctypes.c_int64 * len(PASSES))()
         self._check(self._h.sph_pass_times(self.ptr, ms, calls), "sph_pass_times")
         return {p: {"ms": ms[i], "calls": calls[i]} for i, p in enumerate(PASSES)}
+
+    def set_work_counters(self, on: bool = True):
+        """Records the reference algorithm's density work per particle
+        (field "density_work") in the following density passes."""
+        self._check(self._h.sph_set_work_counters(self.ptr, 1 if on else 0), "sph_set_work_counters")
 
     def reset_pass_times(self):
         self._check(self._h.sph_reset_pass_times(self.ptr), "sph_reset_pass_times")

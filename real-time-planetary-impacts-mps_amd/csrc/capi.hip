@@ -752,6 +752,12 @@ int sph_pass_times(sph_ctx* ctx, double* ms, int64_t* calls) {
     return SPH_OK;
 }
 
+int sph_set_work_counters(sph_ctx* ctx, int enabled) {
+    if (!ctx) return SPH_E_ARG;
+    ctx->count_work = enabled != 0;
+    return SPH_OK;
+}
+
 int sph_reset_pass_times(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
     int rc = ctx_fold_times(ctx);

@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
         a.P[ind] = pc.x;
         a.c[ind] = pc.y;
         a.nr_iters[ind] = 0;
-        a.work[ind] = make_int4(0, 0, 0, 0);
+        if (a.work) a.work[ind] = make_int4(0, 0, 0, 0);
         return;
     }
     const float4 pm = a.s_posm[index];
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_density(DensityArgs a) {
     a.P[ind] = pc.x;
     a.c[ind] = pc.y;
     a.B[ind] = bal;
-    a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
+    if (a.work) a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
 }
 
 /* Group sums over G lanes in a fixed butterfly order (reproducible). */
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
             a.P[ind] = pc.x;
             a.c[ind] = pc.y;
             a.nr_iters[ind] = 0;
-            a.work[ind] = make_int4(0, 0, 0, 0);
+            if (a.work) a.work[ind] = make_int4(0, 0, 0, 0);
         }
         return;
     }
@@ -662,7 +662,7 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     a.P[ind] = pc.x;
     a.c[ind] = pc.y;
     a.B[ind] = bal;
-    a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
+    if (a.work) a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
 }
 
 /* density.metal:40-55 for the inactive particles, in original order
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(256) void k_inactive_eos(DensityArgs a) {
     a.P[i] = pc.x;
     a.c[i] = pc.y;
     a.nr_iters[i] = 0;
-    a.work[i] = make_int4(0, 0, 0, 0);
+    if (a.work) a.work[i] = make_int4(0, 0, 0, 0);
 }
 
 /* Sorted slots of the active particles (for the learned guess's features
@@ -1100,7 +1100,7 @@ int launch_density(sph_ctx* ctx) {
     a.f = ctx->f;
     a.B = ctx->B;
     a.nr_iters = ctx->nr_iters;
-    a.work = ctx->density_work;
+    a.work = ctx->count_work ? ctx->density_work : nullptr;
     a.time = ctx->time;
     a.eos0 = ctx->eos[0];
     a.eos1 = ctx->eos[1];

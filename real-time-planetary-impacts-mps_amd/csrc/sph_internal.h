@@ -154,6 +154,9 @@ struct sph_ctx {
      * particles' positions, masses and velocities (set by a sort, cleared by a
      * step or an upload of those fields) */
     bool morton_fresh = false;
+    /* the density pass fills density_work (and runs without its neighbour
+     * lists) only while this is set: sph_set_work_counters */
+    bool count_work = false;
     float4 *s_posm = nullptr, *s_vel = nullptr;
     float4 *s_posh = nullptr, *s_velb = nullptr, *s_hyd = nullptr;
 

@@ -161,3 +161,115 @@ def test_headon_4e6_matches_oracle(sph, orc, tables256, impact):
         assert int(ctx.download("global_time")[0]) == int(st["global_time"][0])
     assert d < 1e-4
     ctx.close()
+
+
+def _train_headon_guess(sph):
+    """The learned-h network as the head-on run uses it: (features, converged
+    h) dumped by 8 plain steps each of seeded 1e5 (seed 11) and 1e6 (seed 12)
+    head-on impacts, every particle active and the iteration stopped within
+    2.5e-6 of its fixed point (so the targets are the fixed points), 2000
+    Adam steps at batch 65536."""
+    prm = sph.default_params(density_eta=2.5e-6, density_stop=1, max_nr_iterations=400)
+    smp = sph.Samples(8 * 1_100_000)
+    for n, seed in ((100_000, 11), (1_000_000, 12)):
+        hl = sph.Headless(prm, sph.impact_config(n, seed, headon=True), eos_res=256, tree_mode=sph.TREE_EVERY,
+                          tree_threads=8)
+        hl.step()
+        hl.ctx.set_sample_dump(smp)
+        for _ in range(8):
+            hl.step()
+        hl.ctx.set_sample_dump(None)
+        hl.close()
+    net = sph.Mlp(seed=13)
+    net.fit_norm(smp)
+    losses = net.train(smp, steps=2000, batch=65536, lr=2e-3, seed=14)
+    smp.close()
+    assert np.all(np.isfinite(losses)) and np.mean(losses[-10:]) < 0.05 * losses[0]
+    return net
+
+
+def test_headon_4e6_learned_precision(sph, orc):
+    """Config C5's learned-h + precision form at its full size: the 4e6
+    head-on impact (impact parameter 0.3, 12 km/s, u0 5e6 J/kg), README
+    activity rule with up to 8 rungs and the leaf/branch activation syncing,
+    every correction, compensated sums, FP64 time and conservation sums,
+    third-order gravity at GRAVITY_ETA 1e-3, and the trained network as the
+    Newton-Raphson guess, for 10 steps (octree rebuilt before every step).
+    Checked through properties: a finite state; gravity RMS against an FP64
+    direct sum at 8192 particles; energy and angular-momentum drift; and one
+    density pass of the final state run plain and from the network's guess
+    under the comparison's stopping rule: the same converged h to 1e-5."""
+    import time
+    n = 4_000_000
+    base = dict(activity_rule=sph.SPH_ACTIVE_README, max_rungs=8)
+    net = _train_headon_guess(sph)
+    form = dict(fixes=sph.SPH_FIX_ALL, precision=1, expansion_order=3, mac_order=2, gravity_eta=1e-3,
+                max_unchecked_pointers=8192, **base)
+    prm = sph.default_params(**form)
+    hl = sph.Headless(prm, sph.impact_config(n, 5, headon=True), eos_res=256, tree_mode=sph.TREE_EVERY,
+                      tree_threads=8)
+    ctx = hl.ctx
+    ctx.set_learned_guess(net)
+    hl.step()
+    c0 = ctx.conservation()
+    t0 = time.perf_counter()
+    for s in range(9):
+        hl.step()
+        print(f"[C5 4e6] step {s + 2} t {ctx.time():.3f} s active {ctx.download('active').mean():.4f} "
+              f"NR {ctx.download('nr_iters').mean():.3f} wall {time.perf_counter() - t0:.1f} s", flush=True)
+    ctx.sync()
+    c1 = ctx.conservation()
+    e0 = c0["kinetic"] + c0["internal"] + c0["potential"]
+    e1 = c1["kinetic"] + c1["internal"] + c1["potential"]
+    de = abs(e1 - e0) / abs(e0)
+    dl = np.linalg.norm(c1["angular_momentum"] - c0["angular_momentum"]) / np.linalg.norm(c0["angular_momentum"])
+    state = {k: ctx.download(k) for k in ("pos", "vel", "rho", "u", "mass", "mat", "pressure", "h")}
+    for k, v in list(state.items()) + [("acc", ctx.download("acc"))]:
+        assert np.all(np.isfinite(v)), k
+    print(f"[C5 4e6] 10 steps to t = {ctx.time():.3f} s: |dE/E| {de:.3e} |dL|/|L| {dl:.3e}", flush=True)
+    # gravity of the final step against the FP64 direct sum of the same positions
+    ctx.set_learned_guess(None)
+    targets = np.random.default_rng(8192).choice(n, 8192, replace=False).astype(np.int32)
+    t1 = time.perf_counter()
+    ref, _ = orc.direct_sum(state["pos"], state["mass"], targets, prm.gravity_g, -prm.gravity_softening)
+    print(f"[C5 4e6] FP64 direct sum at 8192 targets in {time.perf_counter() - t1:.1f} s", flush=True)
+    tree = orc.build_octree(state["pos"], prm.max_children_in_leaf, prm.max_tree_depth)
+    gn = ctx.download("gravnorm")
+    g = sph.Context(prm, state, sph.make_eos_tables(prm, 64))
+    g.upload("gravnorm", gn)
+    g.upload_tree(tree)
+    g.gravity()
+    g.sync()
+    a = g.download("acc")[targets, :3].astype(np.float64)
+    g.close()
+    rel = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    rms = float(np.sqrt(np.mean(rel ** 2)))
+    print(f"[C5 4e6] gravity RMS {rms:.3e} median {np.median(rel):.3e} max {rel.max():.3e}", flush=True)
+    # one density pass of the final state, plain and from the guess, at the
+    # comparison's stopping rule
+    tight = sph.default_params(density_eta=2.5e-6, density_stop=1, max_nr_iterations=400, **form)
+    d = sph.Context(tight, state, sph.make_eos_tables(tight, 256))
+    d.upload_tree(tree)
+    d.sort()
+    d.sync()
+    keep = {k: d.download(k) for k in ("h", "rho", "global_time", "dt")}
+    out = {}
+    for mode in ("plain", "learned"):
+        for k, v in keep.items():
+            d.upload(k, v)
+        d.set_learned_guess(net if mode == "learned" else None)
+        d.density()
+        d.sync()
+        out[mode] = (d.download("h").astype(np.float64), d.download("nr_iters"))
+    d.set_learned_guess(None)
+    d.close()
+    hl.close()
+    net.close()
+    dh = np.abs(out["learned"][0] / out["plain"][0] - 1)
+    print(f"[C5 4e6] density plain NR {out['plain'][1].mean():.3f} learned {out['learned'][1].mean():.3f}; "
+          f"|dh/h| max {dh.max():.2e}", flush=True)
+    assert rms < 1e-3
+    assert dh.max() <= 1e-5
+    assert out["learned"][1].mean() < out["plain"][1].mean()
+    # twice the measured drifts over the 10 steps (|dE/E| 7.3e-4, |dL|/|L| 1.8e-7)
+    assert de < 1.5e-3 and dl < 4e-7

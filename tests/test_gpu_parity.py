@@ -346,7 +346,10 @@ def test_gravity_parity(sph, orc, tables256, impact, n, seed):
     ctx.gravity()
     mstat, act_same, q = _grav_compare(ctx, st, "warm")
     assert act_same and mstat["quad"] < 1e-4
-    assert q[0] < 1e-5 and q[2] < 1e-4 and q[3] < 1e-2
+    # measured q99.9 1.6e-5 / 1.2e-5 and max 7.3e-5 / 5.9e-5 at 2e4 / 1e5; a
+    # MAC decision that flips on the up pass's rounding moves one particle's
+    # field by up to the acceptance tolerance's share of one node
+    assert q[0] < 1e-5 and q[2] < 5e-5 and q[3] < 1e-3
     qs = _same_input(ctx, st, f"warm n={n}")
     assert qs[1] < 1e-5 and qs[2] < 1e-4
 
@@ -363,7 +366,7 @@ def test_gravity_small_list(sph, orc, tables256, impact):
     st.gravity()
     ctx.gravity()
     _, act_same, q = _grav_compare(ctx, st, "maxu16")
-    assert act_same and q[0] < 1e-5 and q[2] < 1e-4 and q[3] < 1e-2
+    assert act_same and q[0] < 1e-5 and q[2] < 1e-4 and q[3] < 1e-3
     qs = _same_input(ctx, st, "maxu16")
     assert qs[1] < 1e-5 and qs[2] < 1e-4
 
@@ -392,7 +395,8 @@ def test_gravity_near_field_parity(sph, orc, tables256, impact, n, seed):
     ctx.gravity()
     ctx.sync()
     _, act_same, q = _grav_compare(ctx, st, f"near warm n={n}")
-    assert act_same and q[0] < 1e-5 and q[2] < 1e-3
+    # measured q99.9 3.3e-6 / 3.4e-6, max 1.2e-5 / 9.6e-4 (one MAC flip)
+    assert act_same and q[0] < 1e-5 and q[2] < 1e-4 and q[3] < 5e-3
     qs = _same_input(ctx, st, f"near warm n={n}")
     assert qs[1] < 1e-5 and qs[2] < 1e-4
 
@@ -410,7 +414,7 @@ def test_gravity_near_field_small_list(sph, orc, tables256, impact):
     ctx.gravity()
     ctx.sync()
     _, act_same, q = _grav_compare(ctx, st, "near maxu16")
-    assert act_same and q[0] < 1e-5 and q[2] < 1e-3
+    assert act_same and q[0] < 1e-5 and q[2] < 1e-4 and q[3] < 5e-3
     qs = _same_input(ctx, st, "near maxu16")
     assert qs[1] < 1e-5 and qs[2] < 1e-4
 
