@@ -54,6 +54,8 @@ void free_tree(sph_tree_dev& T) {
     T.octree = nullptr;
     T.level_tp = nullptr;
     dfree(T.level_id);
+    dfree(T.act_slots);
+    dfree(T.act_count);
     dfree(T.node_info);
     dfree(T.parent_slot);
     dfree(T.parent_level);
@@ -297,6 +299,7 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->tile_hist, (size_t)ctx->ntiles * 256));
     TRY(dalloc(&ctx->digit_total, 256));
     TRY(dalloc(&ctx->grav_flags, 4));
+    TRY(dalloc(&ctx->tree.act_count, kMaxLevels));
     TRY(hipMemset(ctx->grav_flags, 0, 16));
 
     /* initial conditions (Compute.cpp:52-80) */
@@ -416,6 +419,8 @@ int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int 
             }
             dfree(T.level_id);
             SPH_HIP_TRY(dalloc(&T.level_id, (size_t)cap));
+            dfree(T.act_slots);
+            SPH_HIP_TRY(dalloc(&T.act_slots, (size_t)cap));
             T.level_cap = cap;
             ctx->have_tree = false;
         }

@@ -416,6 +416,10 @@ struct DownArgs {
     const uint8_t* active;
     float4* acc;
     float* gravnorm;
+    /* README rule: the level's slots whose node is active (k_level_active)
+     * and their count on the device; null: every slot of the level */
+    const int32_t* act_slots;
+    const int32_t* act_count;
 };
 
 struct LocalExp {
@@ -1161,7 +1165,7 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
  * order, so results are reproducible), adds the incoming local expansion and
  * runs L2L (lane c -> child c) or L2P + P2P (lane k -> particle k). */
 template <int GS, bool NEAR, bool PREC, int P, int MODE>
-__device__ __forceinline__ void down_group_body(const DownArgs& a) {
+__device__ __forceinline__ void down_group_body(const DownArgs& a, int s, bool valid) {
     constexpr int NTP = nterms<P>();
     constexpr int GPB = 256 / GS;
     constexpr int EPR = GS / 8;
@@ -1171,11 +1175,8 @@ __device__ __forceinline__ void down_group_body(const DownArgs& a) {
     const int gbase = wl - lane;
     const uint64_t gmask = GS == 64 ? ~0ull : (((1ull << GS) - 1ull) << gbase);
     const uint64_t ltmask = wl == 0 ? 0ull : (~0ull >> (64 - wl));
-    const int blk = xcd_block((unsigned)((a.count + GPB - 1) / GPB));
-    if (blk < 0) return;
-    const int s = blk * GPB + tid / GS;
     int id = 0;
-    bool live = s < a.count && a.flags[0] == 0u;
+    bool live = valid && a.flags[0] == 0u;
     if (live) {
         id = a.level_id[s];
         live = a.t.aux[3 * id + 1].w != 0.f;
@@ -1289,9 +1290,53 @@ __device__ __forceinline__ void down_group_body(const DownArgs& a) {
     }
 }
 
+/* One level slot per group of GS lanes.  Every slot of the level: one per
+ * group, workgroups XCD-remapped.  With an active-slot list (README rule,
+ * most nodes inactive): a persistent grid whose XCD x walks the x-th eighth
+ * of the list, so no workgroup is launched for an inactive node. */
+template <int GS, bool NEAR, bool PREC, int P, int MODE>
+__device__ __forceinline__ void down_group_run(const DownArgs& a) {
+    constexpr int GPB = 256 / GS;
+    const int g = (int)threadIdx.x / GS;
+    if (!a.act_slots) {
+        const int blk = xcd_block((unsigned)((a.count + GPB - 1) / GPB));
+        if (blk < 0) return;
+        const int sl = blk * GPB + g;
+        down_group_body<GS, NEAR, PREC, P, MODE>(a, sl, sl < a.count);
+        return;
+    }
+    const int total = *a.act_count;
+    const int nlog = (total + GPB - 1) / GPB;
+    const int per = (nlog + kXcds - 1) / kXcds;
+    const int xcd = (int)(blockIdx.x % kXcds), nper = (int)(gridDim.x / kXcds);
+    for (int k = (int)(blockIdx.x / kXcds); k < per; k += nper) {
+        const int lb = xcd * per + k;
+        if (lb >= nlog) break;
+        const int e = lb * GPB + g;
+        const bool ok = e < total;
+        down_group_body<GS, NEAR, PREC, P, MODE>(a, ok ? a.act_slots[e] : 0, ok);
+    }
+}
+
 template <int GS, bool NEAR, bool PREC, int P, int MODE>
 __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
-    down_group_body<GS, NEAR, PREC, P, MODE>(a);
+    down_group_run<GS, NEAR, PREC, P, MODE>(a);
+}
+
+/* The slots of a level whose node is active (the up pass's activity flag),
+ * appended a wave at a time; the list order varies from run to run, each
+ * node's result does not. */
+__global__ __launch_bounds__(256) void k_level_active(const int32_t* __restrict__ level_id, int count,
+                                                      const float4* __restrict__ aux, int32_t* __restrict__ slots,
+                                                      int32_t* __restrict__ nact) {
+    const int sl = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool act = sl < count && aux[3 * level_id[sl] + 1].w != 0.f;
+    const uint64_t b = __ballot(act);
+    const int lane = (int)(threadIdx.x & 63);
+    int base = 0;
+    if (lane == 0 && b != 0ull) base = atomicAdd(nact, (int)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (act) slots[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = sl;
 }
 
 /* The near-field leaves of a level, M2L part.  Two waves per SIMD are asked
@@ -1299,13 +1344,13 @@ __global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
  * that allows and would run one wave per SIMD. */
 template <int GS, bool PREC, int P>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_down_leaves(DownArgs a) {
-    down_group_body<GS, true, PREC, P, 2>(a);
+    down_group_run<GS, true, PREC, P, 2>(a);
 }
 
 /* The near-field leaves of a level, pair sums and L2P (after k_down_leaves). */
 template <int GS, bool PREC, int P>
 __global__ __launch_bounds__(256) void k_down_leaves_p2p(DownArgs a) {
-    down_group_body<GS, true, PREC, P, 3>(a);
+    down_group_run<GS, true, PREC, P, 3>(a);
 }
 
 /* The near-field leaves of a level in one walk (M2L, pair sums, L2P), for
@@ -1315,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_down_leaves_p2p(DownArgs a) {
  * k_down_leaves_p2p); both give the same results, bit for bit. */
 template <int GS, bool PREC, int P>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_down_leaves_walk(DownArgs a) {
-    down_group_body<GS, true, PREC, P, 4>(a);
+    down_group_run<GS, true, PREC, P, 4>(a);
 }
 
 /* Level 0.  zeroFirstBranches (helpers.h:15-32); the root has node id 0.
@@ -1463,7 +1508,24 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
     a.active = ctx->active;
     a.acc = ctx->acc;
     a.gravnorm = ctx->gravnorm;
+    a.act_slots = nullptr;
+    a.act_count = nullptr;
     const int threads = 256;
+    /* under the README rule most nodes are inactive: the group launches run
+     * over lists of the active slots of each level instead */
+    const bool compact = p.activity_rule == SPH_ACTIVE_README && (a.max_rec == 1 || a.near) &&
+                         T.level_counts.size() <= (size_t)kMaxLevels;
+    if (compact) {
+        k_fill_i32<<<1, 64, 0, ctx->stream>>>(T.act_count, kMaxLevels, 0);
+        SPH_HIP_TRY(hipGetLastError());
+        for (size_t l = 1; l < T.level_counts.size(); ++l) {
+            const int cnt = T.level_counts[l];
+            if (cnt == 0) continue;
+            k_level_active<<<(cnt + threads - 1) / threads, threads, 0, ctx->stream>>>(
+                T.level_id + T.level_offsets[l], cnt, T.mp_aux, T.act_slots + T.level_offsets[l], T.act_count + l);
+            SPH_HIP_TRY(hipGetLastError());
+        }
+    }
     for (size_t l = 0; l < T.level_counts.size(); ++l) {
         int32_t* in = (l % 2 == 0) ? ctx->list_a : ctx->list_b;
         int32_t* out = (l % 2 == 0) ? ctx->list_b : ctx->list_a;
@@ -1483,7 +1545,12 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
         if (a.max_rec == 1 || a.near) {
             constexpr int GS = 32;
             const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
-            const dim3 g(xcd_grid(blocks));
+            if (compact) {
+                a.act_slots = T.act_slots + T.level_offsets[l];
+                a.act_count = T.act_count + l;
+            }
+            /* persistent over the active list: at most kDownBlocks workgroups */
+            const dim3 g(compact ? std::min(xcd_grid(blocks), (unsigned)kDownBlocks) : xcd_grid(blocks));
             /* near-field form: branches (list processing, L2L), then the
              * leaves (balanced walk, pair sums, L2P) in one or two launches */
             if (order3) {
@@ -1511,6 +1578,8 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
                 }
             }
         } else {
+            a.act_slots = nullptr;
+            a.act_count = nullptr;
             k_down<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
         }
         SPH_HIP_TRY(hipGetLastError());

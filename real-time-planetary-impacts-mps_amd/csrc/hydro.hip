@@ -143,6 +143,10 @@ struct DensityArgs {
      * original order, coalesced) and their threads here return at once */
     int skip_inactive;
     int stop_rule; /* sph_params.density_stop */
+    /* README rule: the sorted slots of the active particles and their count
+     * on the device (k_active_slots); null: every slot */
+    const int32_t* slots;
+    const int32_t* nslots;
 };
 
 /* Whether the Newton-Raphson iteration goes on after a scan whose relative
@@ -497,14 +501,9 @@ __device__ __forceinline__ void box_slots(const int mn[3], const int mx[3], int 
     });
 }
 
+/* One particle (sorted slot `index`) on its G lanes (`lane` = 0..G-1). */
 template <int G, int SPLIT, int FLAT>
-__global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
-    constexpr int PPB = 256 / G;
-    const int blk = xcd_block((a.n + PPB - 1) / PPB);
-    if (blk < 0) return;
-    const int lane = threadIdx.x % G;
-    const int index = blk * PPB + (int)threadIdx.x / G;
-    if (index >= a.n) return;
+__device__ __forceinline__ void density_group_particle(const DensityArgs& a, int index, int lane) {
     if (index == 0 && lane == 0) a.time[1] = a.dt_reset;
     const uint32_t ind = a.sorted[index].y;
     const float2* table = a.mat[ind] == 400 ? a.eos0 : a.eos1;
@@ -665,10 +664,38 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     if (a.work) a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
 }
 
+/* Every slot: one particle per group of G lanes, workgroups XCD-remapped.
+ * With a slot list (README rule, most particles inactive): a persistent grid
+ * whose XCD x walks the x-th eighth of the list, so no workgroup is spent on
+ * an inactive particle. */
+template <int G, int SPLIT, int FLAT, bool LIST>
+__global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
+    constexpr int PPB = 256 / G;
+    const int lane = (int)threadIdx.x % G, g = (int)threadIdx.x / G;
+    if constexpr (!LIST) {
+        const int blk = xcd_block((a.n + PPB - 1) / PPB);
+        if (blk < 0) return;
+        const int index = blk * PPB + g;
+        if (index < a.n) density_group_particle<G, SPLIT, FLAT>(a, index, lane);
+        return;
+    }
+    const int total = *a.nslots;
+    const int nlog = (total + PPB - 1) / PPB;
+    const int per = (nlog + kXcds - 1) / kXcds;
+    const int xcd = (int)(blockIdx.x % kXcds), nper = (int)(gridDim.x / kXcds);
+    for (int k = (int)(blockIdx.x / kXcds); k < per; k += nper) {
+        const int lb = xcd * per + k;
+        if (lb >= nlog) break;
+        const int e = lb * PPB + g;
+        if (e < total) density_group_particle<G, SPLIT, FLAT>(a, a.slots[e], lane);
+    }
+}
+
 /* density.metal:40-55 for the inactive particles, in original order
  * (coalesced) instead of through the sorted slots. */
 __global__ __launch_bounds__(256) void k_inactive_eos(DensityArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) a.time[1] = a.dt_reset; /* (the density pass's first thread does too) */
     if (i >= a.n || a.active[i]) return;
     const float2* table = a.mat[i] == 400 ? a.eos0 : a.eos1;
     const float2 pc = eos_sample(table, a.eos_res, a.u[i], a.rho[i], a.min_u, a.max_u, a.min_rho, a.max_rho);
@@ -766,6 +793,10 @@ struct AccelArgs {
     float alpha, cfl, gamma, dt0, dtmin, max_h_len;
     int r_max;
     float dlnh_limit; /* > 0: SPH_FIX_INACTIVE_DRIFT's bound on |dh/dt| dt / h */
+    /* README rule: the sorted slots of the active particles and their count
+     * on the device; null: every slot */
+    const int32_t* slots;
+    const int32_t* nslots;
 };
 
 /* accelerations.metal:130-136: the step goal 2 CFL Gamma h / v_sig; with
@@ -785,12 +816,11 @@ __device__ __forceinline__ int wave_min(int v) {
 }
 
 /* accelerations.metal:12-141 */
-__global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
-    const int blk = xcd_block((a.n + 255) / 256);
-    if (blk < 0) return;
-    const int index = blk * 256 + threadIdx.x;
+/* One particle (sorted slot `index`, -1 for none); returns its tick step
+ * (INT_MAX for none or an inactive particle). */
+__device__ __forceinline__ int accel_particle(const AccelArgs& a, int index) {
     int integerDt = 0x7FFFFFFF;
-    if (index < a.n) {
+    if (index >= 0 && index < a.n) {
         const uint32_t ind = a.sorted[index].y;
         if (a.active[ind]) {
             const float4 xi4 = a.s_posh[index];
@@ -866,6 +896,31 @@ __global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
             }
             integerDt = (int)ceilf(a.dtmin * dtCandidate);
             a.next_active[ind] = a.time[0] + integerDt;
+        }
+    }
+    return integerDt;
+}
+
+/* One thread per slot, workgroups XCD-remapped; with a slot list (README
+ * rule) a persistent grid over the list, XCD x taking its x-th eighth.  The
+ * global timestep is the minimum of the tick steps: wave minimum, then one
+ * atomic per wave. */
+__global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
+    int integerDt = 0x7FFFFFFF;
+    if (!a.slots) {
+        const int blk = xcd_block((a.n + 255) / 256);
+        if (blk < 0) return;
+        integerDt = accel_particle(a, blk * 256 + (int)threadIdx.x);
+    } else {
+        const int total = *a.nslots;
+        const int nlog = (total + 255) / 256;
+        const int per = (nlog + kXcds - 1) / kXcds;
+        const int xcd = (int)(blockIdx.x % kXcds), nper = (int)(gridDim.x / kXcds);
+        for (int k = (int)(blockIdx.x / kXcds); k < per; k += nper) {
+            const int lb = xcd * per + k;
+            if (lb >= nlog) break;
+            const int e = lb * 256 + (int)threadIdx.x;
+            integerDt = min(integerDt, accel_particle(a, e < total ? a.slots[e] : -1));
         }
     }
     const int wmin = wave_min(integerDt);
@@ -1124,15 +1179,27 @@ int launch_density(sph_ctx* ctx) {
      * neighbour loops' latency.) */
     a.skip_inactive = p.activity_rule == SPH_ACTIVE_README ? 1 : 0;
     a.stop_rule = p.density_stop;
+    a.slots = nullptr;
+    a.nslots = nullptr;
     if (a.skip_inactive) {
         k_inactive_eos<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(a);
         SPH_HIP_TRY(hipGetLastError());
+        /* the grouped passes walk the active particles' slots only */
+        SPH_TRY_RC(launch_active_list(ctx));
+        a.slots = ctx->slot_list;
+        a.nslots = ctx->slot_count;
     }
     const int threads = 256;
     const int flat = p.density_flat ? 1 : 0;
     const int dsel = (p.density_lanes * 2 + (p.lane_split ? 1 : 0)) * 2 + flat;
-#define SPH_DENS(G, S, F, PPB) \
-    k_density_group<G, S, F><<<xcd_grid((ctx->n + (PPB) - 1) / (PPB)), threads, 0, ctx->stream>>>(a)
+    /* over a slot list the grid is persistent: at most kListBlocks workgroups */
+#define SPH_DENS(G, S, F, PPB)                                                                           \
+    if (a.slots)                                                                                         \
+        k_density_group<G, S, F, true>                                                                   \
+            <<<std::min(xcd_grid((ctx->n + (PPB) - 1) / (PPB)), (unsigned)kListBlocks), threads, 0,     \
+               ctx->stream>>>(a);                                                                        \
+    else                                                                                                 \
+        k_density_group<G, S, F, false><<<xcd_grid((ctx->n + (PPB) - 1) / (PPB)), threads, 0, ctx->stream>>>(a)
     switch (dsel) {
         case 8: SPH_DENS(2, 0, 0, 128); break;
         case 9: SPH_DENS(2, 0, 1, 128); break;
@@ -1183,6 +1250,14 @@ int launch_acceleration(sph_ctx* ctx) {
     a.dtmin = 1.f / dtmin1;
     a.max_h_len = p.max_smoothing_length;
     a.r_max = p.r_max;
+    a.slots = nullptr;
+    a.nslots = nullptr;
+    if (p.activity_rule == SPH_ACTIVE_README) {
+        /* the thread-per-particle pass walks the active particles' slots only */
+        SPH_TRY_RC(launch_active_list(ctx));
+        a.slots = ctx->slot_list;
+        a.nslots = ctx->slot_count;
+    }
     const int threads = 256;
     if (ctx->morton_fresh)
         k_gather_hydro<true><<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(
@@ -1205,7 +1280,11 @@ int launch_acceleration(sph_ctx* ctx) {
         case 65: k_acceleration_group<32, 1><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
         case 128: k_acceleration_group<64, 0><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
         case 129: k_acceleration_group<64, 1><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
-        default: k_acceleration<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a); break;
+        default: {
+            const unsigned g = xcd_grid((ctx->n + threads - 1) / threads);
+            k_acceleration<<<a.slots ? std::min(g, (unsigned)kListBlocks) : g, threads, 0, ctx->stream>>>(a);
+            break;
+        }
     }
     SPH_HIP_TRY(hipGetLastError());
     ctx->dv_pending = true;

@@ -90,6 +90,8 @@ struct sph_tree_dev {
     int up_cur = 0;                 /* buffer holding the current tree */
     hipEvent_t released[2] = {nullptr, nullptr}; /* compute stream done with buffer k */
     int32_t* level_id = nullptr;    /* same slots, node ids */
+    int32_t* act_slots = nullptr;   /* same slots: per level, the slots of its active nodes (README rule) */
+    int32_t* act_count = nullptr;   /* kMaxLevels counters of act_slots */
     int64_t level_cap = 0;
     int4* node_info = nullptr;      /* (nParticles, treePointer, level, 0) */
     int32_t* parent_slot = nullptr; /* slot of the parent in its level list (parentIndexes) */
@@ -259,6 +261,13 @@ int launch_fill_cells(sph_ctx* ctx);
  * 1/8 of the logical blocks: neighbouring particles then share an L2.
  * Launch with xcd_grid(blocks) workgroups; -1 marks a padding workgroup. */
 constexpr int kXcds = 8;
+/* levels the active-slot lists of the down pass cover, and the largest grid
+ * of its persistent launches over them */
+constexpr int kMaxLevels = 64;
+constexpr int kDownBlocks = 2048;
+/* the largest grid of the hydro passes' persistent launches over the active
+ * particles' slots (README rule) */
+constexpr int kListBlocks = 2048;
 inline unsigned xcd_grid(unsigned blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
 __device__ __forceinline__ int xcd_block(unsigned blocks) {
     const unsigned per = (blocks + kXcds - 1) / kXcds;
