@@ -146,6 +146,12 @@ typedef struct sph_params {
      * 2 density_eta of each other; under rule 0 they can end density_eta /
      * (1 - k) apart, 20x that for weakly contracting particles. */
     int32_t density_stop;
+    /* Density pass with lanes per particle, every particle active: 0 = the
+     * waves take the particles in slot order; W (a power of two, 16..512) =
+     * within each window of W consecutive slots, in ascending order of the
+     * particles' previous Newton-Raphson counts, so that waves of like counts
+     * waste fewer scans on their slowest particle (the same results). */
+    int32_t density_order;
 } sph_params;
 /* Relative change of h (2 ulp) at which density_stop = 1 stops whatever k. */
 #define SPH_NR_NOISE_FLOOR 2.5e-7f

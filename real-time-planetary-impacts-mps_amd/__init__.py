@@ -65,6 +65,7 @@ class SphParams(ctypes.Structure):
         ("accel_lanes", ctypes.c_int32), ("lane_split", ctypes.c_int32),
         ("expansion_order", ctypes.c_int32), ("mac_order", ctypes.c_int32), ("density_flat", ctypes.c_int32),
         ("density_stop", ctypes.c_int32),
+        ("density_order", ctypes.c_int32),
     ]
 
 

@@ -35,6 +35,11 @@ bool valid_params(const sph_params* p, std::string& why) {
     if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }
     if (p->density_flat != 0 && p->density_flat != 1) { why = "density_flat must be 0 or 1"; return false; }
     if (p->density_stop != 0 && p->density_stop != 1) { why = "density_stop must be 0 or 1"; return false; }
+    if (p->density_order != 0 && (p->density_order < 16 || p->density_order > 512 ||
+                                  (p->density_order & (p->density_order - 1)))) {
+        why = "density_order must be 0 or a power of two from 16 to 512";
+        return false;
+    }
     if (p->expansion_order != 2 && p->expansion_order != 3) { why = "expansion_order must be 2 or 3"; return false; }
     if (p->mac_order != 0 && p->mac_order != 2) { why = "mac_order must be 0 or 2"; return false; }
     if (p->expansion_order == 3 && p->gravity_max_recursion != 1 && !(p->fixes & SPH_FIX_NEAR_FIELD)) {
@@ -82,7 +87,7 @@ void destroy_ctx(sph_ctx* ctx) {
     float** f1[] = {&ctx->rho, &ctx->u, &ctx->m, &ctx->P, &ctx->h, &ctx->f, &ctx->c,
                     &ctx->du, &ctx->B, &ctx->dhdt, &ctx->gravnorm, &ctx->phi, &ctx->dlnh};
     for (auto p : f1) dfree(*p);
-    int32_t** i1[] = {&ctx->mat, &ctx->next_active, &ctx->nr_iters, &ctx->time, &ctx->list_a, &ctx->list_b};
+    int32_t** i1[] = {&ctx->mat, &ctx->next_active, &ctx->nr_iters, &ctx->nr_order, &ctx->time, &ctx->list_a, &ctx->list_b};
     for (auto p : i1) dfree(*p);
     dfree(ctx->active);
     dfree(ctx->density_work);
@@ -288,6 +293,7 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->mat, n));
     TRY(dalloc(&ctx->next_active, n));
     TRY(dalloc(&ctx->nr_iters, n));
+    TRY(dalloc(&ctx->nr_order, n));
     TRY(dalloc(&ctx->active, n));
     TRY(dalloc(&ctx->density_work, n));
     TRY(hipMemset(ctx->density_work, 0, 16 * n));

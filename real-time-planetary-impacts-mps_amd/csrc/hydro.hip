@@ -147,6 +147,9 @@ struct DensityArgs {
      * on the device (k_active_slots); null: every slot */
     const int32_t* slots;
     const int32_t* nslots;
+    /* every slot: the order in which the groups take them (k_nr_order), or
+     * null for slot order */
+    const int32_t* order;
 };
 
 /* Whether the Newton-Raphson iteration goes on after a scan whose relative
@@ -668,6 +671,31 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
  * With a slot list (README rule, most particles inactive): a persistent grid
  * whose XCD x walks the x-th eighth of the list, so no workgroup is spent on
  * an inactive particle. */
+/* The sorted slots of each window of W consecutive slots in ascending order
+ * of their particles' previous Newton-Raphson counts (ties by slot), for the
+ * density pass's `order`: a wave runs as many scans as its slowest particle,
+ * and the counts change little from one step to the next, so waves of like
+ * counts waste fewer scans.  One workgroup of W threads per window. */
+template <int W>
+__global__ __launch_bounds__(W) void k_nr_order(int n, const uint2* __restrict__ sorted,
+                                               const int32_t* __restrict__ nr_iters, int32_t* __restrict__ order) {
+    __shared__ int s_pred[W];
+    const int t = (int)threadIdx.x;
+    const int base = (int)blockIdx.x * W;
+    const int pk = base + t < n ? nr_iters[sorted[base + t].y] : 0x7FFFFFFF;
+    s_pred[t] = pk;
+    __syncthreads();
+    int r = 0;
+    for (int j = 0; j < W; ++j) {
+        const int pj = s_pred[j];
+        r += (pj < pk || (pj == pk && j < t)) ? 1 : 0;
+    }
+    if (base + r < n) order[base + r] = base + t;
+}
+
+/* Every slot: one particle per group of G lanes, workgroups XCD-remapped;
+ * with `order` (k_nr_order) the groups take the slots in that order instead
+ * (only which lanes take a particle changes, not its result). */
 template <int G, int SPLIT, int FLAT, bool LIST>
 __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     constexpr int PPB = 256 / G;
@@ -675,8 +703,8 @@ __global__ __launch_bounds__(256) void k_density_group(DensityArgs a) {
     if constexpr (!LIST) {
         const int blk = xcd_block((a.n + PPB - 1) / PPB);
         if (blk < 0) return;
-        const int index = blk * PPB + g;
-        if (index < a.n) density_group_particle<G, SPLIT, FLAT>(a, index, lane);
+        const int k = blk * PPB + g;
+        if (k < a.n) density_group_particle<G, SPLIT, FLAT>(a, a.order ? a.order[k] : k, lane);
         return;
     }
     const int total = *a.nslots;
@@ -1181,6 +1209,26 @@ int launch_density(sph_ctx* ctx) {
     a.stop_rule = p.density_stop;
     a.slots = nullptr;
     a.nslots = nullptr;
+    a.order = nullptr;
+    if (!a.skip_inactive && p.density_order > 0) {
+#define SPH_NR_ORDER(W)                                                                                   \
+    case W:                                                                                               \
+        k_nr_order<W><<<(ctx->n + (W) - 1) / (W), W, 0, ctx->stream>>>(ctx->n, ctx->pairs_a, ctx->nr_iters, \
+                                                                      ctx->nr_order);                     \
+        break
+        switch (p.density_order) {
+            SPH_NR_ORDER(16);
+            SPH_NR_ORDER(32);
+            SPH_NR_ORDER(64);
+            SPH_NR_ORDER(128);
+            SPH_NR_ORDER(256);
+            SPH_NR_ORDER(512);
+            default: return SPH_E_ARG;
+        }
+#undef SPH_NR_ORDER
+        SPH_HIP_TRY(hipGetLastError());
+        a.order = ctx->nr_order;
+    }
     if (a.skip_inactive) {
         k_inactive_eos<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(a);
         SPH_HIP_TRY(hipGetLastError());

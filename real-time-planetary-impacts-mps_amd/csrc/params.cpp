@@ -42,4 +42,8 @@ extern "C" void sph_params_default(sph_params* o) {
     o->accel_lanes = 0;
     o->lane_split = 0;
     o->expansion_order = P; /* Parameters.h */
+    /* measured at 1e6 on MI355X (100 steps): density 3.95 ms in slot order,
+     * 3.85 / 3.78 / 3.95 ms in windows of 16 / 32 / 64 slots ranked by the
+     * previous iteration counts */
+    o->density_order = 32;
 }

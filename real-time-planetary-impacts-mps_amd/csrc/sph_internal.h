@@ -204,6 +204,7 @@ struct sph_ctx {
     /* learned smoothing-length guess (learned.hip) */
     float4* feats = nullptr;        /* 3 float4 (12 features) per sorted slot */
     float* guess = nullptr;         /* starting h per sorted slot */
+    int32_t* nr_order = nullptr;    /* density pass's slot order by previous iteration count (density_order) */
     int32_t* slot_list = nullptr;   /* sorted slots of the active particles (README rule) */
     int32_t* slot_count = nullptr;  /* their number (device) */
     bool slot_list_valid = false;   /* cleared by a sort and by a gravity pass */

@@ -61,8 +61,8 @@ def test_default_params_are_the_reference_values(sph):
 
 def test_params_struct_layout_matches_c(sph):
     # 26 four-byte fields, precision, density_lanes, accel_lanes, lane_split, expansion_order, mac_order,
-    # density_flat, density_stop
-    assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 8)
+    # density_flat, density_stop, density_order
+    assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 9)
 
 
 # ---------------------------------------------------------------- EOS
@@ -185,7 +185,8 @@ def test_create_rejects_empty_and_invalid_inputs_before_touching_a_device(sph):
         assert h.sph_create(ctypes.byref(prm), ctypes.byref(view), arr, 2, ctypes.byref(out)) == sph.SPH_E_ARG
         assert not out.value
     view = sph.SphIcView(1, f(one), f(one), f(s1), f(s1), f(s1), f(mat, ctypes.c_int32), f(s1), f(s1))
-    for field, bad in (("cell_power", 0), ("max_unchecked_pointers", 1), ("expansion_order", 4), ("density_lanes", 3)):
+    for field, bad in (("cell_power", 0), ("max_unchecked_pointers", 1), ("expansion_order", 4), ("density_lanes", 3),
+                       ("density_order", 8), ("density_order", 48), ("density_order", 1024)):
         p = sph.default_params(**{field: bad})
         assert h.sph_create(ctypes.byref(p), ctypes.byref(view), arr, 2, ctypes.byref(out)) == sph.SPH_E_ARG, field
     assert h.sph_create(ctypes.byref(prm), ctypes.byref(view), arr, 0, ctypes.byref(out)) == sph.SPH_E_ARG
