@@ -152,6 +152,18 @@ typedef struct sph_params {
      * particles' previous Newton-Raphson counts, so that waves of like counts
      * waste fewer scans on their slowest particle (the same results). */
     int32_t density_order;
+    /* Lanes per particle of the density and acceleration passes when they
+     * walk the active particles only (activity_rule README), as
+     * density_lanes and accel_lanes (0 = one thread per particle).  Such a
+     * pass has few particles, so its time is the longest particle's chain
+     * of dependent loads: more lanes per particle shorten it. */
+    int32_t list_density_lanes;
+    int32_t list_accel_lanes;
+    /* With the active particles only: 1 = the acceleration pass's
+     * Morton-ordered records are gathered for the cells some active
+     * particle's cell box reaches (the only records the pass reads); 0 =
+     * for every particle.  The same results either way. */
+    int32_t gather_reach;
 } sph_params;
 /* Relative change of h (2 ulp) at which density_stop = 1 stops whatever k. */
 #define SPH_NR_NOISE_FLOOR 2.5e-7f

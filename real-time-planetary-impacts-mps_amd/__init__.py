@@ -65,7 +65,8 @@ class SphParams(ctypes.Structure):
         ("accel_lanes", ctypes.c_int32), ("lane_split", ctypes.c_int32),
         ("expansion_order", ctypes.c_int32), ("mac_order", ctypes.c_int32), ("density_flat", ctypes.c_int32),
         ("density_stop", ctypes.c_int32),
-        ("density_order", ctypes.c_int32),
+        ("density_order", ctypes.c_int32), ("list_density_lanes", ctypes.c_int32),
+        ("list_accel_lanes", ctypes.c_int32), ("gather_reach", ctypes.c_int32),
     ]
 
 

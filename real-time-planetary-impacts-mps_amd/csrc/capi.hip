@@ -30,6 +30,15 @@ bool valid_params(const sph_params* p, std::string& why) {
     auto lanes_ok = [](int v) { return v == 0 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32 || v == 64; };
     if (!lanes_ok(p->density_lanes)) { why = "density_lanes must be 0, 2, 4, 8, 16, 32 or 64"; return false; }
     if (!lanes_ok(p->accel_lanes)) { why = "accel_lanes must be 0, 2, 4, 8, 16, 32 or 64"; return false; }
+    if (!lanes_ok(p->list_density_lanes)) {
+        why = "list_density_lanes must be 0, 2, 4, 8, 16, 32 or 64";
+        return false;
+    }
+    if (!lanes_ok(p->list_accel_lanes)) { why = "list_accel_lanes must be 0, 2, 4, 8, 16, 32 or 64"; return false; }
+    if ((p->list_density_lanes == 2 || p->list_density_lanes == 4 || p->list_accel_lanes == 2 ||
+         p->list_accel_lanes == 4) &&
+        p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
+    if (p->gather_reach != 0 && p->gather_reach != 1) { why = "gather_reach must be 0 or 1"; return false; }
     if ((p->density_lanes == 2 || p->density_lanes == 4 || p->accel_lanes == 2 || p->accel_lanes == 4) &&
         p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }
@@ -94,6 +103,7 @@ void destroy_ctx(sph_ctx* ctx) {
     dfree(ctx->pairs_a);
     dfree(ctx->pairs_b);
     dfree(ctx->cell_se);
+    dfree(ctx->cell_mark);
     dfree(ctx->tile_hist);
     dfree(ctx->grav_flags);
     dfree(ctx->digit_total);
@@ -301,6 +311,8 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->pairs_a, n));
     TRY(dalloc(&ctx->pairs_b, n));
     TRY(dalloc(&ctx->cell_se, (size_t)ctx->ncells));
+    TRY(dalloc(&ctx->cell_mark, (size_t)ctx->ncells));
+    TRY(hipMemset(ctx->cell_mark, 0, (size_t)ctx->ncells));
     ctx->ntiles = (int)((n + 2047) / 2048);
     TRY(dalloc(&ctx->tile_hist, (size_t)ctx->ntiles * 256));
     TRY(dalloc(&ctx->digit_total, 256));

@@ -760,6 +760,9 @@ __global__ void k_zero_i32(int32_t* p) { p[0] = 0; }
  * and velocity come from the sort's Morton-ordered copies (unchanged since
  * the sort), read in order; only the density pass's outputs are gathered
  * from original order.  Otherwise every field is gathered. */
+/* With `mark` (the active particles only), a slot whose cell does not hold
+ * `epoch` is skipped: no active particle's cell box reaches it, so the
+ * acceleration pass reads none of its records. */
 template <bool MORTON>
 __global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __restrict__ sorted,
                                                       const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -768,10 +771,13 @@ __global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __rest
                                                       const float* __restrict__ f, const float* __restrict__ P,
                                                       const float* __restrict__ rho, const float* __restrict__ c,
                                                       const float* __restrict__ B, float4* __restrict__ s_posh,
-                                                      float4* __restrict__ s_velb, float4* __restrict__ s_hyd) {
+                                                      float4* __restrict__ s_velb, float4* __restrict__ s_hyd,
+                                                      const uint8_t* __restrict__ mark, uint8_t epoch) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t j = sorted[k].y;
+    const uint2 sk = sorted[k];
+    if (mark && mark[sk.x] != epoch) return;
+    const uint32_t j = sk.y;
     const float r = rho[j];
     float4 x, v;
     float mj;
@@ -787,6 +793,48 @@ __global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __rest
     s_posh[k] = make_float4(x.x, x.y, x.z, h[j]);
     s_velb[k] = make_float4(v.x, v.y, v.z, B[j]);
     s_hyd[k] = make_float4(mj, f[j] * P[j] / (r * r), r, c[j]);
+}
+
+/* Marks with `epoch` every cell of the acceleration pass's cell box of each
+ * listed (active) particle: the same box from the same position and h as
+ * that pass forms (the sort's Morton copy of the position, or the position
+ * itself when that copy is stale, and h).  G lanes per particle stride over
+ * the box; a persistent grid walks the list, XCD x taking the x-th eighth.
+ * Marks left from 255 calls before only make the gather copy a few records
+ * no pass reads. */
+template <int G>
+__global__ __launch_bounds__(256) void k_mark_cells(const int32_t* __restrict__ slots, const int32_t* __restrict__ nslots,
+                                                    const uint2* __restrict__ sorted,
+                                                    const float4* __restrict__ s_posm, const float4* __restrict__ pos,
+                                                    const float* __restrict__ h, float cell_size, int mask,
+                                                    float max_h_len, uint8_t epoch, uint8_t* __restrict__ mark) {
+    constexpr int PPB = 256 / G;
+    const int lane = (int)threadIdx.x % G, g = (int)threadIdx.x / G;
+    const int total = *nslots;
+    const int nlog = (total + PPB - 1) / PPB;
+    const int per = (nlog + kXcds - 1) / kXcds;
+    const int xcd = (int)(blockIdx.x % kXcds), nper = (int)(gridDim.x / kXcds);
+    for (int q = (int)(blockIdx.x / kXcds); q < per; q += nper) {
+        const int lb = xcd * per + q;
+        if (lb >= nlog) break;
+        const int e = lb * PPB + g;
+        if (e >= total) continue;
+        const int k = slots[e];
+        const uint32_t j = sorted[k].y;
+        const float4 x = s_posm ? s_posm[k] : pos[j];
+        int mn[3], mx[3];
+        cell_box(make_float3(x.x, x.y, x.z), cell_size, fminf(h[j], max_h_len), mask + 1, mn, mx);
+        const int ny = max(mx[1] - mn[1] + 1, 0), nz = max(mx[2] - mn[2] + 1, 0);
+        const int ncell = max(mx[0] - mn[0] + 1, 0) * ny * nz;
+        for (int c = lane; c < ncell; c += G) {
+            const int iz = c % nz, t = c / nz;
+            const int iy = t % ny, ix = t / ny;
+            const uint32_t key = dev_expand_bits((uint32_t)((mn[0] + ix) & mask)) |
+                                 (dev_expand_bits((uint32_t)((mn[1] + iy) & mask)) << 1) |
+                                 (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
+            mark[key] = epoch;
+        }
+    }
 }
 
 /* The sort's Morton-ordered copies (position + mass, velocity) gathered
@@ -961,15 +1009,10 @@ __global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
  * butterfly (sums) and a butterfly maximum (v_sig).  Same terms as the reference; only the
  * order of the additions differs. */
 template <int G, int SPLIT>
-__global__ __launch_bounds__(256) void k_acceleration_group(AccelArgs a) {
-    constexpr int PPB = 256 / G;
-    const int blk = xcd_block((a.n + PPB - 1) / PPB);
-    if (blk < 0) return;
-    const int lane = threadIdx.x % G;
-    const int index = blk * PPB + (int)threadIdx.x / G;
+__device__ __forceinline__ int accel_group_particle(const AccelArgs& a, int index, int lane) {
     int integerDt = 0x7FFFFFFF;
-    const uint32_t ind = index < a.n ? a.sorted[index].y : 0u;
-    if (index < a.n && a.active[ind]) {
+    const uint32_t ind = index >= 0 && index < a.n ? a.sorted[index].y : 0u;
+    if (index >= 0 && index < a.n && a.active[ind]) {
         const float4 xi4 = a.s_posh[index];
         const float4 vi4 = a.s_velb[index];
         const float4 hyd = a.s_hyd[index];
@@ -1051,6 +1094,34 @@ __global__ __launch_bounds__(256) void k_acceleration_group(AccelArgs a) {
             }
             integerDt = (int)ceilf(a.dtmin * dtCandidate);
             a.next_active[ind] = a.time[0] + integerDt;
+        }
+    }
+    return integerDt;
+}
+
+/* LIST false: every slot, workgroups XCD-remapped; LIST true: a persistent
+ * grid over the active particles' slots, XCD x taking the x-th eighth of the
+ * list.  Every group of a wave runs the same number of rounds, so the
+ * butterflies see whole groups. */
+template <int G, int SPLIT, bool LIST>
+__global__ __launch_bounds__(256) void k_acceleration_group(AccelArgs a) {
+    constexpr int PPB = 256 / G;
+    const int lane = threadIdx.x % G, g = (int)threadIdx.x / G;
+    int integerDt = 0x7FFFFFFF;
+    if constexpr (!LIST) {
+        const int blk = xcd_block((a.n + PPB - 1) / PPB);
+        if (blk < 0) return;
+        integerDt = accel_group_particle<G, SPLIT>(a, blk * PPB + g, lane);
+    } else {
+        const int total = *a.nslots;
+        const int nlog = (total + PPB - 1) / PPB;
+        const int per = (nlog + kXcds - 1) / kXcds;
+        const int xcd = (int)(blockIdx.x % kXcds), nper = (int)(gridDim.x / kXcds);
+        for (int k = (int)(blockIdx.x / kXcds); k < per; k += nper) {
+            const int lb = xcd * per + k;
+            if (lb >= nlog) break;
+            const int e = lb * PPB + g;
+            integerDt = min(integerDt, accel_group_particle<G, SPLIT>(a, e < total ? a.slots[e] : -1, lane));
         }
     }
     const int wmin = wave_min(integerDt);
@@ -1239,7 +1310,8 @@ int launch_density(sph_ctx* ctx) {
     }
     const int threads = 256;
     const int flat = p.density_flat ? 1 : 0;
-    const int dsel = (p.density_lanes * 2 + (p.lane_split ? 1 : 0)) * 2 + flat;
+    const int dlanes = a.slots ? p.list_density_lanes : p.density_lanes;
+    const int dsel = (dlanes * 2 + (p.lane_split ? 1 : 0)) * 2 + flat;
     /* over a slot list the grid is persistent: at most kListBlocks workgroups */
 #define SPH_DENS(G, S, F, PPB)                                                                           \
     if (a.slots)                                                                                         \
@@ -1307,33 +1379,57 @@ int launch_acceleration(sph_ctx* ctx) {
         a.nslots = ctx->slot_count;
     }
     const int threads = 256;
+    /* with the active particles only, the records are gathered for the
+     * cells their boxes reach */
+    const uint8_t* mark = nullptr;
+    uint8_t epoch = 0;
+    if (a.slots && p.gather_reach) {
+        ctx->mark_epoch = ctx->mark_epoch % 255 + 1;
+        epoch = (uint8_t)ctx->mark_epoch;
+        k_mark_cells<16><<<std::min(xcd_grid((ctx->n + 15) / 16), (unsigned)kListBlocks), threads, 0, ctx->stream>>>(
+            a.slots, a.nslots, ctx->pairs_a, ctx->morton_fresh ? ctx->s_posm : nullptr, ctx->pos, ctx->h,
+            a.cells.cell_size, a.cells.mask, p.max_smoothing_length, epoch, ctx->cell_mark);
+        SPH_HIP_TRY(hipGetLastError());
+        mark = ctx->cell_mark;
+    }
     if (ctx->morton_fresh)
         k_gather_hydro<true><<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(
             ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m, ctx->s_posm, ctx->s_vel, ctx->h, ctx->f, ctx->P,
-            ctx->rho, ctx->c, ctx->B, ctx->s_posh, ctx->s_velb, ctx->s_hyd);
+            ctx->rho, ctx->c, ctx->B, ctx->s_posh, ctx->s_velb, ctx->s_hyd, mark, epoch);
     else
         k_gather_hydro<false><<<(ctx->n + threads - 1) / threads, threads, 0, ctx->stream>>>(
             ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m, ctx->s_posm, ctx->s_vel, ctx->h, ctx->f, ctx->P,
-            ctx->rho, ctx->c, ctx->B, ctx->s_posh, ctx->s_velb, ctx->s_hyd);
+            ctx->rho, ctx->c, ctx->B, ctx->s_posh, ctx->s_velb, ctx->s_hyd, mark, epoch);
     SPH_HIP_TRY(hipGetLastError());
-    const int asel = p.accel_lanes * 2 + (p.lane_split ? 1 : 0);
+    const int alanes = a.slots ? p.list_accel_lanes : p.accel_lanes;
+    const int asel = alanes * 2 + (p.lane_split ? 1 : 0);
+    /* over a slot list the grid is persistent: at most kListBlocks workgroups */
+#define SPH_ACC(G, S)                                                                                    \
+    if (a.slots)                                                                                         \
+        k_acceleration_group<G, S, true>                                                                 \
+            <<<std::min(xcd_grid((ctx->n + (256 / (G)) - 1) / (256 / (G))), (unsigned)kListBlocks), threads, 0, \
+               ctx->stream>>>(a);                                                                        \
+    else                                                                                                 \
+        k_acceleration_group<G, S, false>                                                                \
+            <<<xcd_grid((ctx->n + (256 / (G)) - 1) / (256 / (G))), threads, 0, ctx->stream>>>(a)
     switch (asel) {
-        case 4: k_acceleration_group<2, 0><<<xcd_grid((ctx->n + 127) / 128), threads, 0, ctx->stream>>>(a); break;
-        case 8: k_acceleration_group<4, 0><<<xcd_grid((ctx->n + 63) / 64), threads, 0, ctx->stream>>>(a); break;
-        case 16: k_acceleration_group<8, 0><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
-        case 17: k_acceleration_group<8, 1><<<xcd_grid((ctx->n + 31) / 32), threads, 0, ctx->stream>>>(a); break;
-        case 32: k_acceleration_group<16, 0><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
-        case 33: k_acceleration_group<16, 1><<<xcd_grid((ctx->n + 15) / 16), threads, 0, ctx->stream>>>(a); break;
-        case 64: k_acceleration_group<32, 0><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
-        case 65: k_acceleration_group<32, 1><<<xcd_grid((ctx->n + 7) / 8), threads, 0, ctx->stream>>>(a); break;
-        case 128: k_acceleration_group<64, 0><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
-        case 129: k_acceleration_group<64, 1><<<xcd_grid((ctx->n + 3) / 4), threads, 0, ctx->stream>>>(a); break;
+        case 4: SPH_ACC(2, 0); break;
+        case 8: SPH_ACC(4, 0); break;
+        case 16: SPH_ACC(8, 0); break;
+        case 17: SPH_ACC(8, 1); break;
+        case 32: SPH_ACC(16, 0); break;
+        case 33: SPH_ACC(16, 1); break;
+        case 64: SPH_ACC(32, 0); break;
+        case 65: SPH_ACC(32, 1); break;
+        case 128: SPH_ACC(64, 0); break;
+        case 129: SPH_ACC(64, 1); break;
         default: {
             const unsigned g = xcd_grid((ctx->n + threads - 1) / threads);
             k_acceleration<<<a.slots ? std::min(g, (unsigned)kListBlocks) : g, threads, 0, ctx->stream>>>(a);
             break;
         }
     }
+#undef SPH_ACC
     SPH_HIP_TRY(hipGetLastError());
     ctx->dv_pending = true;
     return SPH_OK;

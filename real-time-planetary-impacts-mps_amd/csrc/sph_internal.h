@@ -204,7 +204,11 @@ struct sph_ctx {
     /* learned smoothing-length guess (learned.hip) */
     float4* feats = nullptr;        /* 3 float4 (12 features) per sorted slot */
     float* guess = nullptr;         /* starting h per sorted slot */
-    int32_t* nr_order = nullptr;    /* density pass's slot order by previous iteration count (density_order) */
+    int32_t* nr_order = nullptr;
+    /* per cell: the epoch of the last acceleration pass over the active
+     * particles whose cell boxes reach it (1..255, cycling) */
+    uint8_t* cell_mark = nullptr;
+    int mark_epoch = 0;    /* density pass's slot order by previous iteration count (density_order) */
     int32_t* slot_list = nullptr;   /* sorted slots of the active particles (README rule) */
     int32_t* slot_count = nullptr;  /* their number (device) */
     bool slot_list_valid = false;   /* cleared by a sort and by a gravity pass */
