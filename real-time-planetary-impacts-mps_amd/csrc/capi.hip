@@ -39,6 +39,7 @@ bool valid_params(const sph_params* p, std::string& why) {
          p->list_accel_lanes == 4) &&
         p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->gather_reach != 0 && p->gather_reach != 1) { why = "gather_reach must be 0 or 1"; return false; }
+    if (p->near_list_cap < 0 || p->near_list_cap > 4096) { why = "near_list_cap must be 0..4096"; return false; }
     if ((p->density_lanes == 2 || p->density_lanes == 4 || p->accel_lanes == 2 || p->accel_lanes == 4) &&
         p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }
@@ -82,6 +83,8 @@ void free_tree(sph_tree_dev& T) {
     dfree(T.loc3);
     dfree(T.loc);
     dfree(T.loc_tot);
+    dfree(T.near_rec);
+    dfree(T.near_cnt);
     T.octree_cap = T.level_cap = T.node_cap = 0;
 }
 
@@ -469,7 +472,15 @@ int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int 
             }
             SPH_HIP_TRY(dalloc(&T.loc, (size_t)cap * 4));
             dfree(T.loc_tot);
-            if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) SPH_HIP_TRY(dalloc(&T.loc_tot, (size_t)cap * 7));
+            dfree(T.near_rec);
+            dfree(T.near_cnt);
+            if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) {
+                SPH_HIP_TRY(dalloc(&T.loc_tot, (size_t)cap * 7));
+                if (ctx->prm.near_list_cap > 0) {
+                    SPH_HIP_TRY(dalloc(&T.near_rec, (size_t)cap * (size_t)ctx->prm.near_list_cap));
+                    SPH_HIP_TRY(dalloc(&T.near_cnt, (size_t)cap));
+                }
+            }
             T.node_cap = cap;
         }
         if (list_need > ctx->list_cap) {

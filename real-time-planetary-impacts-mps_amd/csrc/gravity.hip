@@ -47,6 +47,12 @@ struct Tree {
     float4* oct;             /* 3 per node (P = 3): (XXX XXY XXZ XYY) (XYZ XZZ YYY YYZ) (YZZ ZZZ power3 -) */
     float4* loc3;            /* 3 per node (P = 3): third-order local terms, same order */
     float4* loc_tot;         /* 7 per node: a near-field leaf's complete local expansion (loc, then loc3 layout) */
+    /* near-field leaves in two launches: per node, the rejected source
+     * leaves the M2L launch's walk found, in walk order (near_cap per node),
+     * and their count (-1: not recorded, the pair launch walks again) */
+    int32_t* near_rec;
+    int32_t* near_cnt;
+    int near_cap;
 };
 
 /* ------------------------------------------------------------ conversion */
@@ -1035,6 +1041,19 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             T.cx[kk] = T.cy[kk] = T.cz[kk] = T.gp[kk] = T.cp[kk] = 0.f;
         }
         const bool do_m2l = PART != 2 && k0 == 0;
+        /* PART 2 with the M2L launch's record of the rejected source leaves:
+         * the pair sums over that list, G-th entries per lane, no walk */
+        if constexpr (PART == 2) {
+            const int rc = a.t.near_cap > 0 ? a.t.near_cnt[id] : -1;
+            if (rc >= 0) {
+                const int32_t* rec = a.t.near_rec + (int64_t)id * a.t.near_cap;
+                for (int e = lane; e < rc; e += GS) near_p2p<PREC, NEAR_K>(a, T, nk, a.t.info[rec[e]]);
+                goto reduce;
+            }
+        }
+        {
+        int nrec = 0;          /* PART 1: leaves recorded so far (group-uniform) */
+        bool rec_off = false;  /* PART 1: some lane walked a subtree alone (unrecorded order) */
         int lp = 0;            /* next list index (group-uniform, as are the rest) */
         bool list_open = true;
         int qh = 0, qt = 0;    /* queue head and tail (monotone; slot = value % QCAP) */
@@ -1056,6 +1075,7 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             qh += take;
             if (nfill + take == 0 && !list_open) break;
             int nch = 0;
+            bool near_leaf = false;
             if (item >= 0 && item != id) {
                 const float4 b0 = a.t.far[3 * item], b1 = a.t.far[3 * item + 1], b2 = a.t.far[3 * item + 2];
                 if (mac_ok(a, A, item, b0, b1, b2)) {
@@ -1066,11 +1086,23 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
                 } else {
                     const int4 ci = a.t.info[item];
                     if (ci.x != 0) {
+                        near_leaf = true;
                         if (PART != 1) near_p2p<PREC, NEAR_K>(a, T, nk, ci);
                     } else {
 #pragma unroll
                         for (int c = 0; c < 8; ++c) nch += a.t.child[8 * item + c] >= 0 ? 1 : 0;
                     }
+                }
+            }
+            if constexpr (PART == 1) {
+                if (a.t.near_cap > 0) {
+                    /* record the round's rejected leaves in lane order */
+                    const uint64_t nm = __ballot(near_leaf) & gmask;
+                    if (near_leaf) {
+                        const int pos = nrec + __popcll(nm & ltmask);
+                        if (pos < a.t.near_cap) a.t.near_rec[(int64_t)id * a.t.near_cap + pos] = item;
+                    }
+                    nrec += __popcll(nm);
                 }
             }
             /* group prefix sum of the child counts (0..8) from four bit ballots */
@@ -1093,6 +1125,7 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
                 qt += total;
             } else if (nch > 0) {
                 /* queue full: walk the branch's subtree here, in pre-order */
+                rec_off = true;
                 int cur = item + 1;
                 const int end = a.t.sub_end[item];
                 while (cur < end) {
@@ -1118,6 +1151,14 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }
+        if constexpr (PART == 1) {
+            if (a.t.near_cap > 0) {
+                const bool off = (__ballot(rec_off) & gmask) != 0ull;
+                if (lane == 0) a.t.near_cnt[id] = (off || nrec > a.t.near_cap) ? -1 : nrec;
+            }
+        }
+        }
+    reduce:
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk) {
             T.gx[kk] = group_sum<GS>(T.gx[kk]);
@@ -1413,6 +1454,9 @@ Tree make_tree(sph_ctx* ctx) {
     t.oct = ctx->tree.mp_oct;
     t.loc3 = ctx->tree.loc3;
     t.loc_tot = ctx->tree.loc_tot;
+    t.near_rec = ctx->tree.near_rec;
+    t.near_cnt = ctx->tree.near_cnt;
+    t.near_cap = ctx->tree.near_rec ? ctx->prm.near_list_cap : 0;
     return t;
 }
 

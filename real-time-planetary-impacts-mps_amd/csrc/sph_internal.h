@@ -106,6 +106,8 @@ struct sph_tree_dev {
     float4* loc3 = nullptr;         /* 3 per node: third-order local terms (expansion order 3) */
     float4* loc = nullptr;          /* 4 per node */
     float4* loc_tot = nullptr;      /* 7 per node: a near-field leaf's complete local expansion */
+    int32_t* near_rec = nullptr;    /* near_list_cap per node: a near-field leaf's rejected source leaves */
+    int32_t* near_cnt = nullptr;    /* per node: their count, -1 when not recorded */
     int64_t node_cap = 0;
     int32_t node_count = 0;
     std::vector<int32_t> level_counts, level_offsets;

@@ -139,3 +139,34 @@ def test_p3_accuracy_against_fp64(sph, orc, tables256, impact):
               flush=True)
         ctx.close()
     assert out[3] < 1e-3 and out[3] < 0.5 * out[2]
+
+
+@pytest.mark.parametrize("cap", [64, 1024])
+def test_p3_near_list_matches_walk(sph, orc, tables256, impact, cap):
+    """Precision form at P = 3 (the C5 gravity: GRAVITY_ETA 1e-3, the P = 2
+    acceptance estimate), 1e5 particles, warm: the pair-sum launch summing
+    over the M2L launch's record of each leaf's rejected source leaves
+    (near_list_cap) against walking the tree again (0).  The pairs are the
+    same; which lane sums which changes, so the compensated sums differ in
+    rounding only (measured: 2e-7 relative at most).  Capacity 64 leaves
+    some leaves unrecorded (they walk)."""
+    n = 100000
+    ic = impact(n, 8)
+    out = {}
+    for c in (0, cap):
+        prm = sph.default_params(fixes=sph.SPH_FIX_ALL, precision=1, expansion_order=3, mac_order=2,
+                                 gravity_eta=1e-3, max_unchecked_pointers=8192, near_list_cap=c)
+        ctx = sph.Context(prm, ic, tables256)
+        ctx.upload_tree(orc.build_octree(ic["pos"], prm.max_children_in_leaf, prm.max_tree_depth))
+        ctx.gravity()
+        ctx.gravity()
+        ctx.sync()
+        out[c] = (ctx.download("acc")[:, :3].astype(np.float64), ctx.download("potential").astype(np.float64))
+        ctx.close()
+    a0, p0 = out[0]
+    a1, p1 = out[cap]
+    ea = np.linalg.norm(a1 - a0, axis=1) / np.maximum(np.linalg.norm(a0, axis=1), 1e-30)
+    ep = np.abs(p1 - p0) / np.maximum(np.abs(p0), 1e-30)
+    print(f"[P3 near list cap={cap}] acc rel diff max {ea.max():.3e} q99.9 {np.quantile(ea, 0.999):.3e}; "
+          f"potential max {ep.max():.3e}", flush=True)
+    assert ea.max() < 1e-6 and ep.max() < 1e-6

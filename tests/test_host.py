@@ -61,8 +61,9 @@ def test_default_params_are_the_reference_values(sph):
 
 def test_params_struct_layout_matches_c(sph):
     # 26 four-byte fields, precision, density_lanes, accel_lanes, lane_split, expansion_order, mac_order,
-    # density_flat, density_stop, density_order, list_density_lanes, list_accel_lanes, gather_reach
-    assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 12)
+    # density_flat, density_stop, density_order, list_density_lanes, list_accel_lanes, gather_reach,
+    # near_list_cap
+    assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 13)
 
 
 # ---------------------------------------------------------------- EOS
