@@ -188,7 +188,7 @@ typedef struct sph_eos_table {
 /* Initial conditions in the reference's DataStruct layout (hdfHandler.hpp:15-26). */
 typedef struct sph_ic_view {
     int32_t n;
-    const float* pos4;          /* n x 4 floats */
+    const float* pos4;          /* n x 4 floats; the fourth is ignored (the library keeps the mass there) */
     const float* vel4;          /* n x 4 floats */
     const float* rho;
     const float* u;
@@ -304,7 +304,7 @@ int sph_sort_noalloc(const float* pos4, int32_t n, float cell_width, int32_t cel
 
 /* ------------------------------------------------------------- state access */
 enum sph_field {
-    SPH_F_POS = 0,        /* float4 x n */
+    SPH_F_POS = 0,        /* float4 x n: x, y, z and the particle's mass (kept by the library) */
     SPH_F_VEL,            /* float4 x n */
     SPH_F_ACC,            /* float4 x n */
     SPH_F_RHO,            /* float x n  */

@@ -200,6 +200,14 @@ int ctx_fail_hip(sph_ctx* ctx, hipError_t e, const char* what) {
 }
 
 namespace {
+/* The position record's fourth component holds the particle's mass (the
+ * reference's float3 positions leave it unused), so that the gathers from
+ * original order (cell copies, P2M, pair sums) read one record, not two. */
+__global__ void k_pos_mass(int n, float4* __restrict__ pos, const float* __restrict__ m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pos[i].w = m[i];
+}
+
 hipError_t take_event(sph_ctx* ctx, hipEvent_t* e) {
     if (!ctx->ev_pool.empty()) {
         *e = ctx->ev_pool.back();
@@ -344,6 +352,9 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(hipMemset(ctx->nr_iters, 0, 4 * n));
     TRY(hipMemset(ctx->active, 1, n));
     TRY(hipMemset(ctx->time, 0, 16));
+    k_pos_mass<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, ctx->pos, ctx->m);
+    TRY(hipGetLastError());
+    TRY(hipStreamSynchronize(ctx->stream));
 
     /* EOS tables (Compute.cpp:45-49): slot 0 = material 400 */
     int res = tables[0].res;
@@ -766,6 +777,10 @@ int sph_upload(sph_ctx* ctx, int field, const void* src) {
     ctx->slot_list_valid = false;
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     SPH_HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    if (field == SPH_F_POS || field == SPH_F_MASS) {
+        k_pos_mass<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, ctx->pos, ctx->m);
+        SPH_HIP_TRY(hipGetLastError());
+    }
     return SPH_OK;
 }
 

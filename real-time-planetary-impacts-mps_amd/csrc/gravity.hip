@@ -213,8 +213,8 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
     const float4 x0 = a.prec ? a.pos[pl[0]] : make_float4(0.f, 0.f, 0.f, 0.f);
     for (int k = 0; k < np; ++k) {
         const int p = pl[k];
-        const float pm = a.m[p];
         const float4 xa = a.pos[p];
+        const float pm = xa.w; /* the position record carries the mass */
         const float4 x = make_float4(xa.x - x0.x, xa.y - x0.y, xa.z - x0.z, 0.f);
         const int na = a.next_active[p];
         const bool isActive = a.readme_rule ? (na <= gt) : (gt <= na);
@@ -235,8 +235,8 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
     float o[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int k = 0; k < np; ++k) {
         const int p = pl[k];
-        const float pm = a.m[p];
         const float4 x = a.pos[p];
+        const float pm = x.w;
         const float rx = x.x - px, ry = x.y - py, rz = x.z - pz;
         X -= rx * pm;
         Y -= ry * pm;
@@ -807,7 +807,7 @@ __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp&
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float r = sqrtf(dx * dx + dy * dy + dz * dz);
         if (r == 0.f) continue;
-        const float mj = a.m[pj];
+        const float mj = xj.w;
         const float w = mj * (a.near ? dphi_dr_m2l(r, a.eps) : dphi_dr(r, a.eps, a.fix_soft));
         if (PREC) {
             kadd(ax, cx, -(w * (dx / r)));
@@ -924,7 +924,7 @@ __device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>&
     for (int q = 0; q < ci.x; ++q) {
         const int pj = src[q];
         const float4 xj = a.pos[pj];
-        const float mj = a.m[pj];
+        const float mj = xj.w;
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk) {
             const float dx = T.tx[kk] - xj.x, dy = T.ty[kk] - xj.y, dz = T.tz[kk] - xj.z;

@@ -788,7 +788,7 @@ __global__ __launch_bounds__(256) void k_gather_hydro(int n, const uint2* __rest
     } else {
         x = pos[j];
         v = vel[j];
-        mj = m[j];
+        mj = x.w;
     }
     s_posh[k] = make_float4(x.x, x.y, x.z, h[j]);
     s_velb[k] = make_float4(v.x, v.y, v.z, B[j]);
@@ -848,8 +848,7 @@ __global__ __launch_bounds__(256) void k_refresh_morton(int n, const uint2* __re
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t j = sorted[k].y;
-    const float4 x = pos[j];
-    s_posm[k] = make_float4(x.x, x.y, x.z, m[j]);
+    s_posm[k] = pos[j]; /* w: the mass */
     s_vel[k] = vel[j];
 }
 

@@ -214,7 +214,7 @@ __global__ void k_cells_gather(const uint2* __restrict__ sorted, int n, uint2* _
     const uint32_t j = pr.y;
     const float4 p = pos[j];
     const float4 v = vel[j];
-    s_posm[i] = make_float4(p.x, p.y, p.z, m[j]);
+    s_posm[i] = p; /* w: the mass */
     s_vel[i] = make_float4(v.x, v.y, v.z, 0.f);
 }
 

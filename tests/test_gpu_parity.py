@@ -280,7 +280,12 @@ def test_step_parity(sph, orc, tables256, impact):
     ctx.step()
     st.step()
     for f in ("pos", "vel", "rho", "u", "h"):
-        r = _rel(ctx.download(f), st[f], 1e-12)
+        g, o = ctx.download(f), st[f]
+        if f == "pos":
+            # the position record's fourth component is the library's copy of the mass
+            assert np.array_equal(g[:, 3], ctx.download("mass"))
+            g, o = g[:, :3], o[:, :3]
+        r = _rel(g, o, 1e-12)
         print(f"[step] {f} max rel {r.max():.3e}", flush=True)
         assert r.max() < 1e-5, f
     assert int(ctx.download("global_time")[0]) == int(st["global_time"][0])
