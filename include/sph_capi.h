@@ -165,10 +165,11 @@ typedef struct sph_params {
      * for every particle.  The same results either way. */
     int32_t gather_reach;
     /* Near-field form, leaves in two launches (precision form, P = 3): the
-     * M2L launch records each leaf's rejected source leaves, up to this many
-     * (0..4096), and the pair-sum launch sums over that record instead of
-     * walking the tree again (0, or more leaves: it walks).  The same
-     * pairs; only which lane sums which changes. */
+     * M2L launch records the particles of each leaf's rejected source
+     * leaves, up to this many (0..4096), and the pair-sum launch sums over
+     * that record, a source particle per lane, instead of walking the tree
+     * again (0, or more particles: it walks).  The same pairs; only which
+     * lane sums which changes. */
     int32_t near_list_cap;
 } sph_params;
 /* Relative change of h (2 ulp) at which density_stop = 1 stops whatever k. */

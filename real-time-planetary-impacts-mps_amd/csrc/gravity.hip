@@ -47,9 +47,10 @@ struct Tree {
     float4* oct;             /* 3 per node (P = 3): (XXX XXY XXZ XYY) (XYZ XZZ YYY YYZ) (YZZ ZZZ power3 -) */
     float4* loc3;            /* 3 per node (P = 3): third-order local terms, same order */
     float4* loc_tot;         /* 7 per node: a near-field leaf's complete local expansion (loc, then loc3 layout) */
-    /* near-field leaves in two launches: per node, the rejected source
-     * leaves the M2L launch's walk found, in walk order (near_cap per node),
-     * and their count (-1: not recorded, the pair launch walks again) */
+    /* near-field leaves in two launches: per node, the particles of the
+     * rejected source leaves the M2L launch's walk found, in walk order
+     * (near_cap per node), and their count (-1: not recorded, the pair
+     * launch walks again) */
     int32_t* near_rec;
     int32_t* near_cnt;
     int near_cap;
@@ -916,26 +917,35 @@ struct NearTargets {
     float cx[NEAR_K], cy[NEAR_K], cz[NEAR_K], gp[NEAR_K], cp[NEAR_K];
 };
 
-/* P2P of one rejected source leaf against the targets, with the M2L
- * softening kernel (the near-field form's single pair force). */
+/* The pair terms of one source particle (position record xj, mass in w)
+ * against the targets, with the M2L softening kernel (the near-field form's
+ * single pair force). */
 template <bool PREC, int NEAR_K>
-__device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>& T, int np, int4 ci) {
-    const int32_t* src = a.t.octree + ci.y + 2;
-    for (int q = 0; q < ci.x; ++q) {
-        const int pj = src[q];
-        const float4 xj = a.pos[pj];
+__device__ __forceinline__ void near_pair(const DownArgs& a, NearTargets<NEAR_K>& T, int np, float4 xj) {
+    {
         const float mj = xj.w;
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk) {
             const float dx = T.tx[kk] - xj.x, dy = T.ty[kk] - xj.y, dz = T.tz[kk] - xj.z;
             if constexpr (PREC) {
+                /* correctly rounded square root and one correctly rounded
+                 * reciprocal, products for the rest (instead of a division
+                 * per component and per kernel term) */
                 const float r = sqrtf(dx * dx + dy * dy + dz * dz);
                 if (kk >= np || r == 0.f) continue;
-                const float w = mj * dphi_dr_m2l(r, a.eps);
-                kadd(T.gx[kk], T.cx[kk], -(w * (dx / r)));
-                kadd(T.gy[kk], T.cy[kk], -(w * (dy / r)));
-                kadd(T.gz[kk], T.cz[kk], -(w * (dz / r)));
-                kadd(T.gp[kk], T.cp[kk], mj * pot_m2l(r, a.eps));
+                const float ri = 1.f / r;
+                float wr, pot;
+                if (r >= a.eps) {
+                    wr = mj * (ri * ri) * ri;
+                    pot = ri;
+                } else {
+                    wr = mj * dphi_dr_m2l(r, a.eps) * ri;
+                    pot = pot_m2l(r, a.eps);
+                }
+                kadd(T.gx[kk], T.cx[kk], -(wr * dx));
+                kadd(T.gy[kk], T.cy[kk], -(wr * dy));
+                kadd(T.gz[kk], T.cz[kk], -(wr * dz));
+                kadd(T.gp[kk], T.cp[kk], mj * pot);
             } else {
                 /* FP32 form: the hardware square root and reciprocal (about
                  * 1 ulp), as in M2L; the precision form keeps the correctly
@@ -950,6 +960,13 @@ __device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>&
             }
         }
     }
+}
+
+/* P2P of one rejected source leaf against the targets. */
+template <bool PREC, int NEAR_K>
+__device__ __forceinline__ void near_p2p(const DownArgs& a, NearTargets<NEAR_K>& T, int np, int4 ci) {
+    const int32_t* src = a.t.octree + ci.y + 2;
+    for (int q = 0; q < ci.x; ++q) near_pair<PREC, NEAR_K>(a, T, np, a.pos[src[q]]);
 }
 
 /* A near-field leaf's complete local expansion, between its two launches. */
@@ -1041,18 +1058,18 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             T.cx[kk] = T.cy[kk] = T.cz[kk] = T.gp[kk] = T.cp[kk] = 0.f;
         }
         const bool do_m2l = PART != 2 && k0 == 0;
-        /* PART 2 with the M2L launch's record of the rejected source leaves:
-         * the pair sums over that list, G-th entries per lane, no walk */
+        /* PART 2 with the M2L launch's record of the rejected source leaves'
+         * particles: every lane takes every GS-th source particle, no walk */
         if constexpr (PART == 2) {
             const int rc = a.t.near_cap > 0 ? a.t.near_cnt[id] : -1;
             if (rc >= 0) {
                 const int32_t* rec = a.t.near_rec + (int64_t)id * a.t.near_cap;
-                for (int e = lane; e < rc; e += GS) near_p2p<PREC, NEAR_K>(a, T, nk, a.t.info[rec[e]]);
+                for (int e = lane; e < rc; e += GS) near_pair<PREC, NEAR_K>(a, T, nk, a.pos[rec[e]]);
                 goto reduce;
             }
         }
         {
-        int nrec = 0;          /* PART 1: leaves recorded so far (group-uniform) */
+        int nrec = 0;          /* PART 1: source particles recorded so far (group-uniform) */
         bool rec_off = false;  /* PART 1: some lane walked a subtree alone (unrecorded order) */
         int lp = 0;            /* next list index (group-uniform, as are the rest) */
         bool list_open = true;
@@ -1076,6 +1093,7 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             if (nfill + take == 0 && !list_open) break;
             int nch = 0;
             bool near_leaf = false;
+            int4 near_ci = make_int4(0, 0, 0, 0);
             if (item >= 0 && item != id) {
                 const float4 b0 = a.t.far[3 * item], b1 = a.t.far[3 * item + 1], b2 = a.t.far[3 * item + 2];
                 if (mac_ok(a, A, item, b0, b1, b2)) {
@@ -1087,6 +1105,7 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
                     const int4 ci = a.t.info[item];
                     if (ci.x != 0) {
                         near_leaf = true;
+                        near_ci = ci;
                         if (PART != 1) near_p2p<PREC, NEAR_K>(a, T, nk, ci);
                     } else {
 #pragma unroll
@@ -1096,13 +1115,21 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             }
             if constexpr (PART == 1) {
                 if (a.t.near_cap > 0) {
-                    /* record the round's rejected leaves in lane order */
-                    const uint64_t nm = __ballot(near_leaf) & gmask;
-                    if (near_leaf) {
-                        const int pos = nrec + __popcll(nm & ltmask);
-                        if (pos < a.t.near_cap) a.t.near_rec[(int64_t)id * a.t.near_cap + pos] = item;
+                    /* record the round's rejected leaves' particles, leaves in
+                     * lane order (a group prefix sum of their counts) */
+                    const int cntp = near_leaf ? near_ci.x : 0;
+                    int incl = cntp;
+#pragma unroll
+                    for (int off = 1; off < GS; off <<= 1) {
+                        const int v = __shfl_up(incl, off, GS);
+                        if (lane >= off) incl += v;
                     }
-                    nrec += __popcll(nm);
+                    const int total = __shfl(incl, GS - 1, GS);
+                    const int pos0 = nrec + incl - cntp;
+                    int32_t* rec = a.t.near_rec + (int64_t)id * a.t.near_cap;
+                    const int32_t* src = a.t.octree + near_ci.y + 2;
+                    for (int q = 0; q < cntp && pos0 + q < a.t.near_cap; ++q) rec[pos0 + q] = src[q];
+                    nrec += total;
                 }
             }
             /* group prefix sum of the child counts (0..8) from four bit ballots */

@@ -53,7 +53,7 @@ extern "C" void sph_params_default(sph_params* o) {
     o->list_accel_lanes = 16;
     o->gather_reach = 1;
     /* measured at 4e6 (head-on, learned-h + precision, P = 3, 100 steps):
-     * gravity down pass 3.90 ms walking twice, 3.40 / 3.42 / 3.50 ms with
-     * records of 128 / 512 / 2048 leaves */
-    o->near_list_cap = 128;
+     * gravity down pass 3.90 ms walking twice, 3.49 / 3.43 / 3.49 / 3.44 ms
+     * with records of 256 / 512 / 1024 / 2048 source particles */
+    o->near_list_cap = 512;
 }
