@@ -1386,8 +1386,12 @@ __device__ __forceinline__ void down_group_run(const DownArgs& a) {
     }
 }
 
+/* Occupancy asked of the compiler, measured on MI355X: the as-written
+ * second-order form at 6 waves per SIMD (95 -> 80 VGPRs, 36 bytes of
+ * spills, +2% steps/s at 1e6), the third-order precision branches at 4
+ * (130 -> 128 VGPRs, no spills). */
 template <int GS, bool NEAR, bool PREC, int P, int MODE>
-__global__ __launch_bounds__(256) void k_down_group(DownArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!NEAR && P == 2) ? 6 : ((PREC && P == 3) ? 4 : 1)))) void k_down_group(DownArgs a) {
     down_group_run<GS, NEAR, PREC, P, MODE>(a);
 }
 
@@ -1415,9 +1419,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     down_group_run<GS, true, PREC, P, 2>(a);
 }
 
-/* The near-field leaves of a level, pair sums and L2P (after k_down_leaves). */
+/* The near-field leaves of a level, pair sums and L2P (after k_down_leaves);
+ * the third-order precision form at 3 waves per SIMD (174 -> 168 VGPRs, no
+ * spills). */
 template <int GS, bool PREC, int P>
-__global__ __launch_bounds__(256) void k_down_leaves_p2p(DownArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREC && P == 3 ? 3 : 1))) void k_down_leaves_p2p(DownArgs a) {
     down_group_run<GS, true, PREC, P, 3>(a);
 }
 
