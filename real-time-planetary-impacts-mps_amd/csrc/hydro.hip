@@ -894,7 +894,11 @@ __device__ __forceinline__ int wave_min(int v) {
 
 /* accelerations.metal:12-141 */
 /* One particle (sorted slot `index`, -1 for none); returns its tick step
- * (INT_MAX for none or an inactive particle). */
+ * (INT_MAX for none or an inactive particle).  The pair term takes the
+ * hardware square root and reciprocals (about 1 ulp, as under the
+ * reference's Metal fast-math defaults) for r, 1/r, 1/h_j, 1/rho_ij and
+ * 1/rho_j instead of correctly rounded sequences: the pass is bound by
+ * VALU issue, and those sequences were a quarter of its pair term. */
 __device__ __forceinline__ int accel_particle(const AccelArgs& a, int index) {
     int integerDt = 0x7FFFFFFF;
     if (index >= 0 && index < a.n) {
@@ -928,15 +932,15 @@ __device__ __forceinline__ int accel_particle(const AccelArgs& a, int index) {
                             const float h_j = xj4.w;
                             const float hj2x4 = h_j * h_j * 4.f;
                             if ((r2 > hi2x4 && r2 > hj2x4) || r2 < 1e-12f) continue;
-                            const float r = sqrtf(r2);
-                            const float r1 = 1.f / r;
+                            const float r = __builtin_amdgcn_sqrtf(r2);
+                            const float r1 = __builtin_amdgcn_rcpf(r);
                             const float4 hj = a.s_hyd[k];
                             const float4 vj4 = a.s_velb[k];
                             const float m_j = hj.x;
                             const float fact_j = hj.y;
                             const float rho_j = hj.z;
                             const float c_j = hj.w;
-                            const float hj1 = 1.f / h_j;
+                            const float hj1 = __builtin_amdgcn_rcpf(h_j);
                             const float vx = vi4.x - vj4.x, vy = vi4.y - vj4.y, vz = vi4.z - vj4.z;
                             const float rho_ij = 0.5f * (rho_i + rho_j);
                             const float v_dot_r = vx * dx + vy * dy + vz * dz;
@@ -944,7 +948,7 @@ __device__ __forceinline__ int accel_particle(const AccelArgs& a, int index) {
                             const float v_sigij = c_i + c_j - 3.f * mu_ij;
                             v_sigi = fmaxf(v_sigij, v_sigi);
                             const float B_ij = 0.5f * (B_i + vj4.w);
-                            const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij / rho_ij;
+                            const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij * __builtin_amdgcn_rcpf(rho_ij);
                             const float rx = dx * r1, ry = dy * r1, rz = dz * r1;
                             const float gi = kGradWFactor(r, hi1), gj = kGradWFactor(r, hj1);
                             const float gix = gi * rx, giy = gi * ry, giz = gi * rz;
@@ -956,7 +960,7 @@ __device__ __forceinline__ int accel_particle(const AccelArgs& a, int index) {
                             const float vgi = vx * gix + vy * giy + vz * giz;
                             const float vgm = vx * gmx + vy * gmy + vz * gmz;
                             du_dt += m_j * (fact_i * vgi + 0.5f * nu_ij * vgm);
-                            dhdt += (m_j / rho_j) * vgi;
+                            dhdt += (m_j * __builtin_amdgcn_rcpf(rho_j)) * vgi;
                         }
                     }
                 }
@@ -1044,15 +1048,15 @@ __device__ __forceinline__ int accel_group_particle(const AccelArgs& a, int inde
                 const float h_j = xj4.w;
                 const float hj2x4 = h_j * h_j * 4.f;
                 if ((r2 > hi2x4 && r2 > hj2x4) || r2 < 1e-12f) continue;
-                const float r = sqrtf(r2);
-                const float r1 = 1.f / r;
+                const float r = __builtin_amdgcn_sqrtf(r2);
+                const float r1 = __builtin_amdgcn_rcpf(r);
                 const float4 hj = a.s_hyd[k];
                 const float4 vj4 = a.s_velb[k];
                 const float m_j = hj.x;
                 const float fact_j = hj.y;
                 const float rho_j = hj.z;
                 const float c_j = hj.w;
-                const float hj1 = 1.f / h_j;
+                const float hj1 = __builtin_amdgcn_rcpf(h_j);
                 const float vx = vi4.x - vj4.x, vy = vi4.y - vj4.y, vz = vi4.z - vj4.z;
                 const float rho_ij = 0.5f * (rho_i + rho_j);
                 const float v_dot_r = vx * dx + vy * dy + vz * dz;
@@ -1060,7 +1064,7 @@ __device__ __forceinline__ int accel_group_particle(const AccelArgs& a, int inde
                 const float v_sigij = c_i + c_j - 3.f * mu_ij;
                 v_sigi = fmaxf(v_sigij, v_sigi);
                 const float B_ij = 0.5f * (B_i + vj4.w);
-                const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij / rho_ij;
+                const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij * __builtin_amdgcn_rcpf(rho_ij);
                 const float rx = dx * r1, ry = dy * r1, rz = dz * r1;
                 const float gi = kGradWFactor(r, hi1), gj = kGradWFactor(r, hj1);
                 const float gix = gi * rx, giy = gi * ry, giz = gi * rz;
@@ -1072,7 +1076,7 @@ __device__ __forceinline__ int accel_group_particle(const AccelArgs& a, int inde
                 const float vgi = vx * gix + vy * giy + vz * giz;
                 const float vgm = vx * gmx + vy * gmy + vz * gmz;
                 du_dt += m_j * (fact_i * vgi + 0.5f * nu_ij * vgm);
-                dhdt += (m_j / rho_j) * vgi;
+                dhdt += (m_j * __builtin_amdgcn_rcpf(rho_j)) * vgi;
             }
         }
         dvx = gsum<G>(dvx);
