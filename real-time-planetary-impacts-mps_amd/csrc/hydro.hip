@@ -504,8 +504,10 @@ __device__ __forceinline__ void box_slots(const int mn[3], const int mx[3], int 
     });
 }
 
-/* One particle (sorted slot `index`) on its G lanes (`lane` = 0..G-1). */
-template <int G, int SPLIT, int FLAT>
+/* One particle (sorted slot `index`) on its G lanes (`lane` = 0..G-1).
+ * COUNT: also count the work (SPH_F_DENSITY_WORK); the pass is bound by
+ * VALU issue, so the production kernel leaves those counts out. */
+template <int G, int SPLIT, int FLAT, bool COUNT>
 __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int index, int lane) {
     if (index == 0 && lane == 0) a.time[1] = a.dt_reset;
     const uint32_t ind = a.sorted[index].y;
@@ -554,13 +556,13 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
         int mn[3], mx[3];
         cell_box(position, cs, hb, mask + 1, mn, mx);
         float part = 0.f;
-        int cnt = 0;
+        int cnt = 0; /* candidates inside the box radius (the NR neighbour count without the cache) */
         /* the candidate body without branches: a wave of 64 lanes almost
          * always holds a lane whose candidate is inside 2h, so the kernel
          * term costs the same either way, and selects leave out the exec-mask
          * bookkeeping of the nested tests (same terms, same order) */
         const float fac = kPi1 * h1 * h1 * h1;
-        box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, [&](uint32_t nc) { w_cand += (int)nc; },
+        box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, [&](uint32_t nc) { if constexpr (COUNT) w_cand += (int)nc; },
                                   [&](uint32_t k) {
             const float4 q = spm[k];
             const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
@@ -568,11 +570,11 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
             const bool in = !(r2 > hb2);
             const float r = __builtin_amdgcn_sqrtf(r2);
             const bool take = in && !(use_cache && r > hsum);
-            w_in += in ? 1 : 0;
-            cnt += (in && !use_cache) ? 1 : 0;
+            cnt += in ? 1 : 0;
             const float w = kW_sel(r * h1, fac);
             part += take ? w * q.w : 0.f;
         });
+        if constexpr (COUNT) w_in += cnt;
         density = gsum<G>(part);
         if (!use_cache) {
             nN = min(gsum_i<G>(cnt), a.ne);
@@ -615,13 +617,13 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
         const float hsum = h_i + h_i;
         int mn[3], mx[3];
         cell_box(position, cs, hb, mask + 1, mn, mx);
-        box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, [&](uint32_t nc) { f_cand += (int)nc; },
+        box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, [&](uint32_t nc) { if constexpr (COUNT) f_cand += (int)nc; },
                                   [&](uint32_t k) {
             const float4 q = spm[k];
             const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (r2 > hb2) return;
-            f_in++;
+            if constexpr (COUNT) f_in++;
             const float r = __builtin_amdgcn_sqrtf(r2);
             if (cached && r > hsum) return;
             const float mass = q.w;
@@ -647,10 +649,12 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
     cx = gsum<G>(cx);
     cy = gsum<G>(cy);
     cz = gsum<G>(cz);
-    w_cand = gsum_i<G>(w_cand);
-    w_in = gsum_i<G>(w_in);
-    f_cand = gsum_i<G>(f_cand);
-    f_in = gsum_i<G>(f_in);
+    if constexpr (COUNT) {
+        w_cand = gsum_i<G>(w_cand);
+        w_in = gsum_i<G>(w_in);
+        f_cand = gsum_i<G>(f_cand);
+        f_in = gsum_i<G>(f_in);
+    }
     if (lane != 0) return;
     a.nr_iters[ind] = count;
     velDiv = fabsf(velDiv) / density;
@@ -664,7 +668,7 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
     a.P[ind] = pc.x;
     a.c[ind] = pc.y;
     a.B[ind] = bal;
-    if (a.work) a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
+    if (COUNT && a.work) a.work[ind] = make_int4(w_cand, w_in, f_cand, f_in);
 }
 
 /* Every slot: one particle per group of G lanes, workgroups XCD-remapped.
@@ -698,7 +702,7 @@ __global__ __launch_bounds__(W) void k_nr_order(int n, const uint2* __restrict__
  * (only which lanes take a particle changes, not its result). */
 /* The list form at 7 waves per SIMD (79 -> 72 VGPRs, no spills): density
  * pass 0.70 -> 0.69 ms at 4e6 with individual timesteps. */
-template <int G, int SPLIT, int FLAT, bool LIST>
+template <int G, int SPLIT, int FLAT, bool LIST, bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIST ? 7 : 1))) void k_density_group(DensityArgs a) {
     constexpr int PPB = 256 / G;
     const int lane = (int)threadIdx.x % G, g = (int)threadIdx.x / G;
@@ -706,7 +710,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIST ? 7 : 
         const int blk = xcd_block((a.n + PPB - 1) / PPB);
         if (blk < 0) return;
         const int k = blk * PPB + g;
-        if (k < a.n) density_group_particle<G, SPLIT, FLAT>(a, a.order ? a.order[k] : k, lane);
+        if (k < a.n) density_group_particle<G, SPLIT, FLAT, COUNT>(a, a.order ? a.order[k] : k, lane);
         return;
     }
     const int total = *a.nslots;
@@ -717,7 +721,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIST ? 7 : 
         const int lb = xcd * per + k;
         if (lb >= nlog) break;
         const int e = lb * PPB + g;
-        if (e < total) density_group_particle<G, SPLIT, FLAT>(a, a.slots[e], lane);
+        if (e < total) density_group_particle<G, SPLIT, FLAT, COUNT>(a, a.slots[e], lane);
     }
 }
 
@@ -1318,13 +1322,20 @@ int launch_density(sph_ctx* ctx) {
     const int dlanes = a.slots ? p.list_density_lanes : p.density_lanes;
     const int dsel = (dlanes * 2 + (p.lane_split ? 1 : 0)) * 2 + flat;
     /* over a slot list the grid is persistent: at most kListBlocks workgroups */
-#define SPH_DENS(G, S, F, PPB)                                                                           \
+#define SPH_DENS_C(G, S, F, PPB, C)                                                                      \
     if (a.slots)                                                                                         \
-        k_density_group<G, S, F, true>                                                                   \
+        k_density_group<G, S, F, true, C>                                                                \
             <<<std::min(xcd_grid((ctx->n + (PPB) - 1) / (PPB)), (unsigned)kListBlocks), threads, 0,     \
                ctx->stream>>>(a);                                                                        \
     else                                                                                                 \
-        k_density_group<G, S, F, false><<<xcd_grid((ctx->n + (PPB) - 1) / (PPB)), threads, 0, ctx->stream>>>(a)
+        k_density_group<G, S, F, false, C><<<xcd_grid((ctx->n + (PPB) - 1) / (PPB)), threads, 0, ctx->stream>>>(a)
+    /* the work-counting form only while sph_set_work_counters is on */
+#define SPH_DENS(G, S, F, PPB)                                                                           \
+    if (a.work) {                                                                                        \
+        SPH_DENS_C(G, S, F, PPB, true);                                                                  \
+    } else {                                                                                             \
+        SPH_DENS_C(G, S, F, PPB, false);                                                                 \
+    }
     switch (dsel) {
         case 8: SPH_DENS(2, 0, 0, 128); break;
         case 9: SPH_DENS(2, 0, 1, 128); break;
@@ -1345,6 +1356,7 @@ int launch_density(sph_ctx* ctx) {
         default: k_density<<<xcd_grid((ctx->n + threads - 1) / threads), threads, 0, ctx->stream>>>(a); break;
     }
 #undef SPH_DENS
+#undef SPH_DENS_C
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }
