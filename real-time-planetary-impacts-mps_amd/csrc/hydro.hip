@@ -95,6 +95,32 @@ __device__ __forceinline__ float kdWdh(float r, float h1) {
     return fac * (1.f + 0.5f * (3.f * q - 5.f) * q * q);
 }
 
+/* The cubic spline of kernels.h:35-85 through its truncated powers
+ * a = (2 - q)+ and b = (1 - q)+:  w(q) = a^3/4 - b^3,  w'(q) = 3 b^2 -
+ * 3 a^2/4  and  w + q w'/3 = a^2 (a - q)/4 - b^2 (b - q).  These are the
+ * reference's piecewise polynomials (each is 0 beyond q = 2) written without
+ * branches or selects: the neighbour passes are bound by VALU issue, and the
+ * pieces share a and b.  They differ from the reference's nested forms in
+ * rounding only. */
+__device__ __forceinline__ float kW_tp(float q, float fac) {
+    const float a = fmaxf(2.f - q, 0.f), b = fmaxf(1.f - q, 0.f);
+    return fac * (0.25f * a * a * a - b * b * b);
+}
+/* kernels.h:52-69: gradW = g x_ij / r with g = h^-4 / pi * w'(q) */
+__device__ __forceinline__ float kGradWFactor_tp(float r, float h1) {
+    const float q = r * h1;
+    const float a = fmaxf(2.f - q, 0.f), b = fmaxf(1.f - q, 0.f);
+    const float fac = kPi1 * h1 * h1 * h1 * h1;
+    return fac * (3.f * b * b - 0.75f * a * a);
+}
+/* kernels.h:71-85: dW/dh = -3 h^-4 / pi (w + q w'/3) */
+__device__ __forceinline__ float kdWdh_tp(float r, float h1) {
+    const float q = r * h1;
+    const float a = fmaxf(2.f - q, 0.f), b = fmaxf(1.f - q, 0.f);
+    const float fac = -3.f * kPi1 * h1 * h1 * h1 * h1;
+    return fac * (0.25f * a * a * (a - q) - b * b * (b - q));
+}
+
 /* Bilinear sample with Metal's texture conventions (density.metal:45-50):
  * normalised coordinates, texel centres at (i + 1/2)/res, clamp to edge.
  * Column = internal energy axis, row = density axis. */
@@ -571,7 +597,7 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
             const float r = __builtin_amdgcn_sqrtf(r2);
             const bool take = in && !(use_cache && r > hsum);
             cnt += in ? 1 : 0;
-            const float w = kW_sel(r * h1, fac);
+            const float w = kW_tp(r * h1, fac);
             part += take ? w * q.w : 0.f;
         });
         if constexpr (COUNT) w_in += cnt;
@@ -627,13 +653,13 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
             const float r = __builtin_amdgcn_sqrtf(r2);
             if (cached && r > hsum) return;
             const float mass = q.w;
-            if (!cached) dpart += kW(r, h1) * mass;
-            density_h += mass * kdWdh(r, h1);
+            if (!cached) dpart += kW_tp(r * h1, kPi1 * h1 * h1 * h1) * mass;
+            density_h += mass * kdWdh_tp(r, h1);
             if (r > 1e-12f) {
                 const float r1 = __builtin_amdgcn_rcpf(r);
                 const float4 vj = a.s_vel[k];
                 const float vx = vj.x - vi4.x, vy = vj.y - vi4.y, vz = vj.z - vi4.z;
-                const float g = kGradWFactor(r, h1);
+                const float g = kGradWFactor_tp(r, h1);
                 const float gx = g * (dx * r1), gy = g * (dy * r1), gz = g * (dz * r1);
                 velDiv += mass * (vx * gx + vy * gy + vz * gz);
                 cx += mass * (vy * gz - vz * gy);
@@ -954,7 +980,7 @@ __device__ __forceinline__ int accel_particle(const AccelArgs& a, int index) {
                             const float B_ij = 0.5f * (B_i + vj4.w);
                             const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij * __builtin_amdgcn_rcpf(rho_ij);
                             const float rx = dx * r1, ry = dy * r1, rz = dz * r1;
-                            const float gi = kGradWFactor(r, hi1), gj = kGradWFactor(r, hj1);
+                            const float gi = kGradWFactor_tp(r, hi1), gj = kGradWFactor_tp(r, hj1);
                             const float gix = gi * rx, giy = gi * ry, giz = gi * rz;
                             const float gjx = gj * rx, gjy = gj * ry, gjz = gj * rz;
                             const float gmx = 0.5f * (gix + gjx), gmy = 0.5f * (giy + gjy), gmz = 0.5f * (giz + gjz);
@@ -1070,7 +1096,7 @@ __device__ __forceinline__ int accel_group_particle(const AccelArgs& a, int inde
                 const float B_ij = 0.5f * (B_i + vj4.w);
                 const float nu_ij = -0.5f * a.alpha * B_ij * mu_ij * v_sigij * __builtin_amdgcn_rcpf(rho_ij);
                 const float rx = dx * r1, ry = dy * r1, rz = dz * r1;
-                const float gi = kGradWFactor(r, hi1), gj = kGradWFactor(r, hj1);
+                const float gi = kGradWFactor_tp(r, hi1), gj = kGradWFactor_tp(r, hj1);
                 const float gix = gi * rx, giy = gi * ry, giz = gi * rz;
                 const float gjx = gj * rx, gjy = gj * ry, gjz = gj * rz;
                 const float gmx = 0.5f * (gix + gjx), gmy = 0.5f * (giy + gjy), gmz = 0.5f * (giz + gjz);
