@@ -588,18 +588,35 @@ __device__ __forceinline__ void density_group_particle(const DensityArgs& a, int
          * term costs the same either way, and selects leave out the exec-mask
          * bookkeeping of the nested tests (same terms, same order) */
         const float fac = kPi1 * h1 * h1 * h1;
-        box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, [&](uint32_t nc) { if constexpr (COUNT) w_cand += (int)nc; },
-                                  [&](uint32_t k) {
-            const float4 q = spm[k];
-            const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const bool in = !(r2 > hb2);
-            const float r = __builtin_amdgcn_sqrtf(r2);
-            const bool take = in && !(use_cache && r > hsum);
-            cnt += in ? 1 : 0;
-            const float w = kW_tp(r * h1, fac);
-            part += take ? w * q.w : 0.f;
-        });
+        auto on_cell = [&](uint32_t nc) { if constexpr (COUNT) w_cand += (int)nc; };
+        if (!use_cache) {
+            /* the box of this h: w(q) is 0 beyond 2h, so the kernel sum needs
+             * no test (a candidate beyond 2h adds 0, or a few ulp of a term
+             * where rounding puts q just under 2), and the normalisation
+             * h^-3 / pi is applied once to the sum */
+            box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, on_cell, [&](uint32_t k) {
+                const float4 q = spm[k];
+                const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                cnt += r2 > hb2 ? 0 : 1;
+                const float r = __builtin_amdgcn_sqrtf(r2);
+                const float ta = fmaxf(fmaf(-h1, r, 2.f), 0.f), tb = fmaxf(fmaf(-h1, r, 1.f), 0.f);
+                part = fmaf(0.25f * ta * ta * ta - tb * tb * tb, q.w, part);
+            });
+            part *= fac;
+        } else {
+            box_slots<G, SPLIT, FLAT>(mn, mx, lane, mask, cse, on_cell, [&](uint32_t k) {
+                const float4 q = spm[k];
+                const float dx = q.x - position.x, dy = q.y - position.y, dz = q.z - position.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const bool in = !(r2 > hb2);
+                const float r = __builtin_amdgcn_sqrtf(r2);
+                const bool take = in && !(r > hsum);
+                cnt += in ? 1 : 0;
+                const float w = kW_tp(r * h1, fac);
+                part += take ? w * q.w : 0.f;
+            });
+        }
         if constexpr (COUNT) w_in += cnt;
         density = gsum<G>(part);
         if (!use_cache) {
