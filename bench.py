@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--c1-steps", type=int, default=100,
                     help="steps of the 2e4-particle run on the CPU path and on the GPU (0 = skip)")
     ap.add_argument("--no-timing", action="store_true", help="do not record per-pass HIP events")
+    ap.add_argument("--density-probe", action="store_true",
+                    help="after the warm-up steps, one sort and two density passes back to back, then exit "
+                         "(for profiler counters of a density launch that follows another)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes 5 times (calibrates the "
                          "HBM byte counters of a profiler against a kernel of known traffic) and exit")
@@ -459,6 +462,13 @@ def main():
         hl.step()
         log(f"warmup step {i} {1000 * (time.perf_counter() - t_w):.1f} ms")
     ctx.sync()
+    if args.density_probe:
+        ctx.sort()
+        ctx.density()
+        ctx.density()
+        ctx.sync()
+        print(json.dumps({"density_probe": args.n, "launches": 2}))
+        return
     trees0, _ = hl.tree_stats()
     if not args.no_timing:
         ctx.reset_pass_times()
@@ -515,8 +525,8 @@ def main():
                                "that overlap, so the passes do not add up to ms_per_step")
         out["density_ms"] = round(per_pass["density"], 4)
         # dominant kernel: the density pass is one kernel
-        kname = (f"k_density_group<{int(prm.density_lanes)}, {int(prm.lane_split)}, {int(prm.density_flat)}>"
-                 if prm.density_lanes > 0 else "k_density")
+        kname = (f"k_density_group<{int(prm.density_lanes)}, {int(prm.lane_split)}, {int(prm.density_flat)}, "
+                 "false, false>" if prm.density_lanes > 0 else "k_density")
         # the reference algorithm's work, counted by one more sort + density
         # pass of the state after the timed steps (untimed: counting is a
         # measurement aid)
