@@ -972,8 +972,12 @@ __device__ __forceinline__ int accel_particle(const AccelArgs& a, int index) {
                         const uint32_t key = exy | (dev_expand_bits((uint32_t)(z & mask)) << 2);
                         const uint2 se = cse[key];
                         if (se.x == 0xFFFFFFFFu) continue;
+                        float4 nxt = a.s_posh[se.x];
                         for (uint32_t k = se.x; k <= se.y; k++) {
-                            const float4 xj4 = a.s_posh[k];
+                            /* the next candidate's record is requested before this
+                             * one is used, so its latency overlaps this step */
+                            const float4 xj4 = nxt;
+                            nxt = a.s_posh[min(k + 1u, se.y)];
                             const float dx = xi4.x - xj4.x, dy = xi4.y - xj4.y, dz = xi4.z - xj4.z;
                             const float r2 = dx * dx + dy * dy + dz * dz;
                             const float h_j = xj4.w;
@@ -1033,7 +1037,8 @@ __device__ __forceinline__ int accel_particle(const AccelArgs& a, int index) {
  * rule) a persistent grid over the list, XCD x taking its x-th eighth.  The
  * global timestep is the minimum of the tick steps: wave minimum, then one
  * atomic per wave. */
-__global__ __launch_bounds__(256) void k_acceleration(AccelArgs a) {
+/* 8 waves per SIMD (64 VGPRs, no spills) with the candidate prefetch */
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_acceleration(AccelArgs a) {
     int integerDt = 0x7FFFFFFF;
     if (!a.slots) {
         const int blk = xcd_block((a.n + 255) / 256);
