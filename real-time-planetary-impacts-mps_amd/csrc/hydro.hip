@@ -1093,8 +1093,14 @@ __device__ __forceinline__ int accel_group_particle(const AccelArgs& a, int inde
                                  (dev_expand_bits((uint32_t)((mn[2] + iz) & mask)) << 2);
             const uint2 se = cse[key];
             if (se.x == 0xFFFFFFFFu) continue;
-            for (uint32_t k = se.x + (SPLIT ? lane : 0); k <= se.y; k += SPLIT ? G : 1) {
-                const float4 xj4 = a.s_posh[k];
+            constexpr uint32_t STEP = SPLIT ? G : 1;
+            uint32_t k = se.x + (SPLIT ? lane : 0);
+            if (k > se.y) continue;
+            float4 nxt = a.s_posh[k];
+            for (; k <= se.y; k += STEP) {
+                /* the next candidate's record is requested before this one is used */
+                const float4 xj4 = nxt;
+                nxt = a.s_posh[min(k + STEP, se.y)];
                 const float dx = xi4.x - xj4.x, dy = xi4.y - xj4.y, dz = xi4.z - xj4.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
                 const float h_j = xj4.w;
