@@ -121,6 +121,8 @@ void destroy_ctx(sph_ctx* ctx) {
     dfree(ctx->snapshot_img);
     if (ctx->snap_done) (void)hipEventDestroy(ctx->snap_done);
     if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
+    if (ctx->tree_free) (void)hipEventDestroy(ctx->tree_free);
+    if (ctx->convert_done) (void)hipEventDestroy(ctx->convert_done);
     for (auto& t : ctx->ev_pending) {
         (void)hipEventDestroy(t.start);
         (void)hipEventDestroy(t.stop);
@@ -245,6 +247,13 @@ int join_gravity(sph_ctx* ctx) {
     return SPH_OK;
 }
 
+int join_tree(sph_ctx* ctx) {
+    if (!ctx->convert_pending) return SPH_OK;
+    ctx->convert_pending = false;
+    SPH_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->convert_done, 0));
+    return SPH_OK;
+}
+
 int ctx_fold_times(sph_ctx* ctx) {
     SPH_TRY_RC(join_gravity(ctx));
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -295,6 +304,8 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(hipEventCreateWithFlags(&ctx->ev_down, hipEventDisableTiming));
     TRY(hipEventCreateWithFlags(&ctx->snap_done, hipEventDisableTiming));
     TRY(hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming));
+    TRY(hipEventCreateWithFlags(&ctx->tree_free, hipEventDisableTiming));
+    TRY(hipEventCreateWithFlags(&ctx->convert_done, hipEventDisableTiming));
     for (auto& e : ctx->tree.released) TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     TRY(dalloc(&ctx->pos, n));
@@ -508,7 +519,8 @@ int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int 
     return SPH_OK;
 }
 
-int tree_commit(sph_ctx* ctx, int k, int64_t n_ints, const int32_t* level_counts, int n_levels, int32_t node_count) {
+int tree_commit(sph_ctx* ctx, int k, int64_t n_ints, const int32_t* level_counts, int n_levels, int32_t node_count,
+                hipStream_t convert_stream) {
     sph_tree_dev& T = ctx->tree;
     int32_t max_nonlast = 0;
     for (int l = 0; l + 1 < n_levels; ++l) max_nonlast = std::max(max_nonlast, level_counts[l]);
@@ -523,7 +535,7 @@ int tree_commit(sph_ctx* ctx, int k, int64_t n_ints, const int32_t* level_counts
     T.level_counts.assign(level_counts, level_counts + n_levels);
     T.level_offsets.assign(n_levels, 0);
     for (int l = 1; l < n_levels; ++l) T.level_offsets[l] = T.level_offsets[l - 1] + level_counts[l - 1];
-    int rc = launch_tree_convert(ctx);
+    int rc = launch_tree_convert(ctx, convert_stream);
     if (rc != SPH_OK) return rc;
     ctx->have_tree = true;
     ctx->tree_check_pending = true;
@@ -547,8 +559,15 @@ int sph_upload_tree(sph_ctx* ctx, const int32_t* octree, int64_t n_ints, const i
     SPH_HIP_TRY(hipMemcpyAsync(T.level_tp_up[k], level_nodes, 4 * (size_t)node_count, hipMemcpyHostToDevice,
                                ctx->copy_stream));
     SPH_HIP_TRY(hipEventRecord(ctx->upload_done, ctx->copy_stream));
-    SPH_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->upload_done, 0));
-    SPH_TRY_RC(tree_commit(ctx, k, n_ints, level_counts, n_levels, node_count));
+    /* The conversion rewrites the node records, which every pass enqueued so
+     * far may read: it runs on the copy stream after them, and the compute
+     * stream waits for it only where the tree is next read (join_tree), so
+     * that work that does not read it (the sort) overlaps the upload. */
+    SPH_HIP_TRY(hipEventRecord(ctx->tree_free, ctx->stream));
+    SPH_HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->tree_free, 0));
+    SPH_TRY_RC(tree_commit(ctx, k, n_ints, level_counts, n_levels, node_count, ctx->copy_stream));
+    SPH_HIP_TRY(hipEventRecord(ctx->convert_done, ctx->copy_stream));
+    ctx->convert_pending = true;
     ctx->upload_pending = true;
     return SPH_OK;
 }
@@ -580,6 +599,7 @@ int sph_gravity(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
     if (!ctx->have_tree) return ctx_fail(ctx, SPH_E_NOTREE, "gravity before any tree upload");
     SPH_TRY_RC(join_gravity(ctx));
+    SPH_TRY_RC(join_tree(ctx));
     SPH_TRY_RC(fold_dv(ctx));
     /* the near-field form checks the node numbering in the up pass */
     if (ctx->prm.fixes & SPH_FIX_NEAR_FIELD) ctx->tree_check_pending = true;
@@ -650,6 +670,7 @@ int sph_step(sph_ctx* ctx) {
 int sph_sync(sph_ctx* ctx) {
     if (!ctx) return SPH_E_ARG;
     SPH_TRY_RC(join_gravity(ctx));
+    SPH_TRY_RC(join_tree(ctx)); /* the malformed-tree flag below */
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     SPH_HIP_TRY(hipGetLastError());
     if (ctx->tree_check_pending) {

@@ -65,10 +65,14 @@ __global__ void k_fill_i32(int32_t* __restrict__ a, int64_t count, int32_t v) {
 struct ConvertArgs {
     const int32_t* octree;
     int64_t n_ints;
-    const int32_t* level_tp;
+    const int32_t* level_tp; /* all level lists, level by level */
     int32_t* level_id;
-    int count;
-    int level;
+    /* level lists first_level .. first_level + chunk_levels - 1 of this
+     * launch: slot s belongs to the level l with off[l] <= s < off[l + 1]
+     * (l relative to first_level) */
+    int32_t off[kMaxLevels + 1];
+    int first_level;
+    int chunk_levels;
     int n_levels;
     int node_count;
     int n_particles;
@@ -80,26 +84,42 @@ struct ConvertArgs {
     uint32_t* flags; /* [0] != 0: the uploaded tree is malformed, gravity is skipped */
 };
 
-/* One thread per level slot: node id, kind, children and the parent slot
- * (parentIndexes), with every index checked against its buffer so that a
- * malformed tree disables gravity instead of reading out of bounds. */
+/* Clears what k_tree_convert counts and links, and the malformed-tree flag. */
+__global__ void k_tree_convert_init(int32_t* __restrict__ parent_slot, int32_t* __restrict__ parent_level,
+                                    int32_t* __restrict__ seen, int count, uint32_t* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    parent_slot[i] = -1;
+    parent_level[i] = -1;
+    seen[i] = 0;
+    if (i == 0) flags[0] = 0u;
+}
+
+/* One thread per level slot, every level in one launch: node id, kind,
+ * children and the parent's slot within its level (parentIndexes), with
+ * every index checked against its buffer so that a malformed tree disables
+ * gravity instead of reading out of bounds.  The levels are independent: a
+ * slot writes only its own node's record and its children's parent links. */
 __global__ void k_tree_convert(ConvertArgs a) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.count) return;
+    const int s = a.off[0] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.off[a.chunk_levels]) return;
+    int l = 0;
+    while (l + 1 < a.chunk_levels && a.off[l + 1] <= s) ++l;
+    const int level = a.first_level + l;
     const int tp = a.level_tp[s];
     bool bad = tp < 0 || (int64_t)tp + 2 > a.n_ints;
     const int np = bad ? 0 : a.octree[tp];
     const int id = bad ? -1 : a.octree[tp + 1];
     bad = bad || id < 0 || id >= a.node_count || np < 0;
     bad = bad || (np > 0 && (int64_t)tp + 2 + np > a.n_ints) || (np == 0 && (int64_t)tp + 10 > a.n_ints);
-    bad = bad || (np == 0 && a.level == a.n_levels - 1) || (a.level == 0 && (tp != 0 || id != 0));
+    bad = bad || (np == 0 && level == a.n_levels - 1) || (level == 0 && (tp != 0 || id != 0));
     if (bad) {
         a.flags[0] = 1u;
         a.level_id[s] = 0;
         return;
     }
     a.level_id[s] = id;
-    a.info[id] = make_int4(np, tp, a.level, 0);
+    a.info[id] = make_int4(np, tp, level, 0);
     atomicAdd(&a.seen[id], 1);
     if (np > 0) {
         for (int k = 0; k < np; ++k) {
@@ -124,8 +144,8 @@ __global__ void k_tree_convert(ConvertArgs a) {
         }
         a.child[8 * id + c] = cid;
         if (cid >= 0) {
-            a.parent_slot[cid] = s;
-            a.parent_level[cid] = a.level;
+            a.parent_slot[cid] = s - a.off[l];
+            a.parent_level[cid] = level;
         }
     }
 }
@@ -1495,19 +1515,18 @@ Tree make_tree(sph_ctx* ctx) {
 
 }  // namespace
 
-int launch_tree_convert(sph_ctx* ctx) {
+int launch_tree_convert(sph_ctx* ctx, hipStream_t st) {
     sph_tree_dev& T = ctx->tree;
     const int threads = 256;
     const int K = T.node_count;
-    const int kb = (K + threads - 1) / threads;
-    k_fill_i32<<<kb, threads, 0, ctx->stream>>>(T.parent_slot, K, -1);
-    k_fill_i32<<<kb, threads, 0, ctx->stream>>>(T.parent_level, K, -1);
-    k_fill_i32<<<kb, threads, 0, ctx->stream>>>(T.seen, K, 0);
-    k_fill_i32<<<1, 1, 0, ctx->stream>>>((int32_t*)ctx->grav_flags, 1, 0);
+    k_tree_convert_init<<<(K + threads - 1) / threads, threads, 0, st>>>(T.parent_slot, T.parent_level, T.seen, K,
+                                                                           ctx->grav_flags);
     SPH_HIP_TRY(hipGetLastError());
     ConvertArgs a;
     a.octree = T.octree;
     a.n_ints = T.n_ints;
+    a.level_tp = T.level_tp;
+    a.level_id = T.level_id;
     a.n_levels = (int)T.level_counts.size();
     a.node_count = K;
     a.n_particles = ctx->n;
@@ -1517,17 +1536,19 @@ int launch_tree_convert(sph_ctx* ctx) {
     a.parent_level = T.parent_level;
     a.seen = T.seen;
     a.flags = ctx->grav_flags;
-    for (size_t l = 0; l < T.level_counts.size(); ++l) {
-        a.count = T.level_counts[l];
-        if (a.count == 0) continue;
-        a.level = (int)l;
-        a.level_tp = T.level_tp + T.level_offsets[l];
-        a.level_id = T.level_id + T.level_offsets[l];
-        k_tree_convert<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+    /* one launch per kMaxLevels levels: one for any tree the library's builders make */
+    for (int l0 = 0; l0 < a.n_levels; l0 += kMaxLevels) {
+        a.first_level = l0;
+        a.chunk_levels = std::min(kMaxLevels, a.n_levels - l0);
+        for (int l = 0; l < a.chunk_levels; ++l) a.off[l] = T.level_offsets[l0 + l];
+        a.off[a.chunk_levels] = a.off[a.chunk_levels - 1] + T.level_counts[l0 + a.chunk_levels - 1];
+        const int count = a.off[a.chunk_levels] - a.off[0];
+        if (count == 0) continue;
+        k_tree_convert<<<(count + threads - 1) / threads, threads, 0, st>>>(a);
         SPH_HIP_TRY(hipGetLastError());
     }
-    k_tree_check<<<kb, threads, 0, ctx->stream>>>(K, T.node_info, T.parent_slot, T.parent_level, T.seen,
-                                                  ctx->grav_flags);
+    k_tree_check<<<(K + threads - 1) / threads, threads, 0, st>>>(K, T.node_info, T.parent_slot, T.parent_level,
+                                                                 T.seen, ctx->grav_flags);
     SPH_HIP_TRY(hipGetLastError());
     return SPH_OK;
 }

@@ -127,6 +127,10 @@ struct sph_ctx {
     hipStream_t gstream = nullptr;
     hipEvent_t ev_up = nullptr, ev_down = nullptr;
     bool grav_pending = false;
+    /* an uploaded tree is converted on copy_stream, overlapping the sort;
+     * join_tree makes `stream` wait for it (convert_done) */
+    hipEvent_t tree_free = nullptr, convert_done = nullptr;
+    bool convert_pending = false;
     std::string err;
 
     /* particle state, original order */
@@ -238,10 +242,15 @@ int launch_step(sph_ctx* ctx);
 int launch_density_prologue(sph_ctx* ctx);
 int launch_density_epilogue(sph_ctx* ctx);
 /* gravity.hip */
-int launch_tree_convert(sph_ctx* ctx);
+/* node records, links and checks of the current tree's upload, on stream st */
+int launch_tree_convert(sph_ctx* ctx, hipStream_t st);
 int launch_gravity_up(sph_ctx* ctx);
 /* makes ctx->stream wait for a gravity down pass still running on gstream */
 int join_gravity(sph_ctx* ctx);
+/* makes ctx->stream wait for a tree upload's conversion still running on
+ * copy_stream (every call that reads the tree's node records or the
+ * malformed-tree flag) */
+int join_tree(sph_ctx* ctx);
 /* stable LSD radix sort of (key, index) pairs by the low `passes` key bytes */
 int launch_sort_pairs(sph_ctx* ctx, uint2* a, uint2* b, int n, int passes, uint32_t* tile_hist, int tile_cap,
                       uint32_t* digit_total, uint2** out);
@@ -253,7 +262,7 @@ const char* noalloc_last_error();
 int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int n_levels, int32_t node_count,
                  int* slot);
 int tree_commit(sph_ctx* ctx, int slot, int64_t n_ints, const int32_t* level_counts, int n_levels,
-                int32_t node_count);
+                int32_t node_count, hipStream_t convert_stream);
 /* sorted slots of the active particles into slot_list / slot_count (README rule) */
 int launch_active_list(sph_ctx* ctx);
 /* bootstrap: the as-written P = 2 down pass (leaves force-accept), used by

@@ -440,6 +440,7 @@ int sph_build_tree(sph_ctx* ctx) {
     if (max_depth < 0 || max_depth > kMaxDepth)
         return ctx_fail(ctx, SPH_E_ARG, "the GPU tree needs max_tree_depth 0..10");
     SPH_TRY_RC(join_gravity(ctx));
+    SPH_TRY_RC(join_tree(ctx)); /* this build's conversion rewrites the node records */
     SPH_TRY_RC(ensure_buffers(ctx));
     SPH_TRY_RC(ensure_node_buffers(ctx, ctx->tb.lv_cap)); /* tile table for the Morton sort */
     SPH_TRY_RC(ctx_begin_pass(ctx, SPH_PASS_TREE_BUILD));
@@ -490,7 +491,7 @@ int sph_build_tree(sph_ctx* ctx) {
     k_tb_levels<<<grid_for(node_count), 256, 0, s>>>(by_level, node_count, ctx->tree.level_tp_up[k]);
     SPH_HIP_TRY(hipGetLastError());
     SPH_TRY_RC(ctx_end_pass(ctx, SPH_PASS_TREE_BUILD));
-    return tree_commit(ctx, k, n_ints, counts, n_levels, node_count);
+    return tree_commit(ctx, k, n_ints, counts, n_levels, node_count, ctx->stream);
 }
 
 int sph_tree_info(sph_ctx* ctx, int64_t* n_ints, int32_t* node_count, int32_t* n_levels, int32_t* level_counts,
@@ -510,6 +511,7 @@ int sph_download_tree(sph_ctx* ctx, int32_t* octree, int32_t* level_nodes) {
     if (!ctx || !octree || !level_nodes) return SPH_E_ARG;
     if (!ctx->have_tree) return ctx_fail(ctx, SPH_E_NOTREE, "no tree");
     SPH_TRY_RC(join_gravity(ctx));
+    SPH_TRY_RC(join_tree(ctx));
     SPH_HIP_TRY(hipStreamSynchronize(ctx->stream));
     const sph_tree_dev& T = ctx->tree;
     SPH_HIP_TRY(hipMemcpy(octree, T.octree, 4 * (size_t)T.n_ints, hipMemcpyDeviceToHost));
