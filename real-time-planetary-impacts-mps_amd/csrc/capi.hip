@@ -339,7 +339,7 @@ int sph_create(const sph_params* params, const sph_ic_view* ic, const sph_eos_ta
     TRY(dalloc(&ctx->tile_hist, (size_t)ctx->ntiles * 256));
     TRY(dalloc(&ctx->digit_total, 256));
     TRY(dalloc(&ctx->grav_flags, 4));
-    TRY(dalloc(&ctx->tree.act_count, kMaxLevels));
+    TRY(dalloc(&ctx->tree.act_count, 2 * kMaxLevels));
     TRY(hipMemset(ctx->grav_flags, 0, 16));
 
     /* initial conditions (Compute.cpp:52-80) */

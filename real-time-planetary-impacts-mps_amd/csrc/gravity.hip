@@ -443,8 +443,9 @@ struct DownArgs {
     const uint8_t* active;
     float4* acc;
     float* gravnorm;
-    /* README rule: the level's slots whose node is active (k_level_active)
-     * and their count on the device; null: every slot of the level */
+    /* README rule: the level's slots whose node is active (k_level_active):
+     * the leaves' from the front, act_count[0] of them, the branches' from
+     * the back, act_count[kMaxLevels] of them; null: every slot of the level */
     const int32_t* act_slots;
     const int32_t* act_count;
 };
@@ -1381,7 +1382,10 @@ __device__ __forceinline__ void down_group_body(const DownArgs& a, int s, bool v
 /* One level slot per group of GS lanes.  Every slot of the level: one per
  * group, workgroups XCD-remapped.  With an active-slot list (README rule,
  * most nodes inactive): a persistent grid whose XCD x walks the x-th eighth
- * of the list, so no workgroup is launched for an inactive node. */
+ * of the list, so no workgroup is launched for an inactive node.  The
+ * branch-only launch (MODE 1) walks the active branches alone and the leaf
+ * launches (MODE 2-4) the active leaves alone, so that no group of a wave
+ * idles on a node its launch skips. */
 template <int GS, bool NEAR, bool PREC, int P, int MODE>
 __device__ __forceinline__ void down_group_run(const DownArgs& a) {
     constexpr int GPB = 256 / GS;
@@ -1393,7 +1397,9 @@ __device__ __forceinline__ void down_group_run(const DownArgs& a) {
         down_group_body<GS, NEAR, PREC, P, MODE>(a, sl, sl < a.count);
         return;
     }
-    const int total = *a.act_count;
+    constexpr int SEL = MODE == 0 ? 0 : (MODE == 1 ? 1 : 2); /* both, branches, leaves */
+    const int nl = a.act_count[0], nb = a.act_count[kMaxLevels];
+    const int total = SEL == 0 ? nl + nb : (SEL == 1 ? nb : nl);
     const int nlog = (total + GPB - 1) / GPB;
     const int per = (nlog + kXcds - 1) / kXcds;
     const int xcd = (int)(blockIdx.x % kXcds), nper = (int)(gridDim.x / kXcds);
@@ -1402,7 +1408,9 @@ __device__ __forceinline__ void down_group_run(const DownArgs& a) {
         if (lb >= nlog) break;
         const int e = lb * GPB + g;
         const bool ok = e < total;
-        down_group_body<GS, NEAR, PREC, P, MODE>(a, ok ? a.act_slots[e] : 0, ok);
+        const bool front = SEL == 2 || (SEL == 0 && e < nl);
+        const int at = front ? e : a.count - 1 - (SEL == 0 ? e - nl : e);
+        down_group_body<GS, NEAR, PREC, P, MODE>(a, ok ? a.act_slots[at] : 0, ok);
     }
 }
 
@@ -1416,19 +1424,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((!NEAR && P
 }
 
 /* The slots of a level whose node is active (the up pass's activity flag),
- * appended a wave at a time; the list order varies from run to run, each
- * node's result does not. */
+ * appended a wave at a time: leaves from the front of the level's list
+ * (counted in nact[0]), branches from its back (nact[kMaxLevels]).  The
+ * list order varies from run to run, each node's result does not. */
 __global__ __launch_bounds__(256) void k_level_active(const int32_t* __restrict__ level_id, int count,
-                                                      const float4* __restrict__ aux, int32_t* __restrict__ slots,
-                                                      int32_t* __restrict__ nact) {
+                                                      const float4* __restrict__ aux, const int4* __restrict__ info,
+                                                      int32_t* __restrict__ slots, int32_t* __restrict__ nact) {
     const int sl = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const bool act = sl < count && aux[3 * level_id[sl] + 1].w != 0.f;
-    const uint64_t b = __ballot(act);
+    const int id = sl < count ? level_id[sl] : 0;
+    const bool act = sl < count && aux[3 * id + 1].w != 0.f;
+    const bool leaf = act && info[id].x != 0;
+    const uint64_t bl = __ballot(leaf), bb = __ballot(act && !leaf);
     const int lane = (int)(threadIdx.x & 63);
-    int base = 0;
-    if (lane == 0 && b != 0ull) base = atomicAdd(nact, (int)__popcll(b));
-    base = __shfl(base, 0, 64);
-    if (act) slots[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = sl;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int basel = 0, baseb = 0;
+    if (lane == 0 && bl != 0ull) basel = atomicAdd(nact, (int)__popcll(bl));
+    if (lane == 0 && bb != 0ull) baseb = atomicAdd(nact + kMaxLevels, (int)__popcll(bb));
+    basel = __shfl(basel, 0, 64);
+    baseb = __shfl(baseb, 0, 64);
+    if (leaf) slots[basel + (int)__popcll(bl & lt)] = sl;
+    else if (act) slots[count - 1 - (baseb + (int)__popcll(bb & lt))] = sl;
 }
 
 /* The near-field leaves of a level, M2L part.  Two waves per SIMD are asked
@@ -1614,13 +1629,14 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
     const bool compact = p.activity_rule == SPH_ACTIVE_README && (a.max_rec == 1 || a.near) &&
                          T.level_counts.size() <= (size_t)kMaxLevels;
     if (compact) {
-        k_fill_i32<<<1, 64, 0, ctx->stream>>>(T.act_count, kMaxLevels, 0);
+        k_fill_i32<<<1, 2 * kMaxLevels, 0, ctx->stream>>>(T.act_count, 2 * kMaxLevels, 0);
         SPH_HIP_TRY(hipGetLastError());
         for (size_t l = 1; l < T.level_counts.size(); ++l) {
             const int cnt = T.level_counts[l];
             if (cnt == 0) continue;
             k_level_active<<<(cnt + threads - 1) / threads, threads, 0, ctx->stream>>>(
-                T.level_id + T.level_offsets[l], cnt, T.mp_aux, T.act_slots + T.level_offsets[l], T.act_count + l);
+                T.level_id + T.level_offsets[l], cnt, T.mp_aux, T.node_info, T.act_slots + T.level_offsets[l],
+                T.act_count + l);
             SPH_HIP_TRY(hipGetLastError());
         }
     }

@@ -91,7 +91,7 @@ struct sph_tree_dev {
     hipEvent_t released[2] = {nullptr, nullptr}; /* compute stream done with buffer k */
     int32_t* level_id = nullptr;    /* same slots, node ids */
     int32_t* act_slots = nullptr;   /* same slots: per level, the slots of its active nodes (README rule) */
-    int32_t* act_count = nullptr;   /* kMaxLevels counters of act_slots */
+    int32_t* act_count = nullptr;   /* 2 kMaxLevels counters of act_slots: leaves, then branches */
     int64_t level_cap = 0;
     int4* node_info = nullptr;      /* (nParticles, treePointer, level, 0) */
     int32_t* parent_slot = nullptr; /* slot of the parent in its level list (parentIndexes) */
