@@ -1597,6 +1597,41 @@ int launch_gravity_up(sph_ctx* ctx) {
     return SPH_OK;
 }
 
+/* One level of the near-field form with GS lanes per node: branches (list
+ * processing, L2L), then the leaves (balanced walk, pair sums, L2P) in one
+ * or two launches; persistent over the active list, at most kDownBlocks
+ * workgroups, when there is one.  The near-field form runs a wave (64
+ * lanes) per node: each node's walk is a chain of dependent rounds, and
+ * twice the lanes halve the rounds (measured against 32 lanes: 1e6 second
+ * order 107.3 -> 113.4 steps/s, third-order precision 33.9 -> 37.1, 4e6
+ * learned-h + precision with individual timesteps 224.0 -> 255.0). */
+template <int GS>
+static void launch_near_level(const DownArgs& a, bool compact, bool order3, hipStream_t st) {
+    const int threads = 256;
+    const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
+    const dim3 g(compact ? std::min(xcd_grid(blocks), (unsigned)kDownBlocks) : xcd_grid(blocks));
+    if (order3) {
+        if (a.phi) {
+            k_down_group<GS, true, true, 3, 1><<<g, threads, 0, st>>>(a);
+            k_down_leaves<GS, true, 3><<<g, threads, 0, st>>>(a);
+            k_down_leaves_p2p<GS, true, 3><<<g, threads, 0, st>>>(a);
+        } else {
+            k_down_group<GS, true, false, 3, 1><<<g, threads, 0, st>>>(a);
+            k_down_leaves<GS, false, 3><<<g, threads, 0, st>>>(a);
+            k_down_leaves_p2p<GS, false, 3><<<g, threads, 0, st>>>(a);
+        }
+    } else {
+        if (a.phi) {
+            k_down_group<GS, true, true, 2, 1><<<g, threads, 0, st>>>(a);
+            k_down_leaves<GS, true, 2><<<g, threads, 0, st>>>(a);
+            k_down_leaves_p2p<GS, true, 2><<<g, threads, 0, st>>>(a);
+        } else {
+            k_down_group<GS, true, false, 2, 1><<<g, threads, 0, st>>>(a);
+            k_down_leaves_walk<GS, false, 2><<<g, threads, 0, st>>>(a);
+        }
+    }
+}
+
 int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
     sph_tree_dev& T = ctx->tree;
     const sph_params& p = ctx->prm;
@@ -1657,39 +1692,21 @@ int launch_gravity_down(sph_ctx* ctx, bool bootstrap) {
         a.out = out;
         /* the near-field form has no recursion limit (leaves walk whole subtrees) */
         if (a.max_rec == 1 || a.near) {
-            constexpr int GS = 32;
-            const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
             if (compact) {
                 a.act_slots = T.act_slots + T.level_offsets[l];
                 a.act_count = T.act_count + l;
             }
-            /* persistent over the active list: at most kDownBlocks workgroups */
-            const dim3 g(compact ? std::min(xcd_grid(blocks), (unsigned)kDownBlocks) : xcd_grid(blocks));
-            /* near-field form: branches (list processing, L2L), then the
-             * leaves (balanced walk, pair sums, L2P) in one or two launches */
-            if (order3) {
-                if (a.phi) {
-                    k_down_group<GS, true, true, 3, 1><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves<GS, true, 3><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves_p2p<GS, true, 3><<<g, threads, 0, ctx->stream>>>(a);
-                } else if (a.near) {
-                    k_down_group<GS, true, false, 3, 1><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves<GS, false, 3><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves_p2p<GS, false, 3><<<g, threads, 0, ctx->stream>>>(a);
-                } else {
-                    k_down_group<GS, false, false, 3, 0><<<g, threads, 0, ctx->stream>>>(a);
-                }
+            if (a.near) {
+                launch_near_level<64>(a, compact, order3, ctx->stream);
             } else {
-                if (a.phi) {
-                    k_down_group<GS, true, true, 2, 1><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves<GS, true, 2><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves_p2p<GS, true, 2><<<g, threads, 0, ctx->stream>>>(a);
-                } else if (a.near) {
-                    k_down_group<GS, true, false, 2, 1><<<g, threads, 0, ctx->stream>>>(a);
-                    k_down_leaves_walk<GS, false, 2><<<g, threads, 0, ctx->stream>>>(a);
-                } else {
-                    k_down_group<GS, false, false, 2, 0><<<g, threads, 0, ctx->stream>>>(a);
-                }
+                /* as written: 32 lanes per node (64 measured slower at 1e6:
+                 * 144.7 steps/s at every level, 151.0 at the levels of fewer
+                 * than 8192 nodes, against 153.4) */
+                constexpr int GS = 32;
+                const unsigned blocks = (unsigned)(((int64_t)a.count * GS + threads - 1) / threads);
+                const dim3 g(compact ? std::min(xcd_grid(blocks), (unsigned)kDownBlocks) : xcd_grid(blocks));
+                if (order3) k_down_group<GS, false, false, 3, 0><<<g, threads, 0, ctx->stream>>>(a);
+                else k_down_group<GS, false, false, 2, 0><<<g, threads, 0, ctx->stream>>>(a);
             }
         } else {
             a.act_slots = nullptr;
