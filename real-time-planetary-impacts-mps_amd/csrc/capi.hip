@@ -80,6 +80,7 @@ void free_tree(sph_tree_dev& T) {
     dfree(T.mp_far);
     dfree(T.mp_aux);
     dfree(T.mp_oct);
+    dfree(T.mp_clo);
     dfree(T.loc3);
     dfree(T.loc);
     dfree(T.loc_tot);
@@ -478,6 +479,7 @@ int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int 
             dfree(T.mp_far);
             dfree(T.mp_aux);
             dfree(T.mp_oct);
+            dfree(T.mp_clo);
             dfree(T.loc3);
             dfree(T.loc);
             SPH_HIP_TRY(dalloc(&T.node_info, (size_t)cap));
@@ -492,6 +494,7 @@ int tree_reserve(sph_ctx* ctx, int64_t n_ints, const int32_t* level_counts, int 
                 SPH_HIP_TRY(dalloc(&T.mp_oct, (size_t)cap * 3));
                 SPH_HIP_TRY(dalloc(&T.loc3, (size_t)cap * 3));
             }
+            if (ctx->prm.precision) SPH_HIP_TRY(dalloc(&T.mp_clo, (size_t)cap));
             SPH_HIP_TRY(dalloc(&T.loc, (size_t)cap * 4));
             dfree(T.loc_tot);
             dfree(T.near_rec);

@@ -54,6 +54,9 @@ struct Tree {
     int32_t* near_rec;
     int32_t* near_cnt;
     int near_cap;
+    /* precision form: each node's centre is far[3 id].xyz + clo[id].xyz, the
+     * second part the rounding residual of the first (null in other forms) */
+    float4* clo;
 };
 
 /* ------------------------------------------------------------ conversion */
@@ -181,7 +184,8 @@ struct UpArgs {
     int fix_zz;
     int fix_range;
     int need_preorder; /* the near-field down pass walks subtrees as id ranges */
-    int prec;          /* precision form: centres of mass summed relative to the first member */
+    int prec;          /* precision form: centres of mass summed relative to the first member and
+                        * kept in two parts (far pos + clo), moments taken about their sum */
     int order;         /* expansion order P: 2 or 3 */
     int fix_p3;        /* SPH_FIX_ORDER3: power[3] summed as intended */
 };
@@ -222,7 +226,16 @@ __device__ __forceinline__ float power2(float XX, float XY, float XZ, float YY, 
     return sqrtf(p2);
 }
 
+/* Knuth's TwoSum: s = fl(a + b) and e with s + e = a + b exactly, for any
+ * a and b (no ordering of their magnitudes needed). */
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
 /* P2M.metal:12-107 */
+template <bool PREC>
 __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
     const int32_t* pl = a.t.octree + tp + 2;
     const int gt = a.time[0];
@@ -231,7 +244,7 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
     bool act = false;
     float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
     /* precision form: sum m (x - x_0) instead of m x, x_0 = first particle */
-    const float4 x0 = a.prec ? a.pos[pl[0]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 x0 = PREC ? a.pos[pl[0]] : make_float4(0.f, 0.f, 0.f, 0.f);
     for (int k = 0; k < np; ++k) {
         const int p = pl[k];
         const float4 xa = a.pos[p];
@@ -249,16 +262,28 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
         if (isActive) minGrav = fminf(minGrav, a.gravnorm[p]);
     }
     for (int k = 0; k < np; ++k) a.active[pl[k]] = act ? 1 : 0;
-    px = px / M + x0.x;
-    py = py / M + x0.y;
-    pz = pz / M + x0.z;
+    /* precision form: the centre as x_0 + (offset) in two parts, the second
+     * the residual of the first's rounding, so that offsets from the centre
+     * keep the bits an FP32 absolute position (|x| ~ 20, ulp ~ 2e-6) drops */
+    float clx = 0.f, cly = 0.f, clz = 0.f;
+    if constexpr (PREC) {
+        two_sum(x0.x, px / M, px, clx);
+        two_sum(x0.y, py / M, py, cly);
+        two_sum(x0.z, pz / M, pz, clz);
+        a.t.clo[id] = make_float4(clx, cly, clz, 0.f);
+    } else {
+        px = px / M + x0.x;
+        py = py / M + x0.y;
+        pz = pz / M + x0.z;
+    }
     float X = 0.f, Y = 0.f, Z = 0.f, XX = 0.f, XY = 0.f, XZ = 0.f, YY = 0.f, YZ = 0.f, ZZ = 0.f;
     float o[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int k = 0; k < np; ++k) {
         const int p = pl[k];
         const float4 x = a.pos[p];
         const float pm = x.w;
-        const float rx = x.x - px, ry = x.y - py, rz = x.z - pz;
+        /* x - px is exact (x and px within a factor 2 of each other) */
+        const float rx = (x.x - px) - clx, ry = (x.y - py) - cly, rz = (x.z - pz) - clz;
         X -= rx * pm;
         Y -= ry * pm;
         Z -= rz * pm;
@@ -307,6 +332,7 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
 }
 
 /* M2M.metal:75-160 with transformMultipole (M2M.metal:12-73) */
+template <bool PREC>
 __device__ void m2m(const UpArgs& a, int id, int tp) {
     const int32_t* ch = a.t.child + 8 * id;
     float minGrav = FLT_MAX, mass = 0.f, px = 0.f, py = 0.f, pz = 0.f;
@@ -314,7 +340,7 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
     float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
     /* precision form: centre of mass relative to the first child's */
     float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (a.prec)
+    if constexpr (PREC)
         for (int c = 0; c < 8; ++c)
             if (ch[c] >= 0) {
                 r0 = a.t.far[3 * ch[c]];
@@ -328,6 +354,12 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
         f0.x -= r0.x;
         f0.y -= r0.y;
         f0.z -= r0.z;
+        if constexpr (PREC) { /* the child's centre in two parts */
+            const float4 cl = a.t.clo[cid];
+            f0.x += cl.x;
+            f0.y += cl.y;
+            f0.z += cl.z;
+        }
         const float4 x0 = a.t.aux[3 * cid], x1 = a.t.aux[3 * cid + 1];
         minGrav = fminf(x0.w, minGrav);
         mass += f1.x;
@@ -340,16 +372,30 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
         mnx = fminf(mnx, x0.x); mny = fminf(mny, x0.y); mnz = fminf(mnz, x0.z);
     }
     const float size = fmaxf(fmaxf(mxx - mnx, mxy - mny), mxz - mnz);
-    px = px / mass + r0.x;
-    py = py / mass + r0.y;
-    pz = pz / mass + r0.z;
+    float clx = 0.f, cly = 0.f, clz = 0.f;
+    if constexpr (PREC) {
+        two_sum(r0.x, px / mass, px, clx);
+        two_sum(r0.y, py / mass, py, cly);
+        two_sum(r0.z, pz / mass, pz, clz);
+        a.t.clo[id] = make_float4(clx, cly, clz, 0.f);
+    } else {
+        px = px / mass + r0.x;
+        py = py / mass + r0.y;
+        pz = pz / mass + r0.z;
+    }
     float M = 0.f, XX = 0.f, XY = 0.f, XZ = 0.f, YY = 0.f, YZ = 0.f, ZZ = 0.f;
     float o[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int c = 0; c < 8; ++c) {
         const int cid = ch[c];
         if (cid < 0) continue;
         const float4 f0 = a.t.far[3 * cid], f1 = a.t.far[3 * cid + 1], f2 = a.t.far[3 * cid + 2];
-        const float rx = px - f0.x, ry = py - f0.y, rz = pz - f0.z;
+        float rx = px - f0.x, ry = py - f0.y, rz = pz - f0.z;
+        if constexpr (PREC) {
+            const float4 cl = a.t.clo[cid];
+            rx += clx - cl.x;
+            ry += cly - cl.y;
+            rz += clz - cl.z;
+        }
         const float cM = f1.x;
         /* as written the ZZ shift uses the child's dipole Z (M2M.metal:29) */
         const float zzw = a.fix_zz ? cM : a.t.aux[3 * cid + 2].z;
@@ -394,17 +440,18 @@ __device__ void m2m(const UpArgs& a, int id, int tp) {
     if (a.order == 3) store_oct(a, id, o);
 }
 
+template <bool PREC>
 __global__ __launch_bounds__(256) void k_up(UpArgs a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.count || a.flags[0] != 0u) return;
     const int id = a.level_id[s];
     const int4 inf = a.t.info[id];
     if (inf.x > 0) {
-        p2m(a, id, inf.x, inf.y);
+        p2m<PREC>(a, id, inf.x, inf.y);
         a.t.sub_end[id] = id + 1;
         return;
     }
-    m2m(a, id, inf.y);
+    m2m<PREC>(a, id, inf.y);
     /* Pre-order numbering (octree.cpp:14-109 numbers nodes as it recurses,
      * children in slot order): the first child is id + 1 and each later
      * child starts where the previous child's subtree ends. */
@@ -452,7 +499,8 @@ struct DownArgs {
 
 struct LocalExp {
     float px, py, pz;
-    float e[eNT3];   /* the first nterms<P>() are used */
+    float lx, ly, lz; /* precision form: the centre's low part (centre = p + l), else 0 */
+    float e[eNT3];    /* the first nterms<P>() are used */
 };
 
 /* accept.metal:14-55 at P = 2 (the n = 1 term has power[1] = 0). */
@@ -502,9 +550,9 @@ __device__ __forceinline__ bool mac_accept3(float ax, float ay, float az, float 
  * Each coefficient is handed to `emit(k, value)` as soon as it is final, so
  * that the accumulation (plain or compensated) does not hold all of them. */
 template <int P, class Emit>
-__device__ __forceinline__ void m2l_emit(const LocalExp& L, float4 b0, float4 b1, float4 b2, const float4* oct,
+__device__ __forceinline__ void m2l_emit(float vx, float vy, float vz, float4 b1, float4 b2, const float4* oct,
                                          float eps, Emit emit) {
-    const float vx = L.px - b0.x, vy = L.py - b0.y, vz = L.pz - b0.z;
+    /* (vx, vy, vz): the expansion centre minus the source's centre */
     const float r2 = vx * vx + vy * vy + vz * vz;
     /* the hardware square root and reciprocal (about 1 ulp) instead of the
      * correctly rounded library sequences: this is the down pass's
@@ -596,7 +644,8 @@ __device__ __forceinline__ void m2l_emit(const LocalExp& L, float4 b0, float4 b1
 
 template <int P>
 __device__ __forceinline__ void m2l_add(const DownArgs& a, LocalExp& L, int node, float4 b0, float4 b1, float4 b2) {
-    m2l_emit<P>(L, b0, b1, b2, a.t.oct + 3 * node, a.eps, [&](int k, float v) { L.e[k] += v; });
+    m2l_emit<P>(L.px - b0.x, L.py - b0.y, L.pz - b0.z, b1, b2, a.t.oct + 3 * node, a.eps,
+                [&](int k, float v) { L.e[k] += v; });
 }
 
 /* Neumaier's compensated sum (Neumaier 1974): s + c carries the sum of the
@@ -611,7 +660,10 @@ __device__ __forceinline__ void kadd(float& s, float& c, float x) {
 template <int P>
 __device__ __forceinline__ void m2l_add_comp(const DownArgs& a, LocalExp& L, float Lc[eNT3], int node, float4 b0,
                                              float4 b1, float4 b2) {
-    m2l_emit<P>(L, b0, b1, b2, a.t.oct + 3 * node, a.eps, [&](int k, float v) { kadd(L.e[k], Lc[k], v); });
+    /* centres in two parts: the high parts' difference plus the low parts' */
+    const float4 cl = a.t.clo[node];
+    m2l_emit<P>((L.px - b0.x) + (L.lx - cl.x), (L.py - b0.y) + (L.ly - cl.y), (L.pz - b0.z) + (L.lz - cl.z), b1, b2,
+                a.t.oct + 3 * node, a.eps, [&](int k, float v) { kadd(L.e[k], Lc[k], v); });
 }
 
 /* Softened 1/r whose derivatives M2L uses (derivatives.metal:12-57,
@@ -708,13 +760,20 @@ __device__ __forceinline__ float dphi_dr_m2l(float r, float eps) {
 }
 
 /* transformLocal (L2L.metal:12-88) of the node's local expansion to one
- * active child, written to the child's record (L2L.metal:90-104). */
-template <int P>
+ * active child, written to the child's record (L2L.metal:90-104).  PREC:
+ * centres in two parts; the child's record holds its centre's high part. */
+template <int P, bool PREC = false>
 __device__ __forceinline__ void l2l_child(const DownArgs& a, const LocalExp& L, int cid) {
     if (cid < 0) return;
     if (a.t.aux[3 * cid + 1].w == 0.f) return;
     const float4 c0 = a.t.far[3 * cid];
-    const float rx = c0.x - L.px, ry = c0.y - L.py, rz = c0.z - L.pz;
+    float rx = c0.x - L.px, ry = c0.y - L.py, rz = c0.z - L.pz;
+    if constexpr (PREC) {
+        const float4 cl = a.t.clo[cid];
+        rx += cl.x - L.lx;
+        ry += cl.y - L.ly;
+        rz += cl.z - L.lz;
+    }
     const float* e = L.e;
     float nM = e[eM];
     nM += rx * e[eX] + ry * e[eY] + rz * e[eZ];
@@ -747,7 +806,8 @@ __device__ __forceinline__ void l2l_child(const DownArgs& a, const LocalExp& L, 
         a.t.loc3[3 * cid + 1] = make_float4(e[eXYZ], e[eXZZ], e[eYYY], e[eYYZ]);
         a.t.loc3[3 * cid + 2] = make_float4(e[eYZZ], e[eZZZ], 0.f, 0.f);
     }
-    a.t.loc[4 * cid + 0] = make_float4(L.px + rx, L.py + ry, L.pz + rz, nM);
+    if constexpr (PREC) a.t.loc[4 * cid + 0] = make_float4(c0.x, c0.y, c0.z, nM);
+    else a.t.loc[4 * cid + 0] = make_float4(L.px + rx, L.py + ry, L.pz + rz, nM);
     a.t.loc[4 * cid + 1] = make_float4(nX, nY, nZ, nXX);
     a.t.loc[4 * cid + 2] = make_float4(nXY, nXZ, nYY, nYZ);
     a.t.loc[4 * cid + 3] = make_float4(nZZ, 0.f, 0.f, 0.f);
@@ -787,14 +847,18 @@ __device__ __forceinline__ void add_incoming(const DownArgs& a, int id, float4 l
  * nz) is the near-field form's sum over rejected source leaves (else 0).
  * PREC (precision form, near field only): compensated sums, and the
  * potential -G (local expansion at x + sum of the pair potentials) with
- * `nphi` the rejected leaves' part. */
-template <bool PREC, int P>
+ * `nphi` the rejected leaves' part.  INTRA false: (nx, ny, nz, nphi) already
+ * hold the intra-leaf pairs (summed over the group's lanes). */
+template <bool PREC, int P, bool INTRA = true>
 __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp& L, const int32_t* pl, int np, int k,
                                               float nx = 0.f, float ny = 0.f, float nz = 0.f, float nphi = 0.f) {
     const int p = pl[k];
     if (!a.active[p]) return;
     const float4 xi = a.pos[p];
-    const float rx = xi.x - L.px, ry = xi.y - L.py, rz = xi.z - L.pz;
+    /* PREC: the offset from the centre's two parts (xi - px is exact) */
+    const float rx = PREC ? (xi.x - L.px) - L.lx : xi.x - L.px;
+    const float ry = PREC ? (xi.y - L.py) - L.ly : xi.y - L.py;
+    const float rz = PREC ? (xi.z - L.pz) - L.lz : xi.z - L.pz;
     const float* e = L.e;
     /* L2P (L2P.metal:12-32) */
     float ax = e[eX], ay = e[eY], az = e[eZ];
@@ -822,7 +886,7 @@ __device__ __forceinline__ void leaf_particle(const DownArgs& a, const LocalExp&
             ps += rx * ry * rz * e[eXYZ];
         }
     }
-    for (int j = 0; j < np; ++j) {
+    for (int j = 0; INTRA && j < np; ++j) {
         if (j == k) continue;
         const int pj = pl[j];
         const float4 xj = a.pos[pj];
@@ -891,6 +955,7 @@ __global__ __launch_bounds__(256) void k_down(DownArgs a) {
         const float4 l0 = a.t.loc[4 * id], l1 = a.t.loc[4 * id + 1], l2 = a.t.loc[4 * id + 2],
                      l3 = a.t.loc[4 * id + 3];
         L.px = l0.x; L.py = l0.y; L.pz = l0.z;
+        L.lx = L.ly = L.lz = 0.f;
         L.e[eM] = l0.w; L.e[eX] = l1.x; L.e[eY] = l1.y; L.e[eZ] = l1.z; L.e[eXX] = l1.w;
         L.e[eXY] = l2.x; L.e[eXZ] = l2.y; L.e[eYY] = l2.z; L.e[eYZ] = l2.w; L.e[eZZ] = l3.x;
     }
@@ -1207,6 +1272,11 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
         }
         }
     reduce:
+        /* the leaf's own particles as sources, every lane taking every GS-th
+         * (a target's pair with itself has r = 0 and adds nothing, as in the
+         * intra-leaf P2P of helpers.h:47-81) */
+        if constexpr (PART != 1)
+            for (int e = lane; e < np; e += GS) near_pair<PREC, NEAR_K>(a, T, nk, a.pos[pl[e]]);
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk) {
             T.gx[kk] = group_sum<GS>(T.gx[kk]);
@@ -1235,7 +1305,7 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk)
             if (lane == kk && kk < nk)
-                leaf_particle<PREC, P>(a, L, pl, np, k0 + kk, T.gx[kk], T.gy[kk], T.gz[kk], T.gp[kk]);
+                leaf_particle<PREC, P, false>(a, L, pl, np, k0 + kk, T.gx[kk], T.gy[kk], T.gz[kk], T.gp[kk]);
     }
 }
 
@@ -1279,6 +1349,15 @@ __device__ __forceinline__ void down_group_body(const DownArgs& a, int s, bool v
     L.px = l0.x;
     L.py = l0.y;
     L.pz = l0.z;
+    L.lx = L.ly = L.lz = 0.f;
+    if constexpr (PREC) {
+        if (live) {
+            const float4 cl = a.t.clo[id];
+            L.lx = cl.x;
+            L.ly = cl.y;
+            L.lz = cl.z;
+        }
+    }
     float Lc[eNT3]; /* compensations (precision form) */
 #pragma unroll
     for (int k = 0; k < NTP; ++k) L.e[k] = Lc[k] = 0.f;
@@ -1373,7 +1452,7 @@ __device__ __forceinline__ void down_group_body(const DownArgs& a, int s, bool v
     if (!live) return;
     add_incoming<P>(a, id, l0, L);
     if (np == 0) {
-        if (lane < 8) l2l_child<P>(a, L, a.t.child[8 * id + lane]);
+        if (lane < 8) l2l_child<P, PREC>(a, L, a.t.child[8 * id + lane]);
     } else {
         leaf_field_lanes<PREC, P>(a, L, inf, lane, GS);
     }
@@ -1489,6 +1568,8 @@ __global__ void k_down_root(DownArgs a) {
         const float4 f0 = t.far[0];
         LocalExp L;
         L.px = f0.x; L.py = f0.y; L.pz = f0.z;
+        const float4 cl = a.phi ? t.clo[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+        L.lx = cl.x; L.ly = cl.y; L.lz = cl.z;
         for (int k = 0; k < eNT3; ++k) L.e[k] = 0.f;
         leaf_field<P>(a, L, inf);
         a.out[0] = -1;
@@ -1525,6 +1606,7 @@ Tree make_tree(sph_ctx* ctx) {
     t.near_rec = ctx->tree.near_rec;
     t.near_cnt = ctx->tree.near_cnt;
     t.near_cap = ctx->tree.near_rec ? ctx->prm.near_list_cap : 0;
+    t.clo = ctx->tree.mp_clo;
     return t;
 }
 
@@ -1591,7 +1673,8 @@ int launch_gravity_up(sph_ctx* ctx) {
         a.count = T.level_counts[l];
         if (a.count == 0) continue;
         a.level_id = T.level_id + T.level_offsets[l];
-        k_up<<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+        if (a.prec) k_up<true><<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
+        else k_up<false><<<(a.count + threads - 1) / threads, threads, 0, ctx->stream>>>(a);
         SPH_HIP_TRY(hipGetLastError());
     }
     return SPH_OK;

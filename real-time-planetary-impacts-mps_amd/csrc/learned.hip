@@ -381,29 +381,42 @@ __global__ __launch_bounds__(256) void k_train(TrainArgs a) {
     }
 }
 
-/* Sum of the workgroup partials (four interleaved running sums, combined
- * in a fixed order), then one Adam update (Kingma & Ba 2015, algorithm 1)
- * when `lr` > 0. */
-__global__ void k_reduce_adam(const float* __restrict__ part, const float* __restrict__ loss_part, int waves,
-                              float* __restrict__ p, float* __restrict__ mom1, float* __restrict__ mom2,
-                              float* __restrict__ grad, float* __restrict__ loss_out, float lr, float bc1, float bc2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        float L = 0.f;
-        for (int k = 0; k < waves; ++k) L += loss_part[k];
-        loss_out[0] = L;
+/* Sum of the workgroup partials, then one Adam update (Kingma & Ba 2015,
+ * algorithm 1) when `lr` > 0.  A workgroup takes RED_P parameters: its 256
+ * threads split the `waves` partials of each into 256 / RED_P interleaved
+ * slices, and the slice sums are added in slice order (a fixed order, so
+ * results are reproducible).  The last workgroup sums the loss partials
+ * (at most 256) with a fixed-order tree. */
+constexpr int RED_P = 16;
+static_assert(TRAIN_BLOCKS <= 256, "the loss partials are summed by one workgroup of 256");
+__global__ __launch_bounds__(256) void k_reduce_adam(const float* __restrict__ part,
+                                                     const float* __restrict__ loss_part, int waves,
+                                                     float* __restrict__ p, float* __restrict__ mom1,
+                                                     float* __restrict__ mom2, float* __restrict__ grad,
+                                                     float* __restrict__ loss_out, float lr, float bc1, float bc2) {
+    constexpr int NS = 256 / RED_P;
+    __shared__ float red[256];
+    const int t = threadIdx.x;
+    if (blockIdx.x == gridDim.x - 1) {
+        red[t] = t < waves ? loss_part[t] : 0.f;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) red[t] += red[t + w];
+            __syncthreads();
+        }
+        if (t == 0) loss_out[0] = red[0];
+        return;
     }
-    if (i >= NP) return;
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    int k = 0;
-    for (; k + 3 < waves; k += 4) {
-        s0 += part[(int64_t)k * NP + i];
-        s1 += part[(int64_t)(k + 1) * NP + i];
-        s2 += part[(int64_t)(k + 2) * NP + i];
-        s3 += part[(int64_t)(k + 3) * NP + i];
-    }
-    for (; k < waves; ++k) s0 += part[(int64_t)k * NP + i];
-    const float gsum = (s0 + s1) + (s2 + s3);
+    const int c = t % RED_P, sl = t / RED_P;
+    const int i = blockIdx.x * RED_P + c;
+    float s = 0.f;
+    if (i < NP)
+        for (int k = sl; k < waves; k += NS) s += part[(int64_t)k * NP + i];
+    red[t] = s;
+    __syncthreads();
+    if (sl != 0 || i >= NP) return;
+    float gsum = red[c];
+    for (int q = 1; q < NS; ++q) gsum += red[q * RED_P + c];
     grad[i] = gsum;
     if (lr <= 0.f) return;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -720,8 +733,10 @@ int run_step(sph_mlp* m, const float4* rec, int64_t count, int batch, int64_t no
         bc1 = (float)(1.0 - std::pow(0.9, (double)m->adam_t));
         bc2 = (float)(1.0 - std::pow(0.999, (double)m->adam_t));
     }
-    k_reduce_adam<<<(NP + 255) / 256, 256, 0, m->stream>>>(m->part, m->loss_part, blocks, m->params, m->mom1,
-                                                           m->mom2, m->grad, loss_dst, lr, bc1, bc2);
+    /* one workgroup per RED_P parameters, one more for the loss (blocks <= 256) */
+    k_reduce_adam<<<(NP + RED_P - 1) / RED_P + 1, 256, 0, m->stream>>>(m->part, m->loss_part, blocks, m->params,
+                                                                     m->mom1, m->mom2, m->grad, loss_dst, lr, bc1,
+                                                                     bc2);
     MLP_TRY(hipGetLastError());
     return SPH_OK;
 }

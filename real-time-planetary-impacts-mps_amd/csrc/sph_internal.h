@@ -103,6 +103,7 @@ struct sph_tree_dev {
     float4* mp_far = nullptr;       /* 3 per node */
     float4* mp_aux = nullptr;       /* 3 per node */
     float4* mp_oct = nullptr;       /* 3 per node: octupoles + power[3] (expansion order 3) */
+    float4* mp_clo = nullptr;       /* 1 per node (precision form): the centre's low part, centre = pos + clo */
     float4* loc3 = nullptr;         /* 3 per node: third-order local terms (expansion order 3) */
     float4* loc = nullptr;          /* 4 per node */
     float4* loc_tot = nullptr;      /* 7 per node: a near-field leaf's complete local expansion */
