@@ -95,11 +95,14 @@ def parse():
 
 
 def apply_overrides(prm, args):
-    """--density-lanes, --accel-lanes and --param NAME=VALUE on an sph_params."""
+    """--density-lanes, --accel-lanes and --param NAME=VALUE on an sph_params
+    (lanes given on the command line are used at any particle count)."""
     if args.density_lanes is not None:
         prm.density_lanes = args.density_lanes
+        prm.small_n_lanes = 0
     if args.accel_lanes is not None:
         prm.accel_lanes = args.accel_lanes
+        prm.small_n_lanes = 0
     for kv in args.param:
         name, value = kv.split("=", 1)
         cur = getattr(prm, name)

@@ -171,6 +171,15 @@ typedef struct sph_params {
      * again (0, or more particles: it walks).  The same pairs; only which
      * lane sums which changes. */
     int32_t near_list_cap;
+    /* Every particle active (no slot list) and fewer than this many
+     * particles (0 = off): the density pass takes 32 lanes per particle and
+     * the acceleration pass 16 below 65,536 particles, else 8, in place of
+     * density_lanes and accel_lanes.  With one thread (or 8 lanes) per
+     * particle such a pass leaves most of the GPU's wave slots empty and
+     * lasts as long as its longest particle's chain of dependent loads.  The
+     * variants are those of density_lanes and accel_lanes: the same terms,
+     * tree-ordered sums. */
+    int32_t small_n_lanes;
 } sph_params;
 /* Relative change of h (2 ulp) at which density_stop = 1 stops whatever k. */
 #define SPH_NR_NOISE_FLOOR 2.5e-7f

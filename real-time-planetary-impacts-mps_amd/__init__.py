@@ -67,7 +67,7 @@ class SphParams(ctypes.Structure):
         ("density_stop", ctypes.c_int32),
         ("density_order", ctypes.c_int32), ("list_density_lanes", ctypes.c_int32),
         ("list_accel_lanes", ctypes.c_int32), ("gather_reach", ctypes.c_int32),
-        ("near_list_cap", ctypes.c_int32),
+        ("near_list_cap", ctypes.c_int32), ("small_n_lanes", ctypes.c_int32),
     ]
 
 

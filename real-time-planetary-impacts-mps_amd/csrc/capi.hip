@@ -40,6 +40,7 @@ bool valid_params(const sph_params* p, std::string& why) {
         p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->gather_reach != 0 && p->gather_reach != 1) { why = "gather_reach must be 0 or 1"; return false; }
     if (p->near_list_cap < 0 || p->near_list_cap > 4096) { why = "near_list_cap must be 0..4096"; return false; }
+    if (p->small_n_lanes < 0) { why = "small_n_lanes must be >= 0"; return false; }
     if ((p->density_lanes == 2 || p->density_lanes == 4 || p->accel_lanes == 2 || p->accel_lanes == 4) &&
         p->lane_split) { why = "lane_split needs 8 or more lanes"; return false; }
     if (p->lane_split != 0 && p->lane_split != 1) { why = "lane_split must be 0 or 1"; return false; }

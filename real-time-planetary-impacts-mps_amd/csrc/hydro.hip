@@ -1373,7 +1373,7 @@ int launch_density(sph_ctx* ctx) {
     }
     const int threads = 256;
     const int flat = p.density_flat ? 1 : 0;
-    const int dlanes = a.slots ? p.list_density_lanes : p.density_lanes;
+    const int dlanes = a.slots ? p.list_density_lanes : (ctx->n < p.small_n_lanes ? 32 : p.density_lanes);
     const int dsel = (dlanes * 2 + (p.lane_split ? 1 : 0)) * 2 + flat;
     /* over a slot list the grid is persistent: at most kListBlocks workgroups */
 #define SPH_DENS_C(G, S, F, PPB, C)                                                                      \
@@ -1472,7 +1472,8 @@ int launch_acceleration(sph_ctx* ctx) {
             ctx->n, ctx->pairs_a, ctx->pos, ctx->vel, ctx->m, ctx->s_posm, ctx->s_vel, ctx->h, ctx->f, ctx->P,
             ctx->rho, ctx->c, ctx->B, ctx->s_posh, ctx->s_velb, ctx->s_hyd, mark, epoch);
     SPH_HIP_TRY(hipGetLastError());
-    const int alanes = a.slots ? p.list_accel_lanes : p.accel_lanes;
+    const int alanes = a.slots ? p.list_accel_lanes
+                               : (ctx->n < p.small_n_lanes ? (ctx->n < 65536 ? 16 : 8) : p.accel_lanes);
     const int asel = alanes * 2 + (p.lane_split ? 1 : 0);
     /* over a slot list the grid is persistent: at most kListBlocks workgroups */
 #define SPH_ACC(G, S)                                                                                    \

@@ -152,7 +152,8 @@ def test_density_lanes_parity(sph, orc, tables256, impact, lanes, split):
     n = 100000
     ic = _stir(impact(n, 2), 2)
     ic["h"][::97] = 1.0
-    prm, ctx, st = _pair(sph, orc, ic, tables256, density_lanes=lanes, lane_split=split, n_neighbours_estim=64)
+    prm, ctx, st = _pair(sph, orc, ic, tables256, density_lanes=lanes, lane_split=split, n_neighbours_estim=64,
+                         small_n_lanes=0)
     ctx.sort()
     st.sort()
     ctx.density()
@@ -175,7 +176,7 @@ def test_density_stop_rule_parity(sph, orc, tables256, impact, lanes):
     eta = 2.5e-6
     ic = _stir(impact(n, 2), 2)
     prm, ctx, st = _pair(sph, orc, ic, tables256, density_eta=eta, density_stop=1, max_nr_iterations=400,
-                         density_lanes=lanes)
+                         density_lanes=lanes, small_n_lanes=0)
     ctx.sort()
     st.sort()
     ctx.density()
@@ -235,13 +236,14 @@ def test_density_cache_paths(sph, orc, tables256, impact):
 
 # ----------------------------------------------------------- acceleration
 @pytest.mark.parametrize("n,seed,lanes,split", [(20000, 1, 0, 0), (100000, 2, 0, 0), (100000, 2, 2, 0),
-                                                (100000, 2, 4, 0), (100000, 2, 8, 0),
+                                                (100000, 2, 4, 0), (100000, 2, 8, 0), (20000, 1, 16, 0),
+                                                (100000, 2, 16, 0),
                                                 (100000, 2, 64, 0), (100000, 2, 8, 1), (100000, 2, 16, 1),
                                                 (100000, 2, 32, 1), (100000, 2, 64, 1)])
 def test_acceleration_parity(sph, orc, tables256, impact, n, seed, lanes, split):
     """lanes > 0: the group-cooperative pass (cell box split over the lanes)."""
     ic = _stir(impact(n, seed), seed)
-    prm, ctx, st = _pair(sph, orc, ic, tables256, accel_lanes=lanes, lane_split=split)
+    prm, ctx, st = _pair(sph, orc, ic, tables256, accel_lanes=lanes, lane_split=split, small_n_lanes=0)
     st.sort()
     st.density()
     ctx.sort()

@@ -56,14 +56,15 @@ def test_default_params_are_the_reference_values(sph):
     assert p.gravity_g == pytest.approx(6.67e-5) and p.fixes == 0
     assert p.activity_rule == sph.SPH_ACTIVE_AS_WRITTEN and p.max_rungs == 0
     assert p.precision == 0 and p.density_lanes == 8 and p.accel_lanes == 0 and p.lane_split == 0
+    assert p.small_n_lanes == 262144
     assert p.expansion_order == 2 and p.mac_order == 0
 
 
 def test_params_struct_layout_matches_c(sph):
     # 26 four-byte fields, precision, density_lanes, accel_lanes, lane_split, expansion_order, mac_order,
     # density_flat, density_stop, density_order, list_density_lanes, list_accel_lanes, gather_reach,
-    # near_list_cap
-    assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 13)
+    # near_list_cap, small_n_lanes
+    assert ctypes.sizeof(sph.SphParams) == 4 * (26 + 14)
 
 
 # ---------------------------------------------------------------- EOS
@@ -187,7 +188,7 @@ def test_create_rejects_empty_and_invalid_inputs_before_touching_a_device(sph):
         assert not out.value
     view = sph.SphIcView(1, f(one), f(one), f(s1), f(s1), f(s1), f(mat, ctypes.c_int32), f(s1), f(s1))
     for field, bad in (("cell_power", 0), ("max_unchecked_pointers", 1), ("expansion_order", 4), ("density_lanes", 3),
-                       ("density_order", 8), ("density_order", 48), ("density_order", 1024)):
+                       ("density_order", 8), ("density_order", 48), ("density_order", 1024), ("small_n_lanes", -1)):
         p = sph.default_params(**{field: bad})
         assert h.sph_create(ctypes.byref(p), ctypes.byref(view), arr, 2, ctypes.byref(out)) == sph.SPH_E_ARG, field
     assert h.sph_create(ctypes.byref(prm), ctypes.byref(view), arr, 0, ctypes.byref(out)) == sph.SPH_E_ARG
