@@ -444,8 +444,9 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
-              "the multi-GPU decomposition of this path is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): "
+              "the step is built and measured for a single MI355X; its multi-GPU decomposition is a design, "
+              "not a code path", file=sys.stderr)
         sys.exit(2)
     sph = importlib.import_module(PKG)
     if args.learned_precision:
