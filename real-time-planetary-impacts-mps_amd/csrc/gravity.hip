@@ -282,7 +282,8 @@ __device__ void p2m(const UpArgs& a, int id, int np, int tp) {
         const int p = pl[k];
         const float4 x = a.pos[p];
         const float pm = x.w;
-        /* x - px is exact (x and px within a factor 2 of each other) */
+        /* x - px is exact where x and px are within a factor 2 of each other
+         * (every leaf away from the coordinate planes) */
         const float rx = (x.x - px) - clx, ry = (x.y - py) - cly, rz = (x.z - pz) - clz;
         X -= rx * pm;
         Y -= ry * pm;
