@@ -1303,10 +1303,19 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             return;
         }
         if constexpr (PART == 2) load_local_total<P>(a, id, L);
+        /* every lane holds every target's sums (butterfly reductions): lane kk
+         * takes target kk's, so that L2P runs once with nk lanes active rather
+         * than once per target with one lane active */
+        float sgx = T.gx[0], sgy = T.gy[0], sgz = T.gz[0], sgp = T.gp[0];
 #pragma unroll
-        for (int kk = 0; kk < NEAR_K; ++kk)
-            if (lane == kk && kk < nk)
-                leaf_particle<PREC, P, false>(a, L, pl, np, k0 + kk, T.gx[kk], T.gy[kk], T.gz[kk], T.gp[kk]);
+        for (int kk = 1; kk < NEAR_K; ++kk)
+            if (lane == kk) {
+                sgx = T.gx[kk];
+                sgy = T.gy[kk];
+                sgz = T.gz[kk];
+                sgp = T.gp[kk];
+            }
+        if (lane < nk) leaf_particle<PREC, P, false>(a, L, pl, np, k0 + lane, sgx, sgy, sgz, sgp);
     }
 }
 
