@@ -1151,8 +1151,11 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             const int rc = a.t.near_cap > 0 ? a.t.near_cnt[id] : -1;
             if (rc >= 0) {
                 const int32_t* rec = a.t.near_rec + (int64_t)id * a.t.near_cap;
-                for (int e = lane; e < rc; e += GS) near_pair<PREC, NEAR_K>(a, T, nk, a.pos[rec[e]]);
-                goto reduce;
+                /* the recorded source particles, then the leaf's own, as one
+                 * sequence over the lanes */
+                for (int e = lane; e < rc + np; e += GS)
+                    near_pair<PREC, NEAR_K>(a, T, nk, a.pos[e < rc ? rec[e] : pl[e - rc]]);
+                goto reduce_sums;
             }
         }
         {
@@ -1272,12 +1275,12 @@ __device__ void down_near_leaf_q(const DownArgs& a, int id, int4 inf, const Node
             }
         }
         }
-    reduce:
         /* the leaf's own particles as sources, every lane taking every GS-th
          * (a target's pair with itself has r = 0 and adds nothing, as in the
          * intra-leaf P2P of helpers.h:47-81) */
         if constexpr (PART != 1)
             for (int e = lane; e < np; e += GS) near_pair<PREC, NEAR_K>(a, T, nk, a.pos[pl[e]]);
+    reduce_sums:
 #pragma unroll
         for (int kk = 0; kk < NEAR_K; ++kk) {
             T.gx[kk] = group_sum<GS>(T.gx[kk]);
