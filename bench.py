@@ -515,8 +515,9 @@ def main():
             "eos_table": "256x256 Tillotson per material",
             "tree_mode": args.tree,
             "tree_threads": args.tree_threads,
-            "density_lanes": int(prm.density_lanes),
-            "accel_lanes": int(prm.accel_lanes),
+            "density_lanes": int(prm.density_lanes) if args.n >= prm.small_n_lanes else 32,
+            "accel_lanes": int(prm.accel_lanes) if args.n >= prm.small_n_lanes else (16 if args.n < 65536 else 8),
+            "small_n_lanes": int(prm.small_n_lanes),
             "lane_split": int(prm.lane_split),
             "trees_swapped_in_timed_region": trees1 - trees0,
             "last_tree_build_ms": round(build_ms, 2),
@@ -529,8 +530,9 @@ def main():
                                "that overlap, so the passes do not add up to ms_per_step")
         out["density_ms"] = round(per_pass["density"], 4)
         # dominant kernel: the density pass is one kernel
-        kname = (f"k_density_group<{int(prm.density_lanes)}, {int(prm.lane_split)}, {int(prm.density_flat)}, "
-                 "false, false>" if prm.density_lanes > 0 else "k_density")
+        dlanes = int(prm.density_lanes) if args.n >= prm.small_n_lanes else 32
+        kname = (f"k_density_group<{dlanes}, {int(prm.lane_split)}, {int(prm.density_flat)}, "
+                 "false, false>" if dlanes > 0 else "k_density")
         # the reference algorithm's work, counted by one more sort + density
         # pass of the state after the timed steps (untimed: counting is a
         # measurement aid)
@@ -551,7 +553,7 @@ def main():
         flops = density_flops(work)
         # the recorded counters cover the default workload and launch shape only
         default_shape = (args.n == 1_000_000 and args.fixes == 0 and not args.param and not args.headon
-                         and prm.density_lanes == 8 and prm.lane_split == 0 and prm.density_flat == 0)
+                         and dlanes == 8 and prm.lane_split == 0 and prm.density_flat == 0)
         traffic_bytes = profiled_traffic() if default_shape else None
         t = per_pass["density"] / 1000.0
         achieved = flops / t / 1e12
