@@ -103,8 +103,11 @@ typedef struct sph_params {
     /* Precision form (no counterpart in the reference; needs
      * SPH_FIX_NEAR_FIELD): compensated (Neumaier) sums in the M2L, L2P and
      * P2P accumulators, centres of mass summed relative to a member in
-     * P2M/M2M, and the gravitational potential per particle (SPH_F_POTENTIAL)
-     * for the FP64 conservation sums of sph_conservation. */
+     * P2M/M2M and kept in two parts (FP32 rounding + residual) so that every
+     * offset from a node centre (moments, M2M and L2L shifts, M2L separation,
+     * L2P) is taken relative to it, and the gravitational potential per
+     * particle (SPH_F_POTENTIAL) for the FP64 conservation sums of
+     * sph_conservation. */
     int32_t precision;
     /* Lanes per particle in the density pass: 0 = one thread per particle
      * (the reference's summation order), 2/4/8/16/32/64 = the particle's cell
